@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Operator-apply benchmark on one MI355X.
+
+Workload: 3D sparse grid, lev 7, deg 3 (27-value blocks), 3 terms (one
+deg x deg-block operator per dimension and term), the maximal connect_nD pair
+list (P = 1,903,000 element pairs over N = 1696 elements), x ~ N(0,1) seeded.
+A "step" is one full operator application y = A x.  The plain FP64 kernel and
+the mixed-precision (FP32) kernels are timed on the same resident operator;
+`value` is the faster mixed-precision variant's applies/s.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "dg-sparsegrid_amd"))
+
+METRIC = "kronmult applies/s, mixed-prec batched vs FP64 plain (1 GPU), L2err≤1e-6; %HBM"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def build_operator(seed=2):
+    """cfg2-sized operator in the reference's conventions (old hash + connect_nD)."""
+    from kmx import coeffs, grid
+
+    nd, lev, deg = 3, 7, 3
+    table = grid.hash_table_nd([lev] * nd, "SG")
+    rp, cols = grid.connect_nd(table, lev, lev, "SG")
+    mats, source = coeffs.operator_mats("continuity3", nd, lev, deg)
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(table.num_elem * deg ** nd)
+    return dict(nd=nd, lev=lev, deg=deg, table=table, rp=rp, cols=cols, mats=mats, x=x, source=source)
+
+
+def time_apply(plan, xd, yd, prec, warmup, steps):
+    import torch
+
+    for _ in range(warmup):
+        plan.apply(xd, yd, prec=prec)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()  # same (current) stream the kernels are launched on
+    for _ in range(steps):
+        plan.apply(xd, yd, prec=prec)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps  # ms per apply
+
+
+def work_model(op, P, prec):
+    """Algorithmic flops and streamed bytes per apply (method-1 chain, one
+    x_j block + T*d coefficient blocks + indices per pair, y written once)."""
+    nd, deg = op["nd"], op["deg"]
+    T = len(op["mats"])
+    N = op["table"].num_elem
+    B = deg ** nd
+    flops = P * T * 2 * nd * deg ** (nd + 1)
+    sA = 8 if prec == "fp64" else 4
+    streamed = P * (B * 8 + T * nd * deg * deg * sA + 4 * (1 + nd)) + N * B * 8
+    return flops, streamed
+
+
+def cpu_baseline(op, budget_s=12.0):
+    """The oracle (C++ FP64 restatement of apply_A.m) on the host cores."""
+    import oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    inp = oracle.ApplyInputs(op["mats"], op["table"].idx1d, op["deg"], op["rp"], op["cols"])
+    y = oracle.apply_A(inp, op["x"], nthreads=threads)  # warm-up + reference result
+    n, t0 = 0, time.perf_counter()
+    while True:
+        oracle.apply_A(inp, op["x"], nthreads=threads)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 20:
+            break
+    dt = (time.perf_counter() - t0) / n
+    return y, {"value": 1.0 / dt, "unit": "applies/s", "cores": threads, "kind": "port",
+               "sample": f"{n} full applies of the same workload (oracle/kmx_oracle.cpp, OpenMP over element rows)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {a.gpus} requested); the operator apply is not "
+              "sharded across GPUs in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import kmx
+
+    torch.cuda.set_device(0)
+    op = build_operator()
+    plan = kmx.Plan(op["mats"], op["table"].idx1d, op["deg"], op["rp"], op["cols"], index_base=1, fp32=True)
+    xd = torch.from_numpy(op["x"]).cuda()
+    yd = torch.empty_like(xd)
+
+    res = {}
+    for prec in ("fp64", "fp32", "fp32_grouped"):
+        res[prec] = time_apply(plan, xd, yd, prec, a.warmup, a.steps)
+    y64 = plan.apply(xd, prec="fp64").cpu().numpy()
+    err = {}
+    for prec in ("fp32", "fp32_grouped"):
+        yp = plan.apply(xd, prec=prec).cpu().numpy()
+        err[prec] = float(np.linalg.norm(yp - y64) / np.linalg.norm(y64))
+    best = min(("fp32", "fp32_grouped"), key=lambda p: res[p])
+    ms = res[best]
+    value = 1000.0 / ms
+
+    flops, _ = work_model(op, plan.P, best)
+    flops64, streamed64 = work_model(op, plan.P, "fp64")
+    _, streamed_best = work_model(op, plan.P, best)
+    achieved = flops / (ms * 1e-3) / 1e12
+    roofline = {
+        "bound": "mfma",
+        "achieved": round(achieved, 3),
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+        "traffic": None,
+        "kernel": f"k_apply_{'grouped' if best == 'fp32_grouped' else 'reg'} (fp32, d=3, deg=3)",
+        "streamed_model_GBs": round(streamed_best / (ms * 1e-3) / 1e9, 1),
+        "streamed_model_frac_hbm": round(streamed_best / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "applies/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic: x ~ N(0,1) seed 2; 1D operators {op['source']}",
+        "config": {"workload": "3D SG lev7 deg3, 3 terms, connect_nD pairs (P=%d, N=%d)" % (plan.P, plan.N),
+                   "variant": best, "block": plan.B},
+        "fp64_plain": {"applies_per_s": round(1000.0 / res["fp64"], 2), "ms_per_apply": round(res["fp64"], 5),
+                       "achieved_TFLOPs": round(flops64 / (res["fp64"] * 1e-3) / 1e12, 3),
+                       "frac_fp64_peak": round(flops64 / (res["fp64"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+                       "streamed_model_GBs": round(streamed64 / (res["fp64"] * 1e-3) / 1e9, 1)},
+        "mixed": {p: {"applies_per_s": round(1000.0 / res[p], 2), "rel_err_vs_fp64": err[p]}
+                  for p in ("fp32", "fp32_grouped")},
+        "speedup_vs_fp64": round(res["fp64"] / ms, 3),
+        "roofline": roofline,
+    }
+    if not a.no_cpu:
+        yo, cb = cpu_baseline(op)
+        out["cpu_baseline"] = cb
+        out["fp64_rel_err_vs_oracle"] = float(np.linalg.norm(y64 - yo) / np.linalg.norm(yo))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

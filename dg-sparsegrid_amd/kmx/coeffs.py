@@ -1,0 +1,463 @@
+"""Discontinuous-Galerkin multiwavelet 1D operators and projections: the CPU
+input provider of the operator apply (host side, FP64, numpy).
+
+Restates, in the reference's order of operations:
+  lgwt.m (Gauss-Legendre rule), DG_loc/lin_legendre.m, lin_dlegendre.m,
+  two_scale/OperatorTwoScale_wavelet2.m (FMWT), coeff_matrix.m (mass / div /
+  grad with Lax-Friedrichs flux and P/D/N boundary handling, rotated to
+  wavelet space), pterm_coeff_matrix.m (M \\ L), sd_term_coeff_matrix.m
+  (chaining), get_coeff_mats.m, forward_wavelet_transform.m,
+  combine_dimensions_D.m, md_eval_function.m.
+The PDE specifications mirror pdes/continuity{1,2,3}.m and pdes/diffusion2.m;
+`diffusion_nd` and `advection_diffusion_varcoeff` generalise them to the
+6D LDG diffusion and variable-coefficient cases.
+"""
+import functools
+import math
+import os
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional
+
+import numpy as np
+
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "two_scale.npz")
+
+
+# ------------------------------------------------------------- quadrature
+@functools.lru_cache(maxsize=None)
+def lgwt(N, a, b):
+    """Quadrature/lgwt.m:16-59 -- N Gauss-Legendre nodes/weights on [a, b]."""
+    N = N - 1
+    N1, N2 = N + 1, N + 2
+    xu = np.linspace(-1.0, 1.0, N1)
+    y = np.cos((2 * np.arange(0, N + 1) + 1) * np.pi / (2 * N + 2)) + (0.27 / N1) * np.sin(np.pi * xu * N / N2)
+    L = np.zeros((N1, N2))
+    y0 = 2.0
+    Lp = None
+    for _ in range(200):
+        if not np.max(np.abs(y - y0)) > np.finfo(float).eps:
+            break
+        L[:, 0] = 1.0
+        L[:, 1] = y
+        for k in range(2, N1 + 1):
+            L[:, k] = ((2 * k - 1) * y * L[:, k - 1] - (k - 1) * L[:, k - 2]) / k
+        Lp = N1 * (L[:, N1 - 1] - y * L[:, N2 - 1]) / (1 - y ** 2)
+        y0 = y
+        y = y0 - L[:, N2 - 1] / Lp
+    x = (a * (1 - y) + b * (1 + y)) / 2
+    w = (b - a) / ((1 - y ** 2) * Lp ** 2)
+    return x[::-1].copy(), w[::-1].copy()
+
+
+def default_quad_number(deg):
+    """Quadrature/default_quad_number.m."""
+    return max(deg + 1, 10)
+
+
+def _legendre_table(x, k):
+    """P_0..P_{k-1} at x (three-term recurrence)."""
+    x = np.asarray(x, dtype=np.float64)
+    P = np.zeros(x.shape + (max(k, 1),))
+    P[..., 0] = 1.0
+    if k > 1:
+        P[..., 1] = x
+    for n in range(2, k):
+        P[..., n] = ((2 * n - 1) * x * P[..., n - 1] - (n - 1) * P[..., n - 2]) / n
+    return P[..., :k]
+
+
+def lin_legendre(x, k):
+    """DG_loc/lin_legendre.m -- orthonormal Legendre on [-1,1], zero outside."""
+    x = np.atleast_1d(np.asarray(x, dtype=np.float64))
+    v = _legendre_table(x, k) * np.sqrt(2 * np.arange(k) + 1.0)
+    v[(x > 1) | (x < -1), :] = 0.0
+    return v
+
+
+def lin_dlegendre(x, k):
+    """DG_loc/lin_dlegendre.m -- derivative of lin_legendre (dp_n = n P_{n-1} + x dp_{n-1})."""
+    x = np.atleast_1d(np.asarray(x, dtype=np.float64))
+    P = _legendre_table(x, max(k, 1))
+    dP = np.zeros(x.shape + (k,))
+    for n in range(1, k):
+        dP[..., n] = n * P[..., n - 1] + x * dP[..., n - 1]
+    return dP * np.sqrt(2 * np.arange(k) + 1.0)
+
+
+# ------------------------------------------------------------------- FMWT
+@functools.lru_cache(maxsize=None)
+def two_scale_filters(deg):
+    z = np.load(_DATA)
+    return z[f"H0_{deg}"].copy(), z[f"G0_{deg}"].copy()
+
+
+@functools.lru_cache(maxsize=32)
+def fmwt(deg, lev):
+    """two_scale/OperatorTwoScale_wavelet2.m:1-144 -- dense forward multiwavelet
+    transform (deg*2^lev square)."""
+    if lev == 0:
+        return np.eye(deg)
+    H0, G0 = two_scale_filters(deg)
+    tol = 1e4 * np.finfo(float).eps
+    H0 = np.where(np.abs(H0) < tol, 0.0, H0)
+    G0 = np.where(np.abs(G0) < tol, 0.0, G0)
+    i = np.arange(deg)
+    sgn = (-1.0) ** (i[:, None] + i[None, :])
+    H1 = sgn * H0
+    G1 = ((-1.0) ** deg) * sgn * G0
+    n = deg * 2 ** lev
+    Gmat = np.zeros((deg, n))
+    Hmat = np.zeros((deg, n))
+    Hmat[:, :deg] = np.eye(deg)
+    blocks = {}
+    for j in range(lev - 1, -1, -1):
+        isize = n // (2 ** j)
+        half = isize // 2
+        Htmp = Hmat[:, :half].copy()
+        Gmat[:, :half] = G0 @ Htmp
+        Gmat[:, half:isize] = G1 @ Htmp
+        Hmat[:, :half] = H0 @ Htmp
+        Hmat[:, half:isize] = H1 @ Htmp
+        blocks[2 * j + 2] = Gmat[:, :isize].copy()
+        blocks[2 * j + 1] = Hmat[:, : (n if j == 0 else isize)].copy()
+    F = np.zeros((n, n))
+    F[:deg, :] = blocks[1]
+    irow = deg
+    for j in range(lev):
+        ncells = 2 ** j
+        isize = n // ncells
+        Fm = blocks[2 + 2 * j]
+        for c in range(ncells):
+            F[irow + c * deg: irow + (c + 1) * deg, c * isize:(c + 1) * isize] = Fm
+        irow += deg * ncells
+    return F
+
+
+# ------------------------------------------------------------ term classes
+def _one(x, p=None, t=None, dat=None):
+    return np.zeros_like(np.asarray(x, dtype=np.float64)) + 1.0
+
+
+@dataclass
+class PTerm:
+    """PARTIAL_SD_TERM.m / MASS.m / DIV.m / GRAD.m (DIV_OR_GRAD.m)."""
+    type: str
+    g: Callable = _one
+    LHS_mass_g: Callable = _one
+    dV: Callable = _one
+    LF: float = 0.0
+    BCL: str = "N"
+    BCR: str = "N"
+    IBCL: str = "N"
+    IBCR: str = "N"
+    BCL_fList: Optional[list] = None
+    BCR_fList: Optional[list] = None
+
+
+def MASS(g=None, LHS_mass_g=None, dV=None):
+    return PTerm("mass", g or _one, LHS_mass_g or _one, dV or _one)
+
+
+def DIV(g=None, LF=0.0, BCL="N", BCR="N", dV=None, LHS_mass_g=None, BCL_fList=None, BCR_fList=None):
+    """DIV.m: IBC = BC."""
+    return PTerm("div", g or _one, LHS_mass_g or _one, dV or _one, LF, BCL, BCR, BCL, BCR, BCL_fList, BCR_fList)
+
+
+def GRAD(g=None, LF=0.0, BCL="N", BCR="N", dV=None, LHS_mass_g=None, BCL_fList=None, BCR_fList=None):
+    """GRAD.m: D and N swap for the internal BCs, LF changes sign."""
+    sw = {"D": "N", "N": "D"}
+    return PTerm("grad", g or _one, LHS_mass_g or _one, dV or _one, -LF, BCL, BCR, sw.get(BCL, BCL),
+                 sw.get(BCR, BCR), BCL_fList, BCR_fList)
+
+
+@dataclass
+class Dimension:
+    min: float
+    max: float
+    lev: int = 3
+    moment_dV: Callable = _one
+    name: str = "x"
+
+
+@dataclass
+class PDE:
+    dims: List[Dimension]
+    terms: List[List[List[PTerm]]]  # terms[t][d] = chain of pterms (SD_TERM)
+    sources: list = field(default_factory=list)  # each: [f_1(x,p,t), ..., f_d, g(t)]
+    initial_conditions: list = field(default_factory=list)
+    solutions: list = field(default_factory=list)
+    params: dict = field(default_factory=dict)
+    set_dt: Optional[Callable] = None
+    name: str = ""
+
+    @property
+    def ndim(self):
+        return len(self.dims)
+
+
+# ---------------------------------------------------------- coeff_matrix.m
+def coeff_matrix(deg, t, dim, pterm, params, lev):
+    """coeff_matrix.m:1-374.  Returns (wavelet-space matrix, real-space matrix)."""
+    assert lev >= dim.lev
+    N = 2 ** lev
+    h = (dim.max - dim.min) / N
+    dof = deg * N
+    qx, qw = lgwt(default_quad_number(deg), -1, 1)
+    p_L = lin_legendre(-1.0, deg)[0] / np.sqrt(h)
+    p_R = lin_legendre(+1.0, deg)[0] / np.sqrt(h)
+    p_val = lin_legendre(qx, deg) / np.sqrt(h)
+    Dp_val = lin_dlegendre(qx, deg) / np.sqrt(h) * 2 / h
+    Jac = h / 2
+    F = fmwt(deg, lev)
+    dat_R = F.T @ np.ones(dof)  # pterm.dat empty -> ones, to real space (:99-103)
+    mass = np.zeros((dof, dof))
+    div = np.zeros((dof, dof))
+    is_dg = pterm.type in ("div", "grad")
+    FluxVal = pterm.LF
+    o = np.outer
+    for i in range(N):
+        xL = dim.min + i * h
+        xR = xL + h
+        c = np.arange(deg * i, deg * (i + 1))
+        qxi = ((qx + 1) / 2 + i) * h + dim.min
+        dq = p_val @ dat_R[c]
+        G1 = pterm.g(qxi, params, t, dq) * pterm.dV(qxi, params, t)
+        val_mass = p_val.T @ (G1[:, None] * p_val * qw[:, None]) * Jac
+        val_div = -Dp_val.T @ (G1[:, None] * p_val * qw[:, None]) * Jac
+        if np.isnan(np.linalg.norm(G1)):
+            raise FloatingPointError("coeff_matrix: NaN coefficient")
+        mass[np.ix_(c, c)] += val_mass
+        div[np.ix_(c, c)] += val_div
+        if not is_dg:
+            continue
+        FCL = float(np.ravel(pterm.g(np.array([xL]), params, t, dq) * pterm.dV(np.array([xL]), params, t))[0])
+        FCR = float(np.ravel(pterm.g(np.array([xR]), params, t, dq) * pterm.dV(np.array([xR]), params, t))[0])
+        Z = np.zeros((deg, deg))
+        blocks = [
+            o(-p_L, p_R) * FCL / 2 + FluxVal * abs(FCL) / 2 * o(-p_L, p_R),
+            o(-p_L, p_L) * FCL / 2 + FluxVal * abs(FCL) / 2 * o(-p_L, -p_L),
+            o(p_R, p_R) * FCR / 2 + FluxVal * abs(FCR) / 2 * o(p_R, p_R),
+            o(p_R, p_L) * FCR / 2 + FluxVal * abs(FCR) / 2 * o(p_R, -p_L),
+        ]
+        if pterm.IBCL == "D" and i == 0:
+            blocks[0] = Z.copy()
+            blocks[1] = Z.copy()
+        if pterm.IBCR == "D" and i == N - 1:
+            blocks[2] = Z.copy()
+            blocks[3] = Z.copy()
+        if pterm.IBCL == "N" and i == 0:
+            blocks[0] = Z.copy()
+            blocks[1] = o(-p_L, p_L) * FCL
+        if pterm.IBCR == "N" and i == N - 1:
+            blocks[2] = o(p_R, p_R) * FCR
+            blocks[3] = Z.copy()
+        cols = [c - deg, c, c, c + deg]
+        use = [0, 1, 2, 3]
+        if i == 0:
+            use = [1, 2, 3]
+        elif i == N - 1:
+            use = [0, 1, 2]
+        if pterm.IBCL == "P" or pterm.IBCR == "P":
+            if lev >= 1:
+                if i == 0:
+                    cols[0] = np.arange(deg * (N - 1), deg * N)
+                if i == N - 1:
+                    cols[3] = np.arange(0, deg)
+            else:
+                cols = [c, c, c, c]
+            use = [0, 1, 2, 3]
+        for u in use:
+            div[np.ix_(c, cols[u])] += blocks[u]
+    grad = -div.T
+    mats = {"mass": mass, "div": div, "grad": grad}
+    unrot = mats[pterm.type]
+    return F @ unrot @ F.T, unrot
+
+
+def pterm_coeff_matrix(deg, t, dim, pterm, params, lev):
+    """pterm_coeff_matrix.m:1-26: the pterm's RHS matrix L moved to the RHS with
+    its own LHS mass matrix M: mat = M \\ L."""
+    L, L_un = coeff_matrix(deg, t, dim, pterm, params, lev)
+    M, M_un = coeff_matrix(deg, t, dim, MASS(pterm.LHS_mass_g, None, dim.moment_dV), params, lev)
+    return np.linalg.solve(M, L), np.linalg.solve(M_un, L_un), M
+
+
+def sd_term_coeff_matrix(deg, t, dim, pterms, params, lev):
+    """sd_term_coeff_matrix.m:1-27: chain the pterm matrices."""
+    mat = None
+    for pt in pterms:
+        m, _, _ = pterm_coeff_matrix(deg, t, dim, pt, params, lev)
+        mat = m if mat is None else mat @ m
+    return mat
+
+
+def get_coeff_mats(pde, deg, t=0.0, max_lev=None):
+    """get_coeff_mats.m:29-53 with max_lev_coeffs: build at max_lev when given."""
+    out = []
+    for tm in pde.terms:
+        row = []
+        for d, dim in enumerate(pde.dims):
+            lev = dim.lev if max_lev is None else max_lev
+            row.append(sd_term_coeff_matrix(deg, t, dim, tm[d], pde.params, lev))
+        out.append(row)
+    return out
+
+
+def restrict(mats, pde, deg):
+    """Top-left deg*2^lev_d block of each matrix built at a larger level (the
+    hierarchical index of a coarse element is the same at every level)."""
+    return [[m[: deg * 2 ** pde.dims[d].lev, : deg * 2 ** pde.dims[d].lev] for d, m in enumerate(tm)] for tm in mats]
+
+
+# ------------------------------------------------------------ projections
+def forward_wavelet_transform(deg, lev, lmin, lmax, foo, dV, params, t):
+    """forward_wavelet_transform.m:1-65."""
+    qx, qw = lgwt(default_quad_number(deg), -1, 1)
+    n = 2 ** lev
+    h = (lmax - lmin) / n
+    p_val = (lin_legendre(qx, deg) / np.sqrt(h)).T
+    f = np.zeros(deg * n)
+    for i in range(n):
+        x = h * (qx / 2 + 0.5 + i) + lmin
+        fx = np.asarray(foo(x, params, t), dtype=np.float64) * np.asarray(dV(x), dtype=np.float64)
+        f[i * deg:(i + 1) * deg] = p_val @ (qw * fx)
+    f = f * h / 2
+    f = fmwt(deg, lev) @ f
+    return f * (np.abs(f) > 1e-12)
+
+
+def combine_dimensions_D(deg, fList, time_multiplier, idx1d):
+    """combine_dimensions_D.m:1-57: per element, krond of the per-dim blocks."""
+    N, nd = idx1d.shape
+    B = deg ** nd
+    out = np.zeros(N * B)
+    for e in range(N):
+        v = np.ones(1)
+        for d in range(nd):
+            I = idx1d[e, d] - 1
+            v = np.kron(v, fList[d][I * deg:(I + 1) * deg])
+        out[e * B:(e + 1) * B] += v * time_multiplier
+    return out
+
+
+def md_eval_function(pde, deg, idx1d, md_funcs, t):
+    """md_eval_function.m:1-50 (unit moment_dV: the dimension mass matrix is
+    the identity up to roundoff and is not applied)."""
+    nd = pde.ndim
+    fval = np.zeros(idx1d.shape[0] * deg ** nd)
+    for md in md_funcs:
+        fl = [forward_wavelet_transform(deg, pde.dims[d].lev, pde.dims[d].min, pde.dims[d].max, md[d],
+                                        lambda x, d=d: pde.dims[d].moment_dV(x), pde.params, t) for d in range(nd)]
+        fval += combine_dimensions_D(deg, fl, md[nd](t), idx1d)
+    return fval
+
+
+# ------------------------------------------------------------- PDE specs
+def _const(c):
+    return lambda x, p=None, t=None, dat=None: np.zeros_like(np.asarray(x, dtype=np.float64)) + c
+
+
+def continuity1(lev=2):
+    """pdes/continuity1.m (1D advection, periodic)."""
+    dims = [Dimension(-1, 1, lev)]
+    terms = [[[DIV(_const(-1), -1, "P", "P")]]]
+    sources = [[lambda x, p, t: np.cos(2 * np.pi * x), lambda t: -2 * np.pi * np.sin(t)],
+               [lambda x, p, t: np.sin(2 * np.pi * x), lambda t: -np.cos(t)]]
+    sol = [[lambda x, p, t: np.cos(2 * np.pi * x), lambda t: np.sin(t)]]
+    return PDE(dims, terms, sources, sol, sol, {}, lambda pde, cfl: (pde.dims[0].max - pde.dims[0].min) / 2 ** pde.dims[0].lev * cfl, "continuity1")
+
+
+def continuity2(lev=3):
+    """pdes/continuity2.m: df/dt = -df/dx - df/dy + s on [-1,1] x [-2,2], periodic."""
+    levs = np.broadcast_to(np.asarray(lev), (2,))
+    dims = [Dimension(-1, 1, int(levs[0]), name="x"), Dimension(-2, 2, int(levs[1]), name="y")]
+    terms = [
+        [[DIV(_const(-1), -1, "P", "P")], [MASS(_const(1))]],
+        [[MASS()], [DIV(_const(-1), -1, "P", "P")]],
+    ]
+    sources = [
+        [lambda x, p, t: np.cos(np.pi * x), lambda y, p, t: np.sin(2 * np.pi * y), lambda t: 2 * np.cos(2 * t)],
+        [lambda x, p, t: np.cos(np.pi * x), lambda y, p, t: np.cos(2 * np.pi * y), lambda t: 2 * np.pi * np.sin(2 * t)],
+        [lambda x, p, t: np.sin(np.pi * x), lambda y, p, t: np.sin(2 * np.pi * y), lambda t: -np.pi * np.sin(2 * t)],
+    ]
+    sol = [[lambda x, p, t: np.cos(np.pi * x), lambda y, p, t: np.sin(2 * np.pi * y), lambda t: np.sin(2 * t)]]
+    set_dt = lambda pde, cfl: (pde.dims[0].max - pde.dims[0].min) / 2 ** pde.dims[0].lev * cfl  # noqa: E731
+    return PDE(dims, terms, sources, sol, sol, {"parameter1": 0, "parameter2": 1}, set_dt, "continuity2")
+
+
+def continuity3(lev=3):
+    """pdes/continuity3.m: three upwind DIV(g=-1, LF=-1, P/P) terms, one per dim,
+    on [-1,1] x [-2,2] x [-3,3]."""
+    levs = np.broadcast_to(np.asarray(lev), (3,))
+    dims = [Dimension(-1, 1, int(levs[0]), name="x"), Dimension(-2, 2, int(levs[1]), name="y"),
+            Dimension(-3, 3, int(levs[2]), name="z")]
+    terms = []
+    for k in range(3):
+        terms.append([[DIV(_const(-1), -1, "P", "P")] if d == k else [MASS()] for d in range(3)])
+    c3 = lambda z, p, t: np.cos(2 * np.pi * z / 3)  # noqa: E731
+    sources = [
+        [lambda x, p, t: np.cos(np.pi * x), lambda y, p, t: np.sin(2 * np.pi * y), c3, lambda t: 2 * np.cos(2 * t)],
+        [lambda x, p, t: np.cos(np.pi * x), lambda y, p, t: np.cos(2 * np.pi * y), c3, lambda t: 2 * np.pi * np.sin(2 * t)],
+        [lambda x, p, t: np.sin(np.pi * x), lambda y, p, t: np.sin(2 * np.pi * y), c3, lambda t: -np.pi * np.sin(2 * t)],
+        [lambda x, p, t: np.cos(np.pi * x), lambda y, p, t: np.sin(2 * np.pi * y),
+         lambda z, p, t: np.sin(2 * np.pi * z / 3), lambda t: -2 / 3 * np.pi * np.sin(2 * t)],
+    ]
+    sol = [[lambda x, p, t: np.cos(np.pi * x), lambda y, p, t: np.sin(2 * np.pi * y), c3, lambda t: np.sin(2 * t)]]
+    zero = lambda x, p, t: np.zeros_like(x)  # noqa: E731
+    ic = [[zero, zero, zero, lambda t: 1.0]]
+    set_dt = lambda pde, cfl: (pde.dims[0].max - pde.dims[0].min) / 2 ** pde.dims[0].lev * cfl  # noqa: E731
+    return PDE(dims, terms, sources, ic, sol, {"parameter1": 0, "parameter2": 1}, set_dt, "continuity3")
+
+
+def diffusion_nd(nd=2, lev=3):
+    """pdes/diffusion2.m generalised to nd dims: per dim the LDG chain
+    DIV(g=1, LF=+1, N/N) * GRAD(g=1, LF=-1, D/D), MASS*MASS in the other dims,
+    on [0,1]^nd; solution prod cos(pi x_i) exp(-nd pi^2 t)."""
+    levs = np.broadcast_to(np.asarray(lev), (nd,))
+    dims = [Dimension(0, 1, int(levs[d]), name=f"x{d + 1}") for d in range(nd)]
+    terms = []
+    for k in range(nd):
+        terms.append([[DIV(_const(1), +1, "N", "N"), GRAD(_const(1), -1, "D", "D")] if d == k
+                      else [MASS(_const(1)), MASS(_const(1))] for d in range(nd)])
+    cosf = lambda x, p, t: np.cos(np.pi * x)  # noqa: E731
+    sol = [[cosf] * nd + [lambda t: np.exp(-nd * np.pi ** 2 * t)]]
+    set_dt = lambda pde, cfl: cfl * (1.0 / 2 ** pde.dims[0].lev) ** 2  # noqa: E731
+    return PDE(dims, terms, [], sol, sol, {"parameter1": 0, "parameter2": 1}, set_dt, f"diffusion{nd}")
+
+
+def advection_diffusion_varcoeff(nd=3, lev=3):
+    """Variable-coefficient advection-diffusion: per dim an upwind
+    DIV(g_i, LF=-1, P/P) term and an LDG diffusion chain, with
+    g_i(x) = 1 + 0.5 sin(pi x_i) in the DIV/GRAD pterms (coeff_matrix.m:146)."""
+    levs = np.broadcast_to(np.asarray(lev), (nd,))
+    dims = [Dimension(-1, 1, int(levs[d]), name=f"x{d + 1}") for d in range(nd)]
+    gvar = lambda x, p=None, t=None, dat=None: 1.0 + 0.5 * np.sin(np.pi * np.asarray(x, dtype=np.float64))  # noqa: E731
+    gneg = lambda x, p=None, t=None, dat=None: -gvar(x)  # noqa: E731
+    terms = []
+    for k in range(nd):
+        terms.append([[DIV(gneg, -1, "P", "P")] if d == k else [MASS()] for d in range(nd)])
+    for k in range(nd):
+        terms.append([[DIV(gvar, +1, "P", "P"), GRAD(_const(1), -1, "P", "P")] if d == k else [MASS(), MASS()]
+                      for d in range(nd)])
+    bump = lambda x, p, t: np.exp(-4 * np.asarray(x) ** 2)  # noqa: E731
+    ic = [[bump] * nd + [lambda t: 1.0]]
+    set_dt = lambda pde, cfl: (pde.dims[0].max - pde.dims[0].min) / 2 ** pde.dims[0].lev * cfl  # noqa: E731
+    return PDE(dims, terms, [], ic, [], {}, set_dt, f"advdiff{nd}_varcoeff")
+
+
+PDES = {"continuity1": continuity1, "continuity2": continuity2, "continuity3": continuity3}
+
+
+def operator_mats(name, nd, lev, deg):
+    """1D matrices of a named PDE at level `lev` (bench / harness helper).
+    Returns (mats[t][d], description)."""
+    if name == "continuity3":
+        pde = continuity3(lev)
+    elif name == "continuity2":
+        pde = continuity2(lev)
+    elif name.startswith("diffusion"):
+        pde = diffusion_nd(nd, lev)
+    else:
+        raise KeyError(name)
+    mats = get_coeff_mats(pde, deg)
+    return mats, f"{pde.name} DG (deg {deg}, lev {lev}, LF upwind, periodic), wavelet space"

@@ -1,0 +1,163 @@
+/*
+ * kmx.h -- C ABI of the MI355X (gfx950) operator-application library.
+ *
+ * Drop-in for ASGarD-MATLAB's (project-asgard/DG-SparseGrid) operator
+ * application: apply_A.m's explicit branch, kron_multd.m / Kronmult/kronmult
+ * {1..6}.m and Kronmult/gemm_vbatched.m.  A thin MATLAB MEX gateway, Python
+ * ctypes or any C caller binds these symbols; no C++ or HIP types appear in
+ * the signatures (streams are passed as void*, i.e. a hipStream_t or NULL for
+ * the default stream).
+ *
+ * Conventions follow the reference: 1D coefficient matrices are dense,
+ * column-major, FP64 (pde.terms{t}.terms_1D{d}.mat); the state vector f is
+ * element-major with deg^d values per element, dim 1 slowest and dim d fastest
+ * inside a block (krond order, apply_A.m:139,185); element 1D indices are
+ * A_data.element_local_index_D (global_matrix.m:44-70); the pair list is the
+ * connectivity cell array of connect_nD.m flattened to CSR, or "all pairs"
+ * (apply_A.m:104-108,126-130).
+ *
+ * Every entry returns 0 (KMX_OK) or a negative KMX_E_* code; kmx_last_error()
+ * gives the calling thread's message.  Nothing throws across this boundary.
+ */
+#ifndef KMX_H
+#define KMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (replace the reference's error()/assert, kron_multd.m:7-12,24-28) */
+#define KMX_OK 0
+#define KMX_E_ARG (-1)         /* bad argument (null pointer, out-of-range value) */
+#define KMX_E_SHAPE (-2)       /* inconsistent sizes (kron_multd.m:24-28, kronmult2.m:17-22) */
+#define KMX_E_HIP (-3)         /* a HIP runtime call failed */
+#define KMX_E_NOMEM (-4)       /* device allocation failed */
+#define KMX_E_UNSUPPORTED (-5) /* valid request this build does not handle */
+
+/* ---- precision of the contractions (x and y are always FP64 in memory) */
+#define KMX_PREC_FP64 0       /* plain FP64, the reference's arithmetic (parity <= 1e-13 rel.) */
+#define KMX_PREC_FP32 1       /* FP32 operands and accumulation, same pair-by-pair algorithm */
+#define KMX_PREC_FP32_GROUPED 2 /* FP32, pairs sharing all but the last 1D index are summed
+                                   before the outer contractions (same sum, fewer flops) */
+
+/* ---- plan flags */
+#define KMX_PLAN_FP32 1u    /* also keep FP32 copies of the coefficient blocks */
+#define KMX_PLAN_ADJOINT 2u /* also keep the transposed pair list / blocks (kmx_apply_transpose) */
+
+typedef struct kmx_plan kmx_plan;
+
+/* Operator description.  All pointers are HOST pointers; kmx_plan_create copies
+ * (and re-lays out) everything to the device, so the caller may free them after
+ * the call.  Replaces the arguments apply_A(pde,opts,A_data,f,deg,...) reads
+ * (apply_A.m:1): pde.terms{t}.terms_1D{d}.mat, A_data.element_local_index_D,
+ * pde.connectivity / opts.use_connectivity, deg. */
+typedef struct {
+    int ndim;                  /* d, 1..6                                         */
+    int deg;                   /* block edge (opts.deg), 1..8                     */
+    int num_elem;              /* N = numel(A_data.element_global_row_index)      */
+    int num_terms;             /* T terms applied (after the imex filter, :23-30) */
+    const int* mat_rows;       /* [ndim] n_d = deg*2^L_d (square 1D matrices)     */
+    const double* const* mats; /* [T*ndim] column-major n_d x n_d; mats[t*ndim+d] */
+    const int* elem_idx1d;     /* [N*ndim] 1D index of element e in dim d, [e*ndim+d] */
+    const int64_t* row_ptr;    /* [N+1] CSR offsets (0-based); NULL => all N^2 pairs */
+    const int* cols;           /* [row_ptr[N]] connected element ids per row       */
+    int index_base;            /* 1: elem_idx1d and cols are 1-based (MATLAB); 0: C */
+    unsigned flags;            /* KMX_PLAN_* */
+} kmx_plan_desc;
+
+/* Build a resident plan (coefficient blocks, element table, pair list stay on
+ * the current HIP device until kmx_plan_destroy). */
+int kmx_plan_create(const kmx_plan_desc* desc, kmx_plan** plan);
+
+/* Replace one 1D matrix (term t, dim d; 0-based) -- the time-dependent
+ * coefficient refresh of get_coeff_mats(...,TD=1) (get_coeff_mats.m:29-53).
+ * Host pointer, column-major n_d x n_d.  Stream-ordered on `stream`. */
+int kmx_plan_update_term(kmx_plan* plan, int term, int dim, const double* mat, void* stream);
+
+/* Scale term t's contribution by alpha (default 1).  Used for per-term
+ * coefficient scalings (batched instances, sensitivities). */
+int kmx_plan_set_term_scale(kmx_plan* plan, int term, double alpha);
+
+int kmx_plan_destroy(kmx_plan* plan);
+
+/* num_pairs = P, block = deg^d, dof = N*deg^d. */
+int kmx_plan_info(const kmx_plan* plan, int64_t* num_pairs, int* block, int64_t* dof);
+
+/* y = A x  (apply_A.m:100-237, explicit branch; y is overwritten, matching the
+ * zero-initialised ftmp of apply_A.m:11).  x, y: DEVICE pointers, N*deg^d FP64.
+ * prec: KMX_PREC_*.  Asynchronous on `stream`. */
+int kmx_apply(kmx_plan* plan, const double* x, double* y, int prec, void* stream);
+
+/* y = A^T x over the same pair list and terms (adjoint; no counterpart in the
+ * reference, which has no gradient code).  Needs KMX_PLAN_ADJOINT. */
+int kmx_apply_transpose(kmx_plan* plan, const double* x, double* y, int prec, void* stream);
+
+/* Host-pointer convenience for callers without device memory (a MEX gateway):
+ * copies x in, applies, copies y out, synchronises. */
+int kmx_apply_host(kmx_plan* plan, const double* x_host, double* y_host, int prec);
+
+/* ---- batched instances: one persistent launch over several independent plans
+ * (many PDE instances / adapted grids of uneven size, levels and deg).  xs[i],
+ * ys[i] are device pointers for plan i; the pointer arrays are host arrays.
+ * Any mix of (ndim, deg) is accepted.  No counterpart in the reference (its
+ * batching, Kronmult/kronmult{k}_batch.m, batches the GEMMs of ONE apply). */
+typedef struct kmx_batch kmx_batch;
+int kmx_batch_create(kmx_plan* const* plans, int count, kmx_batch** batch);
+int kmx_batch_apply(kmx_batch* batch, const double* const* xs, double* const* ys, int prec, void* stream);
+int kmx_batch_destroy(kmx_batch* batch);
+
+/* Y = kron(A1,...,Ak) X for k = 1..6 (kron_multd.m:30-42, Kronmult/kronmult{k}.m).
+ * A[i]: DEVICE pointer, column-major nrow[i] x ncol[i] (rectangular allowed);
+ * X: DEVICE, prod(ncol) x nvec column-major; Y: DEVICE, prod(nrow) x nvec.
+ * nrow/ncol/A are host arrays of length k. */
+int kmx_kronmult(int k, const int* nrow, const int* ncol, const double* const* A, const double* X, int nvec,
+                 double* Y, void* stream);
+
+/* Variable-size batched GEMM (Kronmult/gemm_vbatched.m:1-38, magmablas-style):
+ * C_i = alpha*op(A_i)*op(B_i) + beta*C_i with op from transA/transB ('N'/'T'),
+ * m_i x k_i times k_i x n_i.  k is taken from k[] (the reference reads nlist at
+ * gemm_vbatched.m:25; identical for its square use).  Size/ld arrays and the
+ * pointer arrays are HOST arrays; the pointers address DEVICE memory. */
+int kmx_gemm_vbatched(char transA, char transB, const int* m, const int* n, const int* k, double alpha,
+                      const double* const* A, const int* lda, const double* const* B, const int* ldb, double beta,
+                      double* const* C, const int* ldc, int count, void* stream);
+
+/* One GEMM descriptor of kronmult{k}_batch's per-stage lists
+ * (Kronmult/kronmult1_batch.m:45-61, kronmult2_batch.m:71-111, kronmult6_batch.m:86-125).
+ * Operands are named by buffer + element offset so that stage s's output
+ * aliases stage s+1's input (the reference's Clist holds copies instead). */
+#define KMX_BUF_X 0      /* the input X                            */
+#define KMX_BUF_Y 1      /* the output Y                           */
+#define KMX_BUF_W0 2     /* workspace ping                         */
+#define KMX_BUF_W1 3     /* workspace pong                         */
+#define KMX_BUF_FACTOR 4 /* + i: factor A_{i+1} (0-based i)       */
+typedef struct {
+    int stage; /* 1..k: batch_list{stage} */
+    char transA, transB;
+    int m, n, k;
+    int lda, ldb, ldc;
+    double alpha, beta;
+    int bufA, bufB, bufC;
+    int64_t offA, offB, offC;
+} kmx_gemm_desc;
+
+/* Emit the descriptor lists kronmult{k}_batch would build for factors of the
+ * given shapes and nvec vectors, in the reference's order (stage-major, then
+ * emission order).  Pass out == NULL to query *count and *work_elems (elements
+ * needed by each of W0 and W1).  Method choice per level follows
+ * flops_kron{k} (Kronmult/flops_kron2.m:22-46). */
+int kmx_kronmult_batch_plan(int k, const int* nrow, const int* ncol, int nvec, kmx_gemm_desc* out, int capacity,
+                            int* count, int64_t* work_elems);
+
+const char* kmx_last_error(void);
+int kmx_sync(void* stream);
+/* Library version string. */
+const char* kmx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KMX_H */

@@ -1,0 +1,269 @@
+"""GPU parity of the HIP operator apply against the CPU oracle (apply_A.m's
+explicit branch), through the C ABI.
+
+Bars: plain FP64 within 1e-13 relative (2-norm) of the oracle; FP32 variants
+within 5e-6 relative (FP32 unit roundoff 6e-8 times the accumulation depth).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-13
+TOL32 = 5e-6
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def make_case(nd, lev, deg, T, conn, seed, scale_mass=False):
+    """Random FP64 operator inputs in the reference's layout (1-based indices)."""
+    rng = np.random.default_rng(seed)
+    if conn:
+        h = O.hash_table_nd(nd, lev, "SG")
+        rp, cols = O.connect_nd(nd, lev, "SG", lev, lev)
+    else:
+        h = O.hash_table_sparse_nd([lev] * nd, max(lev, 1), "SG")
+        rp = cols = None
+    n = deg * 2 ** lev
+    mats = [[rng.standard_normal((n, n)) for _ in range(nd)] for _ in range(T)]
+    x = rng.standard_normal(h["idx1d"].shape[0] * deg ** nd)
+    return mats, h["idx1d"], rp, cols, x
+
+
+def run(kmx, mats, idx, deg, rp, cols, x, prec, adjoint=False):
+    import torch
+
+    plan = kmx.Plan(mats, idx, deg, rp, cols, index_base=1, fp32=True, adjoint=adjoint)
+    xd = torch.from_numpy(x).cuda()
+    y = plan.apply(xd, prec=prec)
+    torch.cuda.synchronize()
+    return plan, y.cpu().numpy()
+
+
+def oracle_y(mats, idx, deg, rp, cols, x, threads=8):
+    inp = O.ApplyInputs(mats, idx, deg, rp, cols)
+    return O.apply_A(inp, x, nthreads=threads)
+
+
+# cfg1: 2D continuity-shaped operator, SG lev 3, deg 2, two terms; all pairs
+# (new hash, the reference default) and connect_nD (old hash)
+@pytest.mark.parametrize("conn", [False, True])
+def test_cfg1_shape_fp64(kmx_gpu, conn):
+    mats, idx, rp, cols, x = make_case(2, 3, 2, 2, conn, seed=1)
+    _, y = run(kmx_gpu, mats, idx, 2, rp, cols, x, "fp64")
+    yo = oracle_y(mats, idx, 2, rp, cols, x)
+    assert rel(y, yo) <= TOL64
+
+
+@pytest.mark.parametrize(
+    "nd,lev,deg,T,conn",
+    [
+        (1, 4, 2, 2, False), (1, 3, 5, 1, False), (1, 2, 8, 2, False),
+        (2, 3, 1, 2, True), (2, 3, 3, 2, True), (2, 3, 4, 3, False), (2, 2, 6, 1, False),
+        (3, 3, 2, 3, True), (3, 3, 3, 3, True), (3, 2, 4, 2, False),
+        (4, 2, 2, 4, True), (4, 2, 3, 2, True),  # deg 3 in 4D: generic kernel
+        (5, 2, 2, 2, True), (6, 2, 2, 6, True), (6, 1, 3, 1, False),
+    ],
+)
+def test_shapes_fp64_fp32(kmx_gpu, nd, lev, deg, T, conn):
+    mats, idx, rp, cols, x = make_case(nd, lev, deg, T, conn, seed=nd * 100 + lev * 10 + deg)
+    yo = oracle_y(mats, idx, deg, rp, cols, x)
+    plan, y64 = run(kmx_gpu, mats, idx, deg, rp, cols, x, "fp64")
+    assert rel(y64, yo) <= TOL64, (nd, lev, deg)
+    import torch
+
+    xd = torch.from_numpy(x).cuda()
+    y32 = plan.apply(xd, prec="fp32").cpu().numpy()
+    assert rel(y32, yo) <= TOL32
+    yg = plan.apply(xd, prec="fp32_grouped").cpu().numpy()
+    assert rel(yg, yo) <= TOL32
+
+
+def test_cfg2_full_size_fp64(kmx_gpu):
+    """3D, lev 7, deg 3, 3 terms, connect_nD (P = 1,903,000): FP64 within 1e-13."""
+    import torch
+
+    mats, idx, rp, cols, x = make_case(3, 7, 3, 3, True, seed=2)
+    assert len(cols) == 1903000
+    plan = kmx_gpu.Plan(mats, idx, 3, rp, cols, fp32=True, adjoint=True)
+    xd = torch.from_numpy(x).cuda()
+    y64 = plan.apply(xd, prec="fp64").cpu().numpy()
+    yo = oracle_y(mats, idx, 3, rp, cols, x, threads=16)
+    r64 = rel(y64, yo)
+    print(f"cfg2 fp64 rel err {r64:.3e}")
+    assert r64 <= TOL64
+    y32 = plan.apply(xd, prec="fp32").cpu().numpy()
+    yg = plan.apply(xd, prec="fp32_grouped").cpu().numpy()
+    print(f"cfg2 fp32 rel err {rel(y32, yo):.3e} grouped {rel(yg, yo):.3e}")
+    assert rel(y32, yo) <= TOL32 and rel(yg, yo) <= TOL32
+    # adjoint identity <A x, w> = <x, A^T w> at full size
+    w = np.random.default_rng(5).standard_normal(x.size)
+    wd = torch.from_numpy(w).cuda()
+    aTw = plan.apply_transpose(wd, prec="fp64").cpu().numpy()
+    lhs, rhs = float(np.dot(y64, w)), float(np.dot(x, aTw))
+    assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(y64) * np.linalg.norm(w)
+    # linearity A(2x - 3w) = 2Ax - 3Aw
+    z = plan.apply(2 * xd - 3 * wd, prec="fp64").cpu().numpy()
+    yw = plan.apply(wd, prec="fp64").cpu().numpy()
+    assert rel(z, 2 * y64 - 3 * yw) <= 1e-13
+
+
+@pytest.mark.parametrize("nd,lev,deg,conn", [(2, 3, 2, True), (3, 3, 3, True), (2, 3, 3, False), (4, 2, 3, True)])
+def test_transpose_matches_assembled(kmx_gpu, nd, lev, deg, conn):
+    mats, idx, rp, cols, x = make_case(nd, lev, deg, 2, conn, seed=7 + nd)
+    A = O.build_A(O.ApplyInputs(mats, idx, deg, rp, cols))
+    import torch
+
+    plan = kmx_gpu.Plan(mats, idx, deg, rp, cols, adjoint=True)
+    yT = plan.apply_transpose(torch.from_numpy(x).cuda(), prec="fp64").cpu().numpy()
+    assert rel(yT, A.T @ x) <= TOL64
+    yT32 = plan.apply_transpose(torch.from_numpy(x).cuda(), prec="fp32").cpu().numpy()
+    assert rel(yT32, A.T @ x) <= TOL32
+
+
+def test_edge_cases(kmx_gpu):
+    import torch
+
+    # a single element (lev 0): one 1x1-block pair per term
+    rng = np.random.default_rng(0)
+    mats = [[rng.standard_normal((3, 3)) for _ in range(2)]]
+    idx = np.array([[1, 1]])
+    x = rng.standard_normal(9)
+    _, y = run(kmx_gpu, mats, idx, 3, None, None, x, "fp64")
+    assert rel(y, oracle_y(mats, idx, 3, None, None, x)) <= TOL64
+    # ragged pair list with empty rows: those rows must come out exactly 0
+    mats, idx, _, _, x = make_case(2, 3, 2, 2, False, seed=3)
+    N = idx.shape[0]
+    rows = [sorted(rng.choice(N, size=int(rng.integers(0, 6)), replace=False) + 1) if i % 3 else [] for i in range(N)]
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
+    cols = np.concatenate([np.asarray(r, dtype=np.int32) for r in rows if len(r)] or [np.zeros(0, np.int32)])
+    _, y = run(kmx_gpu, mats, idx, 2, rp, cols, x, "fp64")
+    yo = oracle_y(mats, idx, 2, rp, cols, x)
+    assert rel(y, yo) <= TOL64
+    for i in range(0, N, 3):
+        assert np.all(y[i * 4:(i + 1) * 4] == 0.0)
+    # 0-based indices give the same answer
+    plan0 = kmx_gpu.Plan(mats, idx - 1, 2, rp, cols - 1, index_base=0)
+    y0 = plan0.apply(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert np.array_equal(y0, y)
+
+
+def test_host_entry_and_update_term(kmx_gpu):
+    mats, idx, rp, cols, x = make_case(3, 3, 3, 3, True, seed=11)
+    plan = kmx_gpu.Plan(mats, idx, 3, rp, cols, adjoint=True)
+    y = plan.apply_host(x)
+    assert rel(y, oracle_y(mats, idx, 3, rp, cols, x)) <= TOL64
+    # time-dependent coefficient refresh (get_coeff_mats TD=1): new matrix in term 1, dim 2
+    new = np.random.default_rng(12).standard_normal(mats[1][2].shape)
+    plan.update_term(1, 2, new)
+    mats[1][2] = new
+    assert rel(plan.apply_host(x), oracle_y(mats, idx, 3, rp, cols, x)) <= TOL64
+    # per-term scale
+    plan.set_term_scale(0, 0.5)
+    ref = oracle_y([[0.5 * mats[0][0], mats[0][1], mats[0][2]], mats[1], mats[2]], idx, 3, rp, cols, x)
+    assert rel(plan.apply_host(x), ref) <= TOL64
+
+
+def test_errors(kmx_gpu):
+    import torch
+
+    from kmx._lib import KMX_E_ARG, KMX_E_SHAPE, KmxError
+
+    mats, idx, rp, cols, x = make_case(2, 2, 2, 1, False, seed=4)
+    plan = kmx_gpu.Plan(mats, idx, 2, fp32=False)
+    xd = torch.from_numpy(x).cuda()
+    with pytest.raises(KmxError) as e:
+        plan.apply(xd, prec="fp32")  # no FP32 blocks
+    assert e.value.code == KMX_E_ARG
+    with pytest.raises(KmxError) as e:
+        plan.apply_transpose(xd)  # no adjoint data
+    assert e.value.code == KMX_E_ARG
+    with pytest.raises(KmxError) as e:
+        plan.apply(xd, y=xd)  # aliasing
+    assert e.value.code == KMX_E_ARG
+    bad = idx.copy()
+    bad[0, 0] = 99
+    with pytest.raises(KmxError) as e:
+        kmx_gpu.Plan(mats, bad, 2)
+    assert e.value.code == KMX_E_ARG
+    with pytest.raises(KmxError) as e:
+        kmx_gpu.Plan([[np.zeros((5, 5)), np.zeros((5, 5))]], idx, 2)  # 5 is not a multiple of deg
+    assert e.value.code == KMX_E_SHAPE
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6])
+def test_kronmult_gpu(kmx_gpu, k):
+    """kron_multd.m / kronmult{k}.m on the device, rectangular factors (Kronmult/test1.m)."""
+    import torch
+
+    rng = np.random.default_rng(20 + k)
+    shapes = [(2, 3), (3, 4), (4, 5), (5, 6), (6, 7), (3, 2)][:k]
+    A = [rng.random(s) for s in shapes]
+    X = rng.random((int(np.prod([s[1] for s in shapes])), 2))
+    Y = kmx_gpu.kronmult([torch.from_numpy(a).cuda() for a in A], torch.from_numpy(X).cuda()).cpu().numpy()
+    assert rel(Y, O.kron_multd(A, X)) <= 1e-14
+
+
+def test_gemm_vbatched_and_batch_lists(kmx_gpu):
+    import torch
+
+    rng = np.random.default_rng(31)
+    ms, ns, ks = [3, 5, 1, 4], [2, 4, 3, 4], [4, 2, 5, 3]
+    As = [np.asfortranarray(rng.random((m, k))) for m, k in zip(ms, ks)]
+    Bs = [np.asfortranarray(rng.random((n, k))) for n, k in zip(ns, ks)]  # transB = 'T'
+    Cs = [np.asfortranarray(rng.random((m, n))) for m, n in zip(ms, ns)]
+    dA = [torch.from_numpy(a.ravel(order="F")).cuda() for a in As]
+    dB = [torch.from_numpy(b.ravel(order="F")).cuda() for b in Bs]
+    dC = [torch.from_numpy(c.ravel(order="F").copy()).cuda() for c in Cs]
+    kmx_gpu.gemm_vbatched("N", "T", ms, ns, ks, 1.5, dA, ms, dB, ns, -0.5, dC, ms)
+    for a, b, c, d, m, n in zip(As, Bs, Cs, dC, ms, ns):
+        got = d.cpu().numpy().reshape((m, n), order="F")
+        assert np.allclose(got, 1.5 * a @ b.T - 0.5 * c, rtol=1e-14, atol=1e-14)
+    # replay kronmult3_batch's stage lists stage by stage through gemm_vbatched
+    shapes = [(2, 3), (3, 4), (4, 5)]
+    mats = [rng.random(s) for s in shapes]
+    nvec = 2
+    X = rng.random((60, nvec))
+    descs, work = kmx_gpu.kronmult_batch_plan([s[0] for s in shapes], [s[1] for s in shapes], nvec)
+    bufs = {0: torch.from_numpy(X.ravel(order="F")).cuda(), 1: torch.zeros(24 * nvec, dtype=torch.float64).cuda(),
+            2: torch.zeros(max(1, work), dtype=torch.float64).cuda(), 3: torch.zeros(1, dtype=torch.float64).cuda()}
+    for i, m in enumerate(mats):
+        bufs[4 + i] = torch.from_numpy(m.ravel(order="F")).cuda()
+    for s in sorted({d["stage"] for d in descs}):
+        st = [d for d in descs if d["stage"] == s]
+        for ta in {d["transA"] for d in st}:
+            for tb in {d["transB"] for d in st}:
+                g = [d for d in st if d["transA"] == ta and d["transB"] == tb]
+                ptr = lambda b, o: bufs[b].data_ptr() + 8 * o  # noqa: E731
+                kmx_gpu.gemm_vbatched(ta, tb, [d["m"] for d in g], [d["n"] for d in g], [d["k"] for d in g], 1.0,
+                                      [ptr(d["bufA"], d["offA"]) for d in g], [d["lda"] for d in g],
+                                      [ptr(d["bufB"], d["offB"]) for d in g], [d["ldb"] for d in g], 0.0,
+                                      [ptr(d["bufC"], d["offC"]) for d in g], [d["ldc"] for d in g])
+    Y = bufs[1].cpu().numpy().reshape((24, nvec), order="F")
+    assert rel(Y, np.kron(np.kron(mats[0], mats[1]), mats[2]) @ X) <= 1e-14
+
+
+def test_batch_matches_single_plans(kmx_gpu):
+    """One persistent launch over instances of mixed (d, lev, deg) == their own applies."""
+    import torch
+
+    cases = [(2, 3, 2, 2, False), (3, 2, 3, 3, True), (2, 4, 4, 2, False), (3, 3, 2, 2, True), (2, 2, 3, 1, False),
+             (4, 2, 2, 2, True)]  # the 4D one is served outside the persistent kernel
+    plans, xs, ys, refs = [], [], [], []
+    for c, (nd, lev, deg, T, conn) in enumerate(cases):
+        mats, idx, rp, cols, x = make_case(nd, lev, deg, T, conn, seed=40 + c)
+        p = kmx_gpu.Plan(mats, idx, deg, rp, cols)
+        plans.append(p)
+        xs.append(torch.from_numpy(x).cuda())
+        ys.append(torch.empty_like(xs[-1]))
+        refs.append(oracle_y(mats, idx, deg, rp, cols, x))
+    b = kmx_gpu.Batch(plans)
+    for prec, tol in (("fp64", TOL64), ("fp32", TOL32)):
+        b.apply(xs, ys, prec=prec)
+        torch.cuda.synchronize()
+        for y, r in zip(ys, refs):
+            assert rel(y.cpu().numpy(), r) <= tol, prec
