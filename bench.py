@@ -111,6 +111,7 @@ def main():
     plan = kmx.Plan(op["mats"], op["table"].idx1d, op["deg"], op["rp"], op["cols"], index_base=1, fp32=True)
     xd = torch.from_numpy(op["x"]).cuda()
     yd = torch.empty_like(xd)
+    variants = plan.autotune()  # workgroup size / LDS staging per precision, timed on this plan
 
     res = {}
     for prec in ("fp64", "fp32", "fp32_grouped"):
@@ -153,7 +154,7 @@ def main():
         "dtype": "f32",
         "data": f"synthetic: x ~ N(0,1) seed 2; 1D operators {op['source']}",
         "config": {"workload": "3D SG lev7 deg3, 3 terms, connect_nD pairs (P=%d, N=%d)" % (plan.P, plan.N),
-                   "variant": best, "block": plan.B},
+                   "variant": best, "block": plan.B, "tuned": variants},
         "fp64_plain": {"applies_per_s": round(1000.0 / res["fp64"], 2), "ms_per_apply": round(res["fp64"], 5),
                        "achieved_TFLOPs": round(flops64 / (res["fp64"] * 1e-3) / 1e12, 3),
                        "frac_fp64_peak": round(flops64 / (res["fp64"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),

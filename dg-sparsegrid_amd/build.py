@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "libkmx.so")
-SOURCES = ["kmx_api.hip", "kmx_apply.hip", "kmx_kron.hip"]
+SOURCES = ["kmx_api.hip", "kmx_apply.hip", "kmx_apply_f64.hip", "kmx_apply_f32.hip", "kmx_kron.hip"]
 ARCH = "gfx950"
 
 
@@ -49,7 +49,7 @@ def compile_one(src, verbose):
     return obj
 
 
-def build(jobs=3, verbose=False):
+def build(jobs=5, verbose=False):
     os.makedirs(OBJDIR, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(lambda s: compile_one(s, verbose), SOURCES))
@@ -64,7 +64,7 @@ def build(jobs=3, verbose=False):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("-j", type=int, default=3)
+    ap.add_argument("-j", type=int, default=5)
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args()
     print(build(a.j, a.verbose))

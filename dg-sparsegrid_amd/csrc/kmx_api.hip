@@ -209,6 +209,8 @@ int kmx_plan_create(const kmx_plan_desc* desc, kmx_plan** out) {
         // prefix groups for the grouped FP32 kernel: per row, pairs sorted by
         // (J_1..J_{d-1}) then J_d, groups ordered by size (largest first)
         if (want32 && D >= 2) {
+            int chunk = 4;
+            if (const char* e = getenv("KMX_GROUP_CHUNK")) chunk = std::max(1, atoi(e));
             std::vector<int> gcols;
             gcols.reserve(size_t(p->P));
             std::vector<int> gptr;
@@ -240,6 +242,11 @@ int kmx_plan_create(const kmx_plan_desc* desc, kmx_plan** out) {
                         s = q;
                     }
                 }
+                // chunks of at most `chunk` pairs bound the serial work of one lane
+                std::vector<std::pair<int, int>> chunks;
+                for (auto& g : groups)
+                    for (int q = g.first; q < g.second; q += chunk) chunks.push_back({q, std::min(q + chunk, g.second)});
+                groups.swap(chunks);
                 std::stable_sort(groups.begin(), groups.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
                     return (a.second - a.first) > (b.second - b.first);
                 });
@@ -288,6 +295,7 @@ int kmx_plan_create(const kmx_plan_desc* desc, kmx_plan** out) {
                 p->d_rows = upload(rows.data(), rows.size());
             }
         }
+        kmx_plan_apply_cached_variants(p);
         *out = p;
         return KMX_OK;
     } catch (const std::exception& e) {

@@ -1,494 +1,14 @@
-// kmx_apply.hip -- gfx950 kernels for y = A x with A = sum over element pairs
-// (i, j) and terms t of kron_k A_{t,k}[I_k(i), J_k(j)]  (apply_A.m:100-237).
-//
-// Work decomposition (CDNA4, wave64):
-//   * one workgroup per row element i (rows issued heaviest first), its NT
-//     threads stride over i's pair list, one pair per lane;
-//   * a lane keeps the deg^d block of x_j and its partial y_i in registers and
-//     runs the d contractions of each term in place, fibre by fibre, dim d
-//     first and dim 1 last (the order of kronmult{d} method 1,
-//     Kronmult/kronmult3.m:73-89);
-//   * the deg x deg coefficient blocks A_{t,k}[I_k(i), :] of the row are staged
-//     once per workgroup into LDS when they fit (they are contiguous in the
-//     plan's block-row-major layout), else read through L2;
-//   * y_i is reduced across the workgroup with cross-lane shuffles and one LDS
-//     pass, then written once: no atomics, deterministic.
-// Blocks too large for registers go to a generic LDS kernel (one pair at a time,
-// threads over block entries).
-#include "kmx_internal.h"
+// kmx_apply.hip -- host-side launch logic of the operator apply: variant
+// choice (autotuned per plan and precision), argument packing, batches.
+// Device code lives in kmx_kernels.h (instantiated in kmx_apply_f64.hip and
+// kmx_apply_f32.hip).
+#include "kmx_kernels.h"
 
 #include <algorithm>
 #include <cstring>
-
-namespace kmx {
-
-template <int A, int N>
-struct Pow {
-    static constexpr int v = A * Pow<A, N - 1>::v;
-};
-template <int A>
-struct Pow<A, 0> {
-    static constexpr int v = 1;
-};
-
-template <typename T, int N>
-using RegArr = T[N];
-
-__device__ __forceinline__ double madd(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-
-// In-place contraction of dim K (0 = slowest) of a register block with the
-// deg x deg block m (row-major [a][b]): v[.., a, ..] = sum_b m[a][b] v[.., b, ..].
-template <typename T, int D, int DEG, int K>
-__device__ __forceinline__ void contract_dim(RegArr<T, Pow<DEG, D>::v>& v, const RegArr<T, DEG * DEG>& m) {
-    constexpr int B = Pow<DEG, D>::v;
-    constexpr int S = Pow<DEG, D - 1 - K>::v;
-    constexpr int O = B / (S * DEG);
-#pragma unroll
-    for (int o = 0; o < O; ++o) {
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            T in[DEG];
-#pragma unroll
-            for (int b = 0; b < DEG; ++b) in[b] = v[o * S * DEG + b * S + s];
-#pragma unroll
-            for (int a = 0; a < DEG; ++a) {
-                T acc = m[a * DEG] * in[0];
-#pragma unroll
-                for (int b = 1; b < DEG; ++b) acc = madd(m[a * DEG + b], in[b], acc);
-                v[o * S * DEG + a * S + s] = acc;
-            }
-        }
-    }
-}
-
-// Per-row context shared by the chain steps.
-template <typename T, int D, int DEG, bool USE_LDS>
-struct RowCtx {
-    const KArgs<T>* a;
-    const T* strip;           // LDS strips of this row (USE_LDS)
-    int I[D];                 // row's 0-based 1D indices
-    int soff[D];              // LDS offset of (t=0, k) strip
-    int sterm;                // LDS elements per term
-
-    __device__ __forceinline__ void load_block(int t, int k, int J, RegArr<T, DEG * DEG>& m) const {
-        const T* p;
-        if (USE_LDS) {
-            p = strip + t * sterm + soff[k] + J * (DEG * DEG);
-        } else {
-            p = a->blk + a->off[t * D + k] + ((long long)I[k] * a->nb[k] + J) * (DEG * DEG);
-        }
-#pragma unroll
-        for (int e = 0; e < DEG * DEG; ++e) m[e] = p[e];
-    }
-};
-
-template <typename T, int D, int DEG, bool USE_LDS, int K>
-struct Chain {
-    __device__ __forceinline__ static void run(RegArr<T, Pow<DEG, D>::v>& v, const RowCtx<T, D, DEG, USE_LDS>& c, int t,
-                                               const RegArr<int, D>& J) {
-        T m[DEG * DEG];
-        c.load_block(t, K, J[K], m);
-        contract_dim<T, D, DEG, K>(v, m);
-        Chain<T, D, DEG, USE_LDS, K - 1>::run(v, c, t, J);
-    }
-};
-template <typename T, int D, int DEG, bool USE_LDS>
-struct Chain<T, D, DEG, USE_LDS, -1> {
-    __device__ __forceinline__ static void run(RegArr<T, Pow<DEG, D>::v>&, const RowCtx<T, D, DEG, USE_LDS>&, int,
-                                               const RegArr<int, D>&) {}
-};
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// One row: NT lanes stride over the pair list; y_i reduced and written once.
-template <typename T, int D, int DEG, int NT, bool USE_LDS, bool KEEPX>
-__device__ __forceinline__ void apply_row(const KArgs<T>& a, const double* __restrict__ x, double* __restrict__ y,
-                                          int row, unsigned char* smem) {
-    constexpr int B = Pow<DEG, D>::v;
-    constexpr int NW = NT / 64;
-    const int tid = threadIdx.x;
-
-    RowCtx<T, D, DEG, USE_LDS> c;
-    c.a = &a;
-#pragma unroll
-    for (int k = 0; k < D; ++k) c.I[k] = a.idx1d[(long long)row * D + k];
-
-    if (USE_LDS) {
-        T* strip = reinterpret_cast<T*>(smem);
-        int so = 0;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            c.soff[k] = so;
-            so += a.nb[k] * DEG * DEG;
-        }
-        c.sterm = so;
-        for (int t = 0; t < a.nterms; ++t) {
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                const T* src = a.blk + a.off[t * D + k] + (long long)c.I[k] * a.nb[k] * DEG * DEG;
-                T* dst = strip + t * so + c.soff[k];
-                const int len = a.nb[k] * DEG * DEG;
-                for (int e = tid; e < len; e += NT) dst[e] = src[e];
-            }
-        }
-        __syncthreads();
-        c.strip = strip;
-    }
-
-    long long pb, pe;
-    if (a.row_ptr) {
-        pb = a.row_ptr[row];
-        pe = a.row_ptr[row + 1];
-    } else {
-        pb = 0;
-        pe = a.N;
-    }
-
-    T acc[B];
-#pragma unroll
-    for (int q = 0; q < B; ++q) acc[q] = T(0);
-
-    for (long long q = pb + tid; q < pe; q += NT) {
-        const int j = a.row_ptr ? a.cols[q] : int(q);
-        int J[D];
-#pragma unroll
-        for (int k = 0; k < D; ++k) J[k] = a.idx1d[(long long)j * D + k];
-        const double* xj = x + (long long)j * B;
-        T xv[KEEPX ? B : 1];
-        if (KEEPX) {
-#pragma unroll
-            for (int e = 0; e < B; ++e) xv[e] = T(xj[e]);
-        }
-        for (int t = 0; t < a.nterms; ++t) {
-            T v[B];
-#pragma unroll
-            for (int e = 0; e < B; ++e) v[e] = KEEPX ? xv[e] : T(xj[e]);
-            Chain<T, D, DEG, USE_LDS, D - 1>::run(v, c, t, J);
-            const T s = a.scale[t];
-#pragma unroll
-            for (int e = 0; e < B; ++e) acc[e] = madd(s, v[e], acc[e]);
-        }
-    }
-
-    // workgroup reduction of acc[B]
-    if (USE_LDS) __syncthreads();  // strips are dead; reuse LDS
-    T* red = reinterpret_cast<T*>(smem);
-    const int lane = tid & 63, w = tid >> 6;
-#pragma unroll
-    for (int e = 0; e < B; ++e) {
-        T s = wave_sum(acc[e]);
-        if (lane == 0) red[w * B + e] = s;
-    }
-    __syncthreads();
-    for (int e = tid; e < B; e += NT) {
-        T s = red[e];
-#pragma unroll
-        for (int ww = 1; ww < NW; ++ww) s += red[ww * B + e];
-        y[(long long)row * B + e] = double(s);
-    }
-}
-
-template <typename T, int D, int DEG, int NT, bool USE_LDS, bool KEEPX>
-__global__ void __launch_bounds__(NT) k_apply_reg(KArgs<T> a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int row = a.row_order[blockIdx.x];
-    apply_row<T, D, DEG, NT, USE_LDS, KEEPX>(a, a.x, a.y, row, smem);
-}
-
-// ------------------------------------------------------------ many instances
-// One persistent launch over a work list of (instance, row) items, heaviest
-// first, striding by the grid; each instance keeps its own plan (levels, deg,
-// pair list, coefficient blocks).  The shape switch selects the fully unrolled
-// row body for the instance's (d, deg).
-#define KMX_BATCH_MAX 128
-struct BatchIO {
-    const double* x[KMX_BATCH_MAX];
-    double* y[KMX_BATCH_MAX];
-};
-
-template <typename T, int D, int DEG>
-__device__ __forceinline__ void batch_row(const KArgs<T>& a, const double* x, double* y, int row, unsigned char* sm) {
-    if constexpr (Pow<DEG, D>::v * int(sizeof(T)) <= 256) {
-        constexpr bool KX = 3 * Pow<DEG, D>::v * int(sizeof(T)) / 4 <= 144;
-        apply_row<T, D, DEG, 256, false, KX>(a, x, y, row, sm);
-    }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_apply_batch(const KArgs<T>* __restrict__ args, const int2* __restrict__ items,
-                                                     int nitems, const int* __restrict__ shape, BatchIO io) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const int2 w = items[it];
-        const KArgs<T>& a = args[w.x];
-        const double* x = io.x[w.x];
-        double* y = io.y[w.x];
-        switch (shape[w.x]) {
-            case 0x11: batch_row<T, 1, 1>(a, x, y, w.y, smem); break;
-            case 0x12: batch_row<T, 1, 2>(a, x, y, w.y, smem); break;
-            case 0x13: batch_row<T, 1, 3>(a, x, y, w.y, smem); break;
-            case 0x14: batch_row<T, 1, 4>(a, x, y, w.y, smem); break;
-            case 0x21: batch_row<T, 2, 1>(a, x, y, w.y, smem); break;
-            case 0x22: batch_row<T, 2, 2>(a, x, y, w.y, smem); break;
-            case 0x23: batch_row<T, 2, 3>(a, x, y, w.y, smem); break;
-            case 0x24: batch_row<T, 2, 4>(a, x, y, w.y, smem); break;
-            case 0x31: batch_row<T, 3, 1>(a, x, y, w.y, smem); break;
-            case 0x32: batch_row<T, 3, 2>(a, x, y, w.y, smem); break;
-            case 0x33: batch_row<T, 3, 3>(a, x, y, w.y, smem); break;
-            case 0x34: batch_row<T, 3, 4>(a, x, y, w.y, smem); break;
-            default: break;
-        }
-        __syncthreads();
-    }
-}
-
-// ------------------------------------------------------------------ generic
-// One workgroup per row, one pair at a time, threads over the block entries;
-// any (d, deg) whose three block buffers fit in LDS.
-template <typename T>
-__global__ void __launch_bounds__(256) k_apply_generic(KArgs<T> a, int D, int DEG) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    int B = 1;
-    for (int k = 0; k < D; ++k) B *= DEG;
-    T* acc = reinterpret_cast<T*>(smem);
-    T* v = acc + B;
-    T* w = v + B;
-    const int row = a.row_order[blockIdx.x];
-    const int tid = threadIdx.x;
-    int I[KMX_MAX_DIM], J[KMX_MAX_DIM];
-    for (int k = 0; k < D; ++k) I[k] = a.idx1d[(long long)row * D + k];
-    for (int e = tid; e < B; e += 256) acc[e] = T(0);
-    long long pb = a.row_ptr ? a.row_ptr[row] : 0;
-    long long pe = a.row_ptr ? a.row_ptr[row + 1] : a.N;
-    for (long long q = pb; q < pe; ++q) {
-        const int j = a.row_ptr ? a.cols[q] : int(q);
-        for (int k = 0; k < D; ++k) J[k] = a.idx1d[(long long)j * D + k];
-        for (int t = 0; t < a.nterms; ++t) {
-            __syncthreads();
-            for (int e = tid; e < B; e += 256) v[e] = T(a.x[(long long)j * B + e]);
-            __syncthreads();
-            T* vin = v;
-            T* vout = w;
-            int stride = 1;
-            for (int k = D - 1; k >= 0; --k) {
-                const T* m = a.blk + a.off[t * D + k] + ((long long)I[k] * a.nb[k] + J[k]) * (DEG * DEG);
-                for (int e = tid; e < B; e += 256) {
-                    int ai = (e / stride) % DEG;
-                    int base = e - ai * stride;
-                    T s = m[ai * DEG] * vin[base];
-                    for (int b = 1; b < DEG; ++b) s = madd(m[ai * DEG + b], vin[base + b * stride], s);
-                    vout[e] = s;
-                }
-                __syncthreads();
-                T* tmp = vin;
-                vin = vout;
-                vout = tmp;
-                stride *= DEG;
-            }
-            const T s = a.scale[t];
-            for (int e = tid; e < B; e += 256) acc[e] = madd(s, vin[e], acc[e]);
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < B; e += 256) a.y[(long long)row * B + e] = double(acc[e]);
-}
-
-// -------------------------------------------------------------- grouped FP32
-// Pairs of a row are pre-sorted so that pairs sharing (J_1..J_{d-1}) are
-// adjacent ("prefix groups").  A lane takes one group: it contracts dim d of
-// every member's x_j per term and sums them (W_t), then applies dims d-1..1
-// once per group and term.  Same sum as the pair-by-pair chain, fewer flops.
-template <int D, int DEG, int NT, int TMAX>
-__global__ void __launch_bounds__(NT) k_apply_grouped(KArgs<float> a) {
-    constexpr int B = Pow<DEG, D>::v;
-    constexpr int NW = NT / 64;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int row = a.row_order[blockIdx.x];
-    const int tid = threadIdx.x;
-    RowCtx<float, D, DEG, true> c;
-    c.a = &a;
-#pragma unroll
-    for (int k = 0; k < D; ++k) c.I[k] = a.idx1d[(long long)row * D + k];
-    float* strip = reinterpret_cast<float*>(smem);
-    {
-        int so = 0;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            c.soff[k] = so;
-            so += a.nb[k] * DEG * DEG;
-        }
-        c.sterm = so;
-        for (int t = 0; t < a.nterms; ++t) {
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                const float* src = a.blk + a.off[t * D + k] + (long long)c.I[k] * a.nb[k] * DEG * DEG;
-                float* dst = strip + t * so + c.soff[k];
-                const int len = a.nb[k] * DEG * DEG;
-                for (int e = tid; e < len; e += NT) dst[e] = src[e];
-            }
-        }
-        __syncthreads();
-        c.strip = strip;
-    }
-    float acc[B];
-#pragma unroll
-    for (int e = 0; e < B; ++e) acc[e] = 0.f;
-
-    const long long gb = a.grp_row_ptr[row], ge = a.grp_row_ptr[row + 1] - 1;  // groups gb..ge-1
-    for (long long g = gb + tid; g < ge; g += NT) {
-        const int p0 = a.grp_ptr[g], p1 = a.grp_ptr[g + 1];
-        int J[D];
-        {
-            const int j0 = a.cols[p0];
-#pragma unroll
-            for (int k = 0; k < D - 1; ++k) J[k] = a.idx1d[(long long)j0 * D + k];
-        }
-        float W[TMAX][B];
-#pragma unroll
-        for (int t = 0; t < TMAX; ++t)
-#pragma unroll
-            for (int e = 0; e < B; ++e) W[t][e] = 0.f;
-        for (int p = p0; p < p1; ++p) {
-            const int j = a.cols[p];
-            const int Jd = a.idx1d[(long long)j * D + (D - 1)];
-            float xv[B];
-            const double* xj = a.x + (long long)j * B;
-#pragma unroll
-            for (int e = 0; e < B; ++e) xv[e] = float(xj[e]);
-#pragma unroll
-            for (int t = 0; t < TMAX; ++t) {
-                if (t < a.nterms) {
-                    float m[DEG * DEG];
-                    c.load_block(t, D - 1, Jd, m);
-                    float v[B];
-#pragma unroll
-                    for (int e = 0; e < B; ++e) v[e] = xv[e];
-                    contract_dim<float, D, DEG, D - 1>(v, m);
-#pragma unroll
-                    for (int e = 0; e < B; ++e) W[t][e] += v[e];
-                }
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < TMAX; ++t) {
-            if (t < a.nterms) {
-                Chain<float, D, DEG, true, D - 2>::run(W[t], c, t, J);
-                const float s = a.scale[t];
-#pragma unroll
-                for (int e = 0; e < B; ++e) acc[e] = madd(s, W[t][e], acc[e]);
-            }
-        }
-    }
-    __syncthreads();
-    float* red = reinterpret_cast<float*>(smem);
-    const int lane = tid & 63, w = tid >> 6;
-#pragma unroll
-    for (int e = 0; e < B; ++e) {
-        float s = wave_sum(acc[e]);
-        if (lane == 0) red[w * B + e] = s;
-    }
-    __syncthreads();
-    for (int e = tid; e < B; e += NT) {
-        float s = red[e];
-#pragma unroll
-        for (int ww = 1; ww < NW; ++ww) s += red[ww * B + e];
-        a.y[(long long)row * B + e] = double(s);
-    }
-}
-
-// ---------------------------------------------------------------- dispatch
-using LaunchFn = hipError_t (*)(const void* args, int nrows, size_t smem, hipStream_t s);
-
-template <typename T, int D, int DEG, int NT, bool USE_LDS, bool KEEPX>
-hipError_t launch_reg(const void* args, int nrows, size_t smem, hipStream_t s) {
-    auto kern = k_apply_reg<T, D, DEG, NT, USE_LDS, KEEPX>;
-    if (smem > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(kern, dim3(nrows), dim3(NT), smem, s, *reinterpret_cast<const KArgs<T>*>(args));
-    return hipGetLastError();
-}
-
-template <typename T, int D, int DEG>
-constexpr bool reg_ok() {
-    return Pow<DEG, D>::v * int(sizeof(T)) <= 256;  // B <= 32 (FP64) / 64 (FP32)
-}
-template <typename T, int D, int DEG>
-constexpr bool keep_x() {
-    return 3 * Pow<DEG, D>::v * int(sizeof(T)) / 4 <= 144;
-}
-
-template <typename T, int D, int DEG>
-LaunchFn pick_reg(bool lds) {
-    if constexpr (reg_ok<T, D, DEG>()) {
-        constexpr bool KX = keep_x<T, D, DEG>();
-        return lds ? &launch_reg<T, D, DEG, 256, true, KX> : &launch_reg<T, D, DEG, 256, false, KX>;
-    } else {
-        (void)lds;
-        return nullptr;
-    }
-}
-
-template <typename T, int D>
-LaunchFn pick_deg(int deg, bool lds) {
-    switch (deg) {
-        case 1: return pick_reg<T, D, 1>(lds);
-        case 2: return pick_reg<T, D, 2>(lds);
-        case 3: return pick_reg<T, D, 3>(lds);
-        case 4: return pick_reg<T, D, 4>(lds);
-        case 5: return pick_reg<T, D, 5>(lds);
-        case 6: return pick_reg<T, D, 6>(lds);
-        case 7: return pick_reg<T, D, 7>(lds);
-        case 8: return pick_reg<T, D, 8>(lds);
-        default: return nullptr;
-    }
-}
-
-template <typename T>
-LaunchFn pick(int D, int deg, bool lds) {
-    switch (D) {
-        case 1: return pick_deg<T, 1>(deg, lds);
-        case 2: return pick_deg<T, 2>(deg, lds);
-        case 3: return pick_deg<T, 3>(deg, lds);
-        case 4: return pick_deg<T, 4>(deg, lds);
-        case 5: return pick_deg<T, 5>(deg, lds);
-        case 6: return pick_deg<T, 6>(deg, lds);
-        default: return nullptr;
-    }
-}
-
-// grouped FP32 kernels are built for the configurations that pay off
-template <int D, int DEG, int TMAX>
-hipError_t launch_grouped(const void* args, int nrows, size_t smem, hipStream_t s) {
-    auto kern = k_apply_grouped<D, DEG, 256, TMAX>;
-    if (smem > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(kern, dim3(nrows), dim3(256), smem, s, *reinterpret_cast<const KArgs<float>*>(args));
-    return hipGetLastError();
-}
-
-LaunchFn pick_grouped(int D, int deg, int T) {
-    if (T > 4) return nullptr;
-    if (D == 2 && deg == 2) return &launch_grouped<2, 2, 4>;
-    if (D == 2 && deg == 3) return &launch_grouped<2, 3, 4>;
-    if (D == 2 && deg == 4) return &launch_grouped<2, 4, 4>;
-    if (D == 3 && deg == 2) return &launch_grouped<3, 2, 4>;
-    if (D == 3 && deg == 3) return T <= 3 ? &launch_grouped<3, 3, 3> : &launch_grouped<3, 3, 4>;
-    return nullptr;
-}
-
-}  // namespace kmx
+#include <map>
+#include <mutex>
+#include <tuple>
 
 using namespace kmx;
 
@@ -523,75 +43,72 @@ static void fill_args(KArgs<T>& a, const kmx_plan* p, bool transpose, const doub
     a.strip_elems = se * p->T;
 }
 
-int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, int prec, hipStream_t s) {
-    if (p->N == 0) return KMX_OK;
+static int prec_slot(int prec) { return prec == KMX_PREC_FP64 ? 0 : 1; }
+
+static bool strip_fits(const kmx_plan* p, size_t elem) {
+    size_t se = 0;
+    for (int k = 0; k < p->D; ++k) se += size_t(p->nb[k]) * p->deg * p->deg;
+    return se * p->T * elem <= lds_limit();
+}
+
+// One launch of the pair-by-pair kernel family with an explicit variant.
+static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* y, int prec, kmx_variant v,
+                          hipStream_t s) {
     const int D = p->D, deg = p->deg, B = p->B;
     hipError_t err = hipSuccess;
-    if (prec == KMX_PREC_FP32_GROUPED && (transpose || !p->d_gcols)) prec = KMX_PREC_FP32;  // no groups: same sum pair by pair
-    if (prec == KMX_PREC_FP32_GROUPED) {
+    if (prec == KMX_PREC_FP64 || prec == KMX_PREC_FP32) {
+        const bool f64 = prec == KMX_PREC_FP64;
+        const size_t elem = f64 ? sizeof(double) : sizeof(float);
+        if (!f64 && !(transpose ? p->d_blkT32 : p->d_blk32)) {
+            set_error("kmx_apply: plan has no FP32 blocks (create it with KMX_PLAN_FP32)");
+            return KMX_E_ARG;
+        }
+        const bool fits = strip_fits(p, elem);
+        const bool lds = (v.lds < 0) ? fits : (v.lds != 0 && fits);
+        const int nt = v.nt > 0 ? v.nt : 256;
+        LaunchFn f = f64 ? pick_f64(D, deg, nt, lds) : pick_f32(D, deg, nt, lds);
+        if (f) {
+            size_t strip = 0;
+            for (int k = 0; k < D; ++k) strip += size_t(p->nb[k]) * deg * deg;
+            strip *= p->T * elem;
+            const size_t smem = std::max(lds ? strip : size_t(0), size_t(nt / 64) * B * elem);
+            if (f64) {
+                KArgs<double> a;
+                fill_args(a, p, transpose, x, y);
+                err = f(&a, p->N, smem, s);
+            } else {
+                KArgs<float> a;
+                fill_args(a, p, transpose, x, y);
+                err = f(&a, p->N, smem, s);
+            }
+        } else {
+            const size_t smem = size_t(3 * B) * elem;
+            if (smem > 160 * 1024) {
+                set_error("kmx_apply: block too large for the generic kernel");
+                return KMX_E_UNSUPPORTED;
+            }
+            if (f64) {
+                KArgs<double> a;
+                fill_args(a, p, transpose, x, y);
+                err = launch_generic_f64(a, D, deg, p->N, smem, s);
+            } else {
+                KArgs<float> a;
+                fill_args(a, p, transpose, x, y);
+                err = launch_generic_f32(a, D, deg, p->N, smem, s);
+            }
+        }
+    } else if (prec == KMX_PREC_FP32_GROUPED) {
         KArgs<float> a;
         fill_args(a, p, false, x, y);
         a.cols = p->d_gcols;
         a.grp_ptr = p->d_grp_ptr;
         a.grp_row_ptr = p->d_grp_row_ptr;
-        LaunchFn f = pick_grouped(D, deg, p->T);
-        size_t smem = std::max(size_t(a.strip_elems) * sizeof(float), size_t(4 * B) * sizeof(float));
-        if (f && smem <= lds_limit()) {
-            err = f(&a, p->N, smem, s);
-            if (err != hipSuccess) {
-                set_error(std::string("grouped apply launch: ") + hipGetErrorString(err));
-                return KMX_E_HIP;
-            }
-            return KMX_OK;
-        }
-        prec = KMX_PREC_FP32;  // no grouped kernel for this shape: same sum pair by pair
-    }
-    if (prec == KMX_PREC_FP64) {
-        KArgs<double> a;
-        fill_args(a, p, transpose, x, y);
-        size_t strip = size_t(a.strip_elems) * sizeof(double);
-        bool lds = strip <= lds_limit();
-        LaunchFn f = pick<double>(D, deg, lds);
-        if (f) {
-            size_t smem = std::max(lds ? strip : size_t(0), size_t(4 * B) * sizeof(double));
-            err = f(&a, p->N, smem, s);
-        } else {
-            size_t smem = size_t(3 * B) * sizeof(double);
-            if (smem > 160 * 1024) {
-                set_error("kmx_apply: block too large for the generic kernel");
-                return KMX_E_UNSUPPORTED;
-            }
-            if (smem > 65536)
-                hipFuncSetAttribute(reinterpret_cast<const void*>(k_apply_generic<double>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
-            hipLaunchKernelGGL(k_apply_generic<double>, dim3(p->N), dim3(256), smem, s, a, D, deg);
-            err = hipGetLastError();
-        }
-    } else if (prec == KMX_PREC_FP32) {
-        if (!(transpose ? p->d_blkT32 : p->d_blk32)) {
-            set_error("kmx_apply: plan has no FP32 blocks (create it with KMX_PLAN_FP32)");
-            return KMX_E_ARG;
-        }
-        KArgs<float> a;
-        fill_args(a, p, transpose, x, y);
-        size_t strip = size_t(a.strip_elems) * sizeof(float);
-        bool lds = strip <= lds_limit();
-        LaunchFn f = pick<float>(D, deg, lds);
-        if (f) {
-            size_t smem = std::max(lds ? strip : size_t(0), size_t(4 * B) * sizeof(float));
-            err = f(&a, p->N, smem, s);
-        } else {
-            size_t smem = size_t(3 * B) * sizeof(float);
-            if (smem > 160 * 1024) {
-                set_error("kmx_apply: block too large for the generic kernel");
-                return KMX_E_UNSUPPORTED;
-            }
-            if (smem > 65536)
-                hipFuncSetAttribute(reinterpret_cast<const void*>(k_apply_generic<float>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
-            hipLaunchKernelGGL(k_apply_generic<float>, dim3(p->N), dim3(256), smem, s, a, D, deg);
-            err = hipGetLastError();
-        }
+        const int nt = (p->N && p->G / p->N >= 384) ? 512 : 256;  // about one chunk per lane
+        LaunchFn f = pick_grouped(D, deg, p->T, nt);
+        if (!f) f = pick_grouped(D, deg, p->T, 256);
+        size_t smem = std::max(size_t(a.strip_elems) * sizeof(float), size_t(nt / 64) * B * sizeof(float));
+        if (!f || smem > lds_limit()) return launch_variant(p, transpose, x, y, KMX_PREC_FP32, v, s);
+        err = f(&a, p->N, smem, s);
     } else {
         set_error("kmx_apply: unknown precision");
         return KMX_E_ARG;
@@ -600,6 +117,125 @@ int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, in
         set_error(std::string("apply launch: ") + hipGetErrorString(err));
         return KMX_E_HIP;
     }
+    return KMX_OK;
+}
+
+int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, int prec, hipStream_t s) {
+    if (p->N == 0) return KMX_OK;
+    if (prec == KMX_PREC_FP32_GROUPED && (transpose || !p->d_gcols)) prec = KMX_PREC_FP32;  // same sum pair by pair
+    if (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32 && prec != KMX_PREC_FP32_GROUPED) {
+        set_error("kmx_apply: unknown precision");
+        return KMX_E_ARG;
+    }
+    kmx_variant v = prec == KMX_PREC_FP32_GROUPED ? p->var[1] : p->var[prec_slot(prec)];
+    return launch_variant(p, transpose, x, y, prec, v, s);
+}
+
+// ---------------------------------------------------------------- autotuner
+// Per (d, deg, terms, precision, pair-density bucket): the workgroup size and
+// LDS staging that ran fastest on a real plan of that shape.  Plans created
+// later with the same key start from the cached choice.
+namespace {
+struct TuneKey {
+    int D, deg, T, prec, dens;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(D, deg, T, prec, dens) < std::tie(o.D, o.deg, o.T, o.prec, o.dens);
+    }
+};
+std::map<TuneKey, kmx_variant>& tune_cache() {
+    static std::map<TuneKey, kmx_variant> c;
+    return c;
+}
+std::mutex& tune_mutex() {
+    static std::mutex m;
+    return m;
+}
+TuneKey key_of(const kmx_plan* p, int prec) {
+    const double per_row = p->N ? double(p->P) / p->N : 0.0;
+    int dens = 0;
+    while ((1 << dens) < per_row && dens < 30) ++dens;
+    return TuneKey{p->D, p->deg, p->T, prec, dens};
+}
+}  // namespace
+
+void kmx_plan_apply_cached_variants(kmx_plan* p) {
+    std::lock_guard<std::mutex> g(tune_mutex());
+    for (int prec : {KMX_PREC_FP64, KMX_PREC_FP32}) {
+        auto it = tune_cache().find(key_of(p, prec));
+        if (it != tune_cache().end()) p->var[prec_slot(prec)] = it->second;
+    }
+}
+
+extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
+    if (!p) {
+        set_error("kmx_plan_autotune: null plan");
+        return KMX_E_ARG;
+    }
+    if (p->N == 0) return KMX_OK;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const size_t n = size_t(p->N) * p->B;
+    double *x = nullptr, *y = nullptr;
+    if (hipMalloc(&x, n * sizeof(double)) != hipSuccess || hipMalloc(&y, n * sizeof(double)) != hipSuccess) {
+        if (x) hipFree(x);
+        set_error("kmx_plan_autotune: allocation failed");
+        return KMX_E_NOMEM;
+    }
+    hipMemsetAsync(x, 0, n * sizeof(double), s);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    int rc = KMX_OK;
+    for (int prec : {KMX_PREC_FP64, KMX_PREC_FP32}) {
+        if (prec == KMX_PREC_FP32 && !p->d_blk32) continue;
+        const size_t elem = prec == KMX_PREC_FP64 ? sizeof(double) : sizeof(float);
+        if (!(prec == KMX_PREC_FP64 ? pick_f64(p->D, p->deg, 256, false) : pick_f32(p->D, p->deg, 256, false)))
+            continue;  // generic kernel only: nothing to choose
+        const bool fits = strip_fits(p, elem);
+        float best = 1e30f;
+        kmx_variant bestv = p->var[prec_slot(prec)];
+        for (int nt : {128, 256, 512})
+            for (int lds : {1, 0}) {
+                if (lds && !fits) continue;
+                kmx_variant v{nt, lds};
+                for (int w = 0; w < 2 && rc == KMX_OK; ++w) rc = launch_variant(p, false, x, y, prec, v, s);
+                float ms = 1e30f;
+                for (int round = 0; round < 2 && rc == KMX_OK; ++round) {  // best of two rounds
+                    hipEventRecord(e0, s);
+                    const int reps = 8;
+                    for (int r = 0; r < reps && rc == KMX_OK; ++r) rc = launch_variant(p, false, x, y, prec, v, s);
+                    hipEventRecord(e1, s);
+                    hipEventSynchronize(e1);
+                    float t = 0.f;
+                    hipEventElapsedTime(&t, e0, e1);
+                    ms = std::min(ms, t / reps);
+                }
+                if (rc != KMX_OK) break;
+                if (ms < best) {
+                    best = ms;
+                    bestv = v;
+                }
+            }
+        if (rc != KMX_OK) break;
+        p->var[prec_slot(prec)] = bestv;
+        std::lock_guard<std::mutex> g(tune_mutex());
+        tune_cache()[key_of(p, prec)] = bestv;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(x);
+    hipFree(y);
+    return rc;
+}
+
+extern "C" int kmx_plan_get_variant(const kmx_plan* p, int prec, int* nt, int* lds) {
+    if (!p || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
+        set_error("kmx_plan_get_variant: bad arguments");
+        return KMX_E_ARG;
+    }
+    const kmx_variant v = p->var[prec_slot(prec)];
+    const bool fits = strip_fits(p, prec == KMX_PREC_FP64 ? sizeof(double) : sizeof(float));
+    if (nt) *nt = v.nt > 0 ? v.nt : 256;
+    if (lds) *lds = (v.lds < 0) ? int(fits) : int(v.lds != 0 && fits);
     return KMX_OK;
 }
 
@@ -698,11 +334,8 @@ extern "C" int kmx_batch_create(kmx_plan* const* plans, int count, kmx_batch** o
     int dev = 0, cus = 256;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    int occ64 = 1, occ32 = 1;
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ64, reinterpret_cast<const void*>(k_apply_batch<double>), 256,
-                                                 size_t(4 * b->maxB) * sizeof(double));
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ32, reinterpret_cast<const void*>(k_apply_batch<float>), 256,
-                                                 size_t(4 * b->maxB) * sizeof(float));
+    const int occ64 = batch_occupancy_f64(size_t(4 * b->maxB) * sizeof(double));
+    const int occ32 = batch_occupancy_f32(size_t(4 * b->maxB) * sizeof(float));
     b->grid64 = std::max(1, std::min(b->nitems, cus * std::max(1, occ64)));
     b->grid32 = std::max(1, std::min(b->nitems, cus * std::max(1, occ32)));
     *out = b;
@@ -735,13 +368,11 @@ extern "C" int kmx_batch_apply(kmx_batch* b, const double* const* xs, double* co
                 return KMX_E_ARG;
             }
         }
-        if (prec == KMX_PREC_FP64)
-            hipLaunchKernelGGL(k_apply_batch<double>, dim3(b->grid64), dim3(256), size_t(4 * b->maxB) * sizeof(double),
-                               s, b->d_args64, b->d_items, b->nitems, b->d_shape, io);
-        else
-            hipLaunchKernelGGL(k_apply_batch<float>, dim3(b->grid32), dim3(256), size_t(4 * b->maxB) * sizeof(float),
-                               s, b->d_args32, b->d_items, b->nitems, b->d_shape, io);
-        hipError_t e = hipGetLastError();
+        hipError_t e = (prec == KMX_PREC_FP64)
+                           ? launch_batch_f64(b->d_args64, b->d_items, b->nitems, b->d_shape, io, b->grid64,
+                                              size_t(4 * b->maxB) * sizeof(double), s)
+                           : launch_batch_f32(b->d_args32, b->d_items, b->nitems, b->d_shape, io, b->grid32,
+                                              size_t(4 * b->maxB) * sizeof(float), s);
         if (e != hipSuccess) {
             set_error(std::string("kmx_batch_apply: ") + hipGetErrorString(e));
             return KMX_E_HIP;

@@ -1,0 +1,33 @@
+// kmx_apply_f64.hip -- FP64 instantiations of the operator-apply kernels
+// (the plain path: the reference's arithmetic, kmx_kernels.h).
+#include "kmx_kernels.h"
+
+namespace kmx {
+
+LaunchFn pick_f64(int D, int deg, int nt, bool lds) { return pick_impl<double>(D, deg, nt, lds); }
+
+hipError_t launch_generic_f64(const KArgs<double>& a, int D, int deg, int nrows, size_t smem, hipStream_t s) {
+    if (smem > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_apply_generic<double>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_apply_generic<double>, dim3(nrows), dim3(256), smem, s, a, D, deg);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_f64(const KArgs<double>* args, const int2* items, int nitems, const int* shape,
+                            const BatchIO& io, int grid, size_t smem, hipStream_t s) {
+    hipLaunchKernelGGL(k_apply_batch<double>, dim3(grid), dim3(256), smem, s, args, items, nitems, shape, io);
+    return hipGetLastError();
+}
+
+int batch_occupancy_f64(size_t smem) {
+    int occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_apply_batch<double>), 256,
+                                                     smem) != hipSuccess)
+        occ = 1;
+    return occ;
+}
+
+}  // namespace kmx

@@ -45,6 +45,13 @@ struct KArgs {
 
 }  // namespace kmx
 
+// Launch variant of the pair-by-pair kernels: workgroup size (0 = default
+// 256) and LDS staging of the row's coefficient strips (-1 = when they fit).
+struct kmx_variant {
+    int nt = 0;
+    int lds = -1;
+};
+
 struct kmx_plan {
     int D = 0, deg = 0, N = 0, T = 0, B = 0;
     long long P = 0;
@@ -56,6 +63,7 @@ struct kmx_plan {
     long long mat_elems = 0;  // total elements of all blocked matrices
     double scale[KMX_MAX_TERMS];
     int device = 0;
+    kmx_variant var[2];  // [0] FP64, [1] FP32 (pair-by-pair kernels)
 
     // forward operator
     int* d_idx1d = nullptr;
@@ -86,3 +94,4 @@ struct kmx_plan {
 
 // Launchers (kmx_apply.hip)
 int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, int prec, hipStream_t s);
+void kmx_plan_apply_cached_variants(kmx_plan* p);

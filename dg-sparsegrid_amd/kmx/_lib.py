@@ -86,6 +86,8 @@ PROTOTYPES = {
     "kmx_plan_update_term": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "kmx_plan_set_term_scale": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "kmx_plan_destroy": (C.c_int, [C.c_void_p]),
+    "kmx_plan_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kmx_plan_get_variant": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "kmx_plan_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
     "kmx_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "kmx_apply_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),

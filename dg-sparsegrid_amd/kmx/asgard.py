@@ -1,0 +1,135 @@
+"""Host-side mirror of ASGarD-MATLAB's operator-application interface.
+
+Same names and argument meaning as the reference, executing on the gfx950
+library:
+
+    ftmp = apply_A(pde, opts, A_data, f, deg, Vmax, Emax, imex_flag)   apply_A.m:1
+    f    = time_advance(pde, opts, A_data, f, t, dt, deg, hash_table)   time_advance.m:1
+    err, fval = asgard(pde_fn, **opts)                                  asgard.m:3 (RK3/BE subset)
+
+`pde` is a kmx.coeffs.PDE whose 1D matrices have been built with
+get_coeff_mats (stored as pde.mats[t][d]); `A_data` comes from
+kmx.grid.global_matrix; connectivity, when used, is pde.connectivity =
+(row_ptr, cols) from kmx.grid.connect_nd.  Vectors may be numpy arrays (a
+host round trip, like the MEX path) or float64 CUDA tensors (resident).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import coeffs as C
+from . import grid as G
+from . import timestep as TS
+from .plan import Plan
+
+
+@dataclass
+class OPTS:
+    """OPTS.m:5-41 defaults for the fields this path reads."""
+    lev: object = 3
+    deg: int = 2
+    CFL: float = 0.01
+    grid_type: str = "SG"
+    timestep_method: str = "RK3"
+    use_oldhash: bool = False
+    use_connectivity: bool = False
+    max_lev: int = 8
+    max_lev_coeffs: bool = True
+    num_steps: int = 5
+    quiet: bool = True
+    prec: str = "fp64"  # contraction precision of the device apply (not a reference option)
+    extra: dict = field(default_factory=dict)
+
+
+def _plan_for(pde, opts, A_data, deg, imex_flag):
+    """Build (and cache on the pde object) the resident plan for this operator."""
+    key = (id(A_data), deg, imex_flag, opts.use_connectivity)
+    cache = pde.__dict__.setdefault("_kmx_plans", {})
+    if key in cache:
+        return cache[key]
+    flags = getattr(pde, "imex", None) or ["N"] * len(pde.mats)
+    term_vec = [t for t in range(len(pde.mats)) if flags[t] == imex_flag]  # apply_A.m:23-30
+    mats = [pde.mats[t] for t in term_vec]
+    idx = np.asarray(A_data["element_local_index_D"])
+    rp = cols = None
+    if opts.use_connectivity:
+        rp, cols = pde.connectivity
+    plan = Plan(mats, idx, deg, rp, cols, index_base=1, fp32=True, adjoint=True) if mats else None
+    cache[key] = plan
+    return plan
+
+
+def apply_A(pde, opts, A_data, f, deg, Vmax=None, Emax=None, imex_flag="N"):
+    """apply_A.m:1-252, explicit branch: ftmp = A f over the pair list and the
+    terms whose imex flag matches.  Vmax/Emax are unused, as in the reference."""
+    import torch
+
+    plan = _plan_for(pde, opts, A_data, deg, imex_flag)
+    host = isinstance(f, np.ndarray)
+    x = torch.from_numpy(np.ascontiguousarray(f, dtype=np.float64)).cuda() if host else f
+    if plan is None:
+        y = torch.zeros_like(x)
+    else:
+        y = plan.apply(x, prec=opts.prec)
+    return y.cpu().numpy() if host else y
+
+
+def time_advance(pde, opts, A_data, f, t, dt, deg, hash_table=None, Vmax=None, Emax=None):
+    """time_advance.m:1-34 dispatch for 'RK3' (default) and 'BE'."""
+    import torch
+
+    host = isinstance(f, np.ndarray)
+    x = torch.from_numpy(np.ascontiguousarray(f, dtype=np.float64)).cuda() if host else f
+    src = pde.__dict__.get("_kmx_source")
+    if src is None and pde.sources:
+        src = TS.SeparableSource(pde, deg, np.asarray(A_data["element_local_index_D"]))
+        pde.__dict__["_kmx_source"] = src
+    if opts.timestep_method == "BE":
+        plan = _plan_for(pde, opts, A_data, deg, "N")
+        rhs = src(t + dt) if src is not None else None  # backward_euler: s(t+dt), time_advance.m:309-327
+        y, _, _ = TS.backward_euler(plan, x, dt, rhs, tol=opts.extra.get("be_tol", 1e-12))
+    else:
+        y = TS.rk3_step(lambda v: apply_A(pde, opts, A_data, v, deg), x, t, dt, src)
+    return y.cpu().numpy() if host else y
+
+
+def build(pde_fn, opts):
+    """Set-up of asgard_run_pde.m:20-70: element table, 1D matrices, A_data,
+    initial condition.  Returns (pde, hash_table, A_data, f0, dt)."""
+    pde = pde_fn(opts.lev)
+    lev_vec = [d.lev for d in pde.dims]
+    if opts.use_oldhash or opts.use_connectivity:
+        table = G.hash_table_nd(lev_vec, opts.grid_type)
+    else:
+        table = G.hash_table_sparse_nd(lev_vec, opts.max_lev, opts.grid_type)
+    if opts.use_connectivity:
+        L = max(lev_vec)
+        pde.connectivity = G.connect_nd(table, L, L, opts.grid_type)
+    mats = C.get_coeff_mats(pde, opts.deg, 0.0, opts.max_lev if opts.max_lev_coeffs else None)
+    pde.mats = C.restrict(mats, pde, opts.deg) if opts.max_lev_coeffs else mats
+    A_data = G.global_matrix(table)
+    f0 = C.md_eval_function(pde, opts.deg, table.idx1d, pde.initial_conditions, 0.0)
+    dt = pde.set_dt(pde, opts.CFL)
+    return pde, table, A_data, f0, dt
+
+
+def asgard(pde_fn, **kw):
+    """asgard.m:3 / asgard_run_pde.m time loop (RK3 or BE), FP64 state on the
+    device.  Returns (err, fval) with err the RMS wavelet-space error against
+    the analytic solution at the final time (asgard_run_pde.m:456-461) when
+    the PDE has one, else None."""
+    import torch
+
+    opts = OPTS(**kw)
+    pde, table, A_data, f0, dt = build(pde_fn, opts)
+    f = torch.from_numpy(f0).cuda()
+    t = 0.0
+    for _ in range(opts.num_steps):
+        f = time_advance(pde, opts, A_data, f, t, dt, opts.deg, table)
+        t += dt
+    fval = f.cpu().numpy()
+    err = None
+    if pde.solutions:
+        exact = C.md_eval_function(pde, opts.deg, table.idx1d, pde.solutions, t)
+        err = float(np.sqrt(np.mean((fval - exact) ** 2)))
+    return err, fval

@@ -215,3 +215,69 @@ def connect_nd(table, levsum, levmax, grid_type="SG", chunk=4096):
     row_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
     cols = np.concatenate(parts) if parts else np.zeros(0, np.int32)
     return row_ptr, cols
+
+
+# ------------------------------------------------------------ adaptivity
+def get_child_elements(lev_vec, pos_vec, max_lev, refinement_method=1):
+    """get_child_elements.m:59-121 (method 1): per dimension the first child
+    (lev+1, 2*pos) and, from level 1 on, the second (lev+1, 2*pos+1), when
+    lev+1 < max_lev.  Returns (lev array, pos array) of the children in order."""
+    if refinement_method != 1:
+        raise NotImplementedError("only refinement method 1 is provided")
+    lev_vec = list(int(v) for v in lev_vec)
+    pos_vec = list(int(v) for v in pos_vec)
+    levs, poss = [], []
+    for d in range(len(lev_vec)):
+        if lev_vec[d] + 1 < max_lev:
+            l2, p2 = list(lev_vec), list(pos_vec)
+            l2[d] += 1
+            p2[d] *= 2
+            levs.append(l2)
+            poss.append(p2)
+        if lev_vec[d] >= 1 and lev_vec[d] + 1 < max_lev:
+            l2, p2 = list(lev_vec), list(pos_vec)
+            l2[d] += 1
+            p2[d] = 2 * p2[d] + 1
+            levs.append(l2)
+            poss.append(p2)
+    nd = len(lev_vec)
+    return np.asarray(levs, dtype=np.int64).reshape(-1, nd), np.asarray(poss, dtype=np.int64).reshape(-1, nd)
+
+
+def lev_pos_to_id(lev, pos, max_lev):
+    """md_lev_pos_to_idx.m vectorised over rows (1-based ids)."""
+    lev = np.atleast_2d(lev)
+    pos = np.atleast_2d(pos)
+    idx = lev_cell_to_1d(lev, pos).astype(np.uint64)
+    stride = np.uint64(1) << np.uint64(max_lev)
+    acc = np.ones(lev.shape[0], dtype=np.uint64)
+    mul = np.uint64(1)
+    for d in range(lev.shape[1]):
+        acc += (idx[:, d] - np.uint64(1)) * mul
+        mul *= stride
+    return acc
+
+
+def refine_random(table, max_lev, keep_prob, rng):
+    """One adaptive refinement sweep with a seeded mask: every method-1 child
+    of every element that is not already present is kept with probability
+    keep_prob; new elements are appended in parent order (the order adapt.m
+    grows the element list).  Returns a new ElementTable (ids relative to max_lev)."""
+    have = set(lev_pos_to_id(table.lev, table.cell, max_lev).tolist())
+    new_lev, new_pos = [], []
+    for e in range(table.num_elem):
+        cl, cp = get_child_elements(table.lev[e], table.cell[e], max_lev)
+        if cl.shape[0] == 0:
+            continue
+        ids = lev_pos_to_id(cl, cp, max_lev)
+        keep = rng.random(cl.shape[0]) < keep_prob
+        for k in range(cl.shape[0]):
+            if keep[k] and int(ids[k]) not in have:
+                have.add(int(ids[k]))
+                new_lev.append(cl[k])
+                new_pos.append(cp[k])
+    if not new_lev:
+        return table
+    lev = np.vstack([table.lev, np.asarray(new_lev)])
+    cell = np.vstack([table.cell, np.asarray(new_pos)])
+    return ElementTable(lev, cell, lev_cell_to_1d(lev, cell), lev_pos_to_id(lev, cell, max_lev))

@@ -145,6 +145,16 @@ class Plan:
             raise ValueError("update_term: wrong matrix size")
         check(lib().kmx_plan_update_term(self.handle, int(t), int(d), m.ctypes.data_as(C.c_void_p), _stream(stream)))
 
+    def autotune(self, stream=None):
+        """Time the kernel variants on this plan and keep the fastest (kmx_plan_autotune)."""
+        check(lib().kmx_plan_autotune(self.handle, _stream(stream)))
+        return {p: self.variant(p) for p in ("fp64", "fp32")}
+
+    def variant(self, prec="fp64"):
+        nt, lds = C.c_int(), C.c_int()
+        check(lib().kmx_plan_get_variant(self.handle, _prec(prec), C.byref(nt), C.byref(lds)))
+        return {"threads": nt.value, "lds_staged": bool(lds.value)}
+
     def set_term_scale(self, t, alpha):
         check(lib().kmx_plan_set_term_scale(self.handle, int(t), float(alpha)))
 

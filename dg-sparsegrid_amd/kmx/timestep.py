@@ -1,0 +1,243 @@
+"""Time integration around the operator apply, on the device.
+
+* `SeparableSource`  -- md_eval_function.m sources are sums of separable terms
+  prod_d f_{s,d}(x_d) * g_s(t); the spatial parts are projected once
+  (forward_wavelet_transform.m, combine_dimensions_D.m) and only the scalar
+  g_s(t) is evaluated per stage.
+* `rk3_step`         -- time_advance.m:151-204 (Kutta RK3, a21=1/2, a31=-1,
+  a32=2, b=[1/6, 2/3, 1/6]); three applies per step.
+* `RK3Graph`         -- the same step captured once in a HIP graph and
+  replayed (inputs: the state and the stage source coefficients).
+* `backward_euler`   -- time_advance.m:267-350 solves (I - dt A) f1 = f0 +
+  dt (s + bc) with a direct solve; here matrix-free restarted GMRES in FP64
+  around mixed-precision inner solves, refined until the FP64 residual
+  ||(I - dt A) f1 - b|| / ||b|| reaches the requested tolerance.
+"""
+import math
+
+import numpy as np
+
+from . import coeffs as _coeffs
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class SeparableSource:
+    """sum_s g_s(t) * S_s with S_s = combine_dimensions_D(forward transforms)."""
+
+    def __init__(self, pde, deg, idx1d, funcs=None, device="cuda"):
+        torch = _torch()
+        funcs = pde.sources if funcs is None else funcs
+        nd = pde.ndim
+        self.g = [md[nd] for md in funcs]
+        spatial = []
+        for md in funcs:
+            fl = [_coeffs.forward_wavelet_transform(deg, pde.dims[d].lev, pde.dims[d].min, pde.dims[d].max, md[d],
+                                                    lambda x, d=d: pde.dims[d].moment_dV(x), pde.params, 0.0)
+                  for d in range(nd)]
+            spatial.append(_coeffs.combine_dimensions_D(deg, fl, 1.0, np.asarray(idx1d)))
+        self.S_host = np.stack(spatial) if spatial else np.zeros((0, np.asarray(idx1d).shape[0] * deg ** nd))
+        self.S = torch.from_numpy(self.S_host).to(device)
+        self.dof = self.S_host.shape[1]
+
+    def __len__(self):
+        return len(self.g)
+
+    def coeffs(self, t):
+        return np.asarray([float(np.asarray(g(t))) for g in self.g], dtype=np.float64)
+
+    def host(self, t):
+        out = np.zeros(self.dof)
+        for c, S in zip(self.coeffs(t), self.S_host):
+            out = out + S * c
+        return out
+
+    def into(self, out, coef):
+        """out = sum_s coef[s] * S_s (device; coef a device tensor), same order as host()."""
+        out.zero_()
+        for s in range(len(self.g)):
+            out.add_(self.S[s] * coef[s])
+        return out
+
+    def __call__(self, t):
+        torch = _torch()
+        out = torch.zeros(self.dof, dtype=torch.float64, device=self.S.device)
+        if len(self.g):
+            self.into(out, torch.from_numpy(self.coeffs(t)).to(self.S.device))
+        return out
+
+
+A21, A31, A32 = 0.5, -1.0, 2.0
+B1, B2, B3 = 1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0
+C2, C3 = 0.5, 1.0
+
+
+def rk3_step(apply, f, t, dt, source=None, bc=None):
+    """time_advance.m:151-204 (no LHS mass matrix).  apply(v) -> A v; source(t)
+    and bc(t) return vectors (or None)."""
+
+    def rhs(tt):
+        r = 0.0
+        if source is not None:
+            r = source(tt)
+        if bc is not None:
+            r = r + bc(tt)
+        return r
+
+    s1, s2, s3 = rhs(t), rhs(t + C2 * dt), rhs(t + C3 * dt)
+    k1 = apply(f) + s1
+    y1 = f + dt * A21 * k1
+    k2 = apply(y1) + s2
+    y2 = f + dt * (A31 * k1 + A32 * k2)
+    k3 = apply(y2) + s3
+    return f + dt * (B1 * k1 + B2 * k2 + B3 * k3)
+
+
+class RK3Graph:
+    """One RK3 step of y' = A y + sum_s g_s(t) S_s captured in a HIP graph.
+
+    Static device buffers hold the state and the 3 x n_src stage coefficients;
+    `step(t)` writes the coefficients, replays, and returns the new state
+    (in place).  The plan's kernels are launched on the capture stream."""
+
+    def __init__(self, plan, f0, dt, source=None, prec="fp64", warmup=2):
+        torch = _torch()
+        self.plan, self.dt, self.source, self.prec = plan, float(dt), source, prec
+        dev = f0.device
+        n = plan.dof
+        self.f = f0.clone()
+        self.k = [torch.empty(n, dtype=torch.float64, device=dev) for _ in range(3)]
+        self.y = torch.empty(n, dtype=torch.float64, device=dev)
+        self.s = torch.zeros(n, dtype=torch.float64, device=dev)
+        ns = len(source) if source is not None else 0
+        self.coef = torch.zeros((3, max(ns, 1)), dtype=torch.float64, device=dev)
+        self.graph = None
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(warmup):  # warm the kernels (first-launch set-up stays out of the graph)
+                self._body(dry=True)
+        torch.cuda.current_stream().wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._body(dry=False)
+
+    def _stage_rhs(self, i):
+        if self.source is not None and len(self.source):
+            return self.source.into(self.s, self.coef[i])
+        return self.s.zero_()
+
+    def _body(self, dry):
+        dt = self.dt
+        f, (k1, k2, k3), y = self.f, self.k, self.y
+        src = self.f.clone() if dry else f  # a dry run must not advance the state
+        self.plan.apply(src, k1, prec=self.prec)
+        k1.add_(self._stage_rhs(0))
+        torch = _torch()
+        torch.add(src, k1, alpha=dt * A21, out=y)
+        self.plan.apply(y, k2, prec=self.prec)
+        k2.add_(self._stage_rhs(1))
+        y.copy_((A31 * k1 + A32 * k2) * dt).add_(src)
+        self.plan.apply(y, k3, prec=self.prec)
+        k3.add_(self._stage_rhs(2))
+        upd = (B1 * k1 + B2 * k2 + B3 * k3) * dt
+        if not dry:
+            f.add_(upd)
+
+    def step(self, t):
+        torch = _torch()
+        if self.source is not None and len(self.source):
+            c = np.stack([self.source.coeffs(t), self.source.coeffs(t + C2 * self.dt),
+                          self.source.coeffs(t + C3 * self.dt)])
+            self.coef.copy_(torch.from_numpy(c))
+        self.graph.replay()
+        return self.f
+
+
+# ------------------------------------------------------------ implicit (BE)
+def gmres(matvec, b, x0=None, tol=1e-6, restart=40, maxiter=400, dot=None):
+    """Restarted GMRES (modified Gram-Schmidt, Givens rotations), FP64 on the
+    device.  Returns (x, relative residual, iterations)."""
+    torch = _torch()
+    n = b.numel()
+    x = torch.zeros_like(b) if x0 is None else x0.clone()
+    bnorm = float(torch.linalg.vector_norm(b))
+    if bnorm == 0.0:
+        return x, 0.0, 0
+    it = 0
+    rel = 1.0
+    while it < maxiter:
+        r = b - matvec(x)
+        beta = float(torch.linalg.vector_norm(r))
+        rel = beta / bnorm
+        if rel <= tol:
+            break
+        m = min(restart, maxiter - it)
+        V = torch.empty((m + 1, n), dtype=b.dtype, device=b.device)
+        H = np.zeros((m + 1, m))
+        cs, sn = np.zeros(m), np.zeros(m)
+        g = np.zeros(m + 1)
+        g[0] = beta
+        V[0] = r / beta
+        k_used = 0
+        for k in range(m):
+            w = matvec(V[k])
+            for j in range(k + 1):
+                H[j, k] = float(torch.dot(w, V[j]))
+                w = w - H[j, k] * V[j]
+            H[k + 1, k] = float(torch.linalg.vector_norm(w))
+            if H[k + 1, k] > 0:
+                V[k + 1] = w / H[k + 1, k]
+            for j in range(k):
+                tmp = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
+                H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
+                H[j, k] = tmp
+            den = math.hypot(H[k, k], H[k + 1, k])
+            cs[k], sn[k] = (1.0, 0.0) if den == 0 else (H[k, k] / den, H[k + 1, k] / den)
+            H[k, k] = cs[k] * H[k, k] + sn[k] * H[k + 1, k]
+            H[k + 1, k] = 0.0
+            g[k + 1] = -sn[k] * g[k]
+            g[k] = cs[k] * g[k]
+            it += 1
+            k_used = k + 1
+            if abs(g[k + 1]) / bnorm <= tol or it >= maxiter:
+                break
+        yk = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used]) if k_used else np.zeros(0)
+        x = x + torch.from_numpy(yk).to(b.device) @ V[:k_used]
+    r = b - matvec(x)
+    return x, float(torch.linalg.vector_norm(r)) / bnorm, it
+
+
+def backward_euler(plan, f0, dt, rhs=None, tol=1e-10, inner_prec="fp32", inner_tol=1e-5, restart=40,
+                   max_refine=20, max_inner=400):
+    """Solve (I - dt A) f1 = f0 + dt*rhs  (time_advance.m:328-349's system).
+
+    Iterative refinement: the FP64 residual r = b - (I - dt A) f1 (plain FP64
+    apply) drives a correction solved by GMRES with `inner_prec` applies to
+    relative accuracy inner_tol, until ||r|| / ||b|| <= tol.
+    Returns (f1, history of FP64 relative residuals, inner iterations)."""
+    b = f0.clone() if rhs is None else f0 + dt * rhs
+    bnorm = float(_torch().linalg.vector_norm(b))
+
+    def op64(v):
+        return v - dt * plan.apply(v, prec="fp64")
+
+    def op_in(v):
+        return v - dt * plan.apply(v.contiguous(), prec=inner_prec)
+
+    f = f0.clone()  # f0 is a good first guess for small dt
+    hist, inner = [], 0
+    for _ in range(max_refine):
+        r = b - op64(f)
+        rel = float(_torch().linalg.vector_norm(r)) / bnorm if bnorm else 0.0
+        hist.append(rel)
+        if rel <= tol:
+            break
+        d, _, its = gmres(op_in, r, tol=inner_tol, restart=restart, maxiter=max_inner)
+        inner += its
+        f = f + d
+    return f, hist, inner

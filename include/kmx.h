@@ -83,6 +83,17 @@ int kmx_plan_set_term_scale(kmx_plan* plan, int term, double alpha);
 
 int kmx_plan_destroy(kmx_plan* plan);
 
+/* Time the kernel variants (workgroup size 128/256/512, LDS staging of the
+ * row's coefficient strips on/off) on this plan for FP64 and FP32 and keep the
+ * fastest; plans of the same (d, deg, terms, pair density) created later in
+ * the process start from that choice.  Synchronous on `stream`; not for use
+ * under graph capture.  No counterpart in the reference (MATLAB picks nothing;
+ * its flops_kron{k} method choice is the only cost model it has). */
+int kmx_plan_autotune(kmx_plan* plan, void* stream);
+
+/* The variant a plan currently uses for prec (KMX_PREC_FP64 / KMX_PREC_FP32). */
+int kmx_plan_get_variant(const kmx_plan* plan, int prec, int* threads, int* lds_staged);
+
 /* num_pairs = P, block = deg^d, dof = N*deg^d. */
 int kmx_plan_info(const kmx_plan* plan, int64_t* num_pairs, int* block, int64_t* dof);
 

@@ -22,7 +22,7 @@ OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gold_fixtures.np
 
 # which fixtures pin which part of the path (see the test files for the use)
 PATTERNS = [
-    r"element_table/(table|ids)_[123]d_(SG|FG)$",
+    r"element_table/(table|ids|child_ids)_[123]d_(SG|FG)$",
     r"element_table/1d_index_\d+_\d+$",
     r"connectivity/connect_1_\d+$",
     r"connectivity/connect_n_2_3_FG_\d+$",

@@ -1,0 +1,159 @@
+"""CPU pins of the operator's input provider and callers against the
+reference's gold fixtures: FMWT, quadrature, DG coefficient matrices, the
+separable-source projection, and whole RK3 / backward-Euler trajectories
+computed with the oracle apply (apply_A.m + time_advance.m end to end).
+
+The gold trajectories (gold/generated-inputs/time_advance/*, gold_time_advance.m)
+were written by a reference revision whose continuity PDEs used the central
+flux (LF = 0); the current pdes/continuity2.m passes LF = -1.  The runs below
+therefore set LF = 0; the LF != 0 flux terms are exercised by the GPU parity
+tests through the same builder."""
+import numpy as np
+import pytest
+
+import oracle as O
+from kmx import coeffs as K
+from kmx import grid as G
+
+
+def central_flux(pde):
+    for tm in pde.terms:
+        for chain in tm:
+            for pt in chain:
+                if pt.type in ("div", "grad"):
+                    pt.LF = 0.0
+    return pde
+
+
+@pytest.mark.parametrize("key", ["2_2", "2_3", "2_6", "4_3", "5_5"])
+def test_fmwt(gold, key):
+    deg, lev = map(int, key.split("_"))
+    F = K.fmwt(deg, lev)
+    assert np.max(np.abs(F - gold[f"transformations/operator_two_scale_{key}"])) <= 1e-14
+    assert np.max(np.abs(F @ F.T - np.eye(F.shape[0]))) <= 1e-12  # orthogonal (the deg-5 filters reach 2.3e-13)
+
+
+def test_lgwt(gold):
+    for n, a, b, tag in [(10, -1, 1, "10_neg1_1"), (32, -5, 2, "32_neg5_2")]:
+        x, w = K.lgwt(n, a, b)
+        assert np.max(np.abs(x - gold[f"quadrature/lgwt_roots_{tag}"].ravel())) <= 1e-14
+        assert np.max(np.abs(w - gold[f"quadrature/lgwt_weights_{tag}"].ravel())) <= 1e-14
+
+
+def test_mass_factors_match_gold(gold):
+    """continuity2 lev [4,4] deg 3: the MASS factors (identity + roundoff)."""
+    pde = K.continuity2(4)
+    for t, d in [(0, 1), (1, 0)]:
+        g = gold[f"coefficients/continuity2_coefficients_l4_4_d3_{t + 1}_{d + 1}"]
+        m = K.sd_term_coeff_matrix(3, 1.0, pde.dims[d], pde.terms[t][d], pde.params, 4)
+        assert np.max(np.abs(m - g)) <= 1e-13
+
+
+def test_operator_identities():
+    """grad = -div^T (coeff_matrix.m:323) and level consistency of the
+    hierarchical basis (a matrix built at max_lev restricted to lev equals
+    the one built at lev)."""
+    dim = K.Dimension(-1, 1, 3)
+    pt = K.DIV(K._const(-1.0), -1.0, "P", "P")
+    _, div = K.coeff_matrix(3, 0.0, dim, pt, {}, 3)
+    g = K.GRAD(K._const(-1.0), 1.0, "P", "P")
+    _, grad = K.coeff_matrix(3, 0.0, dim, g, {}, 3)
+    assert np.allclose(grad, -div.T, atol=1e-12)
+    pde = K.continuity2(3)
+    m3 = K.get_coeff_mats(pde, 2, 0.0, None)
+    m8 = K.restrict(K.get_coeff_mats(pde, 2, 0.0, 8), pde, 2)
+    for a, b in zip(sum(m3, []), sum(m8, [])):
+        assert np.max(np.abs(a - b)) <= 1e-11 * max(1.0, np.max(np.abs(a)))
+
+
+def _setup(pde, deg, grid_type="SG", max_lev=8):
+    lev_vec = [d.lev for d in pde.dims]
+    tab = G.hash_table_sparse_nd(lev_vec, max_lev, grid_type)
+    mats = K.restrict(K.get_coeff_mats(pde, deg, 0.0, max_lev), pde, deg)
+    return tab, mats
+
+
+@pytest.mark.parametrize("name,pfn,lev,deg,grid", [
+    ("continuity2_sg_l2_d2", K.continuity2, 2, 2, "SG"),
+    ("continuity2_sg_l4_d3", K.continuity2, 4, 3, "SG"),
+    ("continuity2_fg_l2_d2", K.continuity2, 2, 2, "FG"),
+    ("continuity3_sg_l2_d2", K.continuity3, 2, 2, "SG"),
+])
+def test_rk3_trajectory_matches_gold(gold, name, pfn, lev, deg, grid):
+    """time_advance.m:151-204 with the oracle apply: states after 1..5 steps."""
+    pde = central_flux(pfn(lev))
+    tab, mats = _setup(pde, deg, grid)
+    inp = O.ApplyInputs(mats, tab.idx1d, deg)
+    A = lambda v: O.apply_A(inp, v)  # noqa: E731
+    src = lambda t: K.md_eval_function(pde, deg, tab.idx1d, pde.sources, t)  # noqa: E731
+    f = K.md_eval_function(pde, deg, tab.idx1d, pde.initial_conditions, 0.0)
+    dt = pde.set_dt(pde, 0.01)
+    for i in range(5):
+        t = i * dt
+        k1 = A(f) + src(t)
+        y1 = f + dt * 0.5 * k1
+        k2 = A(y1) + src(t + dt / 2)
+        y2 = f + dt * (-1.0 * k1 + 2.0 * k2)
+        k3 = A(y2) + src(t + dt)
+        f = f + dt * (k1 / 6 + 2 * k2 / 3 + k3 / 6)
+        g = gold[f"time_advance/{name}_t{i}"].ravel()
+        assert np.linalg.norm(f - g) <= 1e-13 * np.linalg.norm(g), i
+
+
+def test_backward_euler_matches_gold(gold):
+    """time_advance.m:267-349 (direct solve of (I - dt A) f1 = f0 + dt s(t+dt))."""
+    pde = central_flux(K.continuity2(2))
+    tab, mats = _setup(pde, 2)
+    A = O.build_A(O.ApplyInputs(mats, tab.idx1d, 2))
+    f = K.md_eval_function(pde, 2, tab.idx1d, pde.initial_conditions, 0.0)
+    dt = pde.set_dt(pde, 0.01)
+    Id = np.eye(A.shape[0])
+    for i in range(5):
+        f = np.linalg.solve(Id - dt * A, f + dt * K.md_eval_function(pde, 2, tab.idx1d, pde.sources, (i + 1) * dt))
+        g = gold[f"time_advance/continuity2_implicit_l2_d2_t{i}"].ravel()
+        assert np.linalg.norm(f - g) <= 1e-13 * np.linalg.norm(g), i
+
+
+def test_planted_flux_error_is_caught(gold):
+    """The upwind flux (LF = -1) does not reproduce the central-flux gold: the
+    trajectory check can tell the two apart."""
+    pde = K.continuity2(2)  # LF = -1
+    tab, mats = _setup(pde, 2)
+    inp = O.ApplyInputs(mats, tab.idx1d, 2)
+    f = np.zeros(tab.num_elem * 4)
+    dt = pde.set_dt(pde, 0.01)
+    src = lambda t: K.md_eval_function(pde, 2, tab.idx1d, pde.sources, t)  # noqa: E731
+    k1 = O.apply_A(inp, f) + src(0)
+    y1 = f + dt * 0.5 * k1
+    k2 = O.apply_A(inp, y1) + src(dt / 2)
+    y2 = f + dt * (-k1 + 2 * k2)
+    k3 = O.apply_A(inp, y2) + src(dt)
+    f = f + dt * (k1 / 6 + 2 * k2 / 3 + k3 / 6)
+    g = gold["time_advance/continuity2_sg_l2_d2_t0"].ravel()
+    assert np.linalg.norm(f - g) > 1e-4 * np.linalg.norm(g)
+
+
+@pytest.mark.parametrize("lv,ml,gt", [([7], 7, "SG"), ([5, 2], 7, "SG"), ([3, 2, 3], 7, "FG"), ([4, 4, 4], 8, "SG"),
+                                      ([2, 3, 2, 3, 3, 2], 4, "SG")])
+def test_product_element_tables_match_oracle(lv, ml, gt):
+    t = G.hash_table_sparse_nd(lv, ml, gt)
+    o = O.hash_table_sparse_nd(lv, ml, gt)
+    for k in ("lev", "cell", "idx1d", "ids"):
+        assert np.array_equal(getattr(t, k), o[k]), k
+
+
+@pytest.mark.parametrize("nd,lev,gt", [(2, 3, "SG"), (3, 4, "SG"), (2, 3, "FG"), (3, 7, "SG"), (6, 3, "SG")])
+def test_product_connectivity_matches_oracle(nd, lev, gt):
+    t = G.hash_table_nd([lev] * nd, gt)
+    o = O.hash_table_nd(nd, lev, gt)
+    assert np.array_equal(t.idx1d, o["idx1d"])
+    rp, cc = G.connect_nd(t, lev, lev, "SG")
+    orp, occ = O.connect_nd(nd, lev, gt, lev, lev)
+    assert np.array_equal(rp, orp) and np.array_equal(cc, occ)
+
+
+def test_product_connectivity_gold(gold):
+    t = G.hash_table_nd([3, 3], "FG")
+    rp, cc = G.connect_nd(t, 6, 3, "SG")
+    for i in range(64):
+        assert np.array_equal(gold[f"connectivity/connect_n_2_3_FG_{i + 1}"].ravel().astype(int), cc[rp[i]:rp[i + 1]])

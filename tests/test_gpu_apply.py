@@ -112,6 +112,26 @@ def test_cfg2_full_size_fp64(kmx_gpu):
     assert rel(z, 2 * y64 - 3 * yw) <= 1e-13
 
 
+def test_autotune_keeps_results(kmx_gpu):
+    """Every kernel variant the autotuner may pick computes the same sum."""
+    import torch
+
+    mats, idx, rp, cols, x = make_case(3, 4, 3, 3, True, seed=8)
+    plan = kmx_gpu.Plan(mats, idx, 3, rp, cols)
+    xd = torch.from_numpy(x).cuda()
+    before = {p: plan.apply(xd, prec=p).cpu().numpy() for p in ("fp64", "fp32")}
+    chosen = plan.autotune()
+    assert chosen["fp64"]["threads"] in (128, 256, 512)
+    yo = oracle_y(mats, idx, 3, rp, cols, x)
+    for p, tol in (("fp64", TOL64), ("fp32", TOL32)):
+        after = plan.apply(xd, prec=p).cpu().numpy()
+        assert rel(after, yo) <= tol
+        assert rel(after, before[p]) <= tol
+    # a new plan of the same shape starts from the tuned choice
+    plan2 = kmx_gpu.Plan(mats, idx, 3, rp, cols)
+    assert plan2.variant("fp64") == chosen["fp64"]
+
+
 @pytest.mark.parametrize("nd,lev,deg,conn", [(2, 3, 2, True), (3, 3, 3, True), (2, 3, 3, False), (4, 2, 3, True)])
 def test_transpose_matches_assembled(kmx_gpu, nd, lev, deg, conn):
     mats, idx, rp, cols, x = make_case(nd, lev, deg, 2, conn, seed=7 + nd)
