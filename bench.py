@@ -90,12 +90,52 @@ def cpu_baseline(op, budget_s=12.0):
                "sample": f"{n} full applies of the same workload (oracle/kmx_oracle.cpp, OpenMP over element rows)"}
 
 
+def batched_leg(steps=20, warmup=3):
+    """64 adapted 2D/3D advection-diffusion instances, one persistent batched
+    apply per RK3 stage, whole step replayed from a HIP graph; FP32 vs FP64."""
+    import torch
+
+    from kmx import batch as BT
+
+    t0 = time.perf_counter()
+    inst = BT.make_instances(count=64, seed=0)
+    build_s = time.perf_counter() - t0
+    out = {"instances": len(inst), "build_s": round(build_s, 1)}
+    states = {}
+    for prec in ("fp64", "fp32"):
+        sysm = BT.BatchedSystem(inst)
+        g = BT.BatchRK3Graph(sysm, prec=prec)
+        for _ in range(warmup):
+            g.step()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            g.step()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        states[prec] = sysm.f.cpu().numpy()
+        out[prec] = {"ms_per_rk3_step": round(ms, 4),
+                     "instance_applies_per_s": round(3 * len(inst) / (ms * 1e-3), 1),
+                     "TFLOPs": round(3 * sysm.flops / (ms * 1e-3) / 1e12, 3)}
+        out["pairs"] = int(sysm.pairs)
+        out["dof"] = int(sysm.dof)
+    d = states["fp32"] - states["fp64"]
+    out["rk3_rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(states["fp64"]))
+    out["speedup_fp32_vs_fp64"] = round(out["fp64"]["ms_per_rk3_step"] / out["fp32"]["ms_per_rk3_step"], 3)
+    out["steps"] = steps + warmup
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {a.gpus} requested); the operator apply is not "
@@ -164,6 +204,8 @@ def main():
         "speedup_vs_fp64": round(res["fp64"] / ms, 3),
         "roofline": roofline,
     }
+    if not a.no_batch:
+        out["batched"] = batched_leg()
     if not a.no_cpu:
         yo, cb = cpu_baseline(op)
         out["cpu_baseline"] = cb

@@ -59,6 +59,10 @@ def build(jobs=5, verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # the plain-C example caller of the ABI (host code only)
+    demo = os.path.join(LIBDIR, "kmx_demo")
+    subprocess.run(["gcc", "-O2", "-o", demo, os.path.join(HERE, "examples", "kmx_demo.c"), "-L" + LIBDIR, "-lkmx",
+                    "-Wl,-rpath,$ORIGIN", "-lm"], check=True)
     return LIB
 
 

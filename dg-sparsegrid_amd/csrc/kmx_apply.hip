@@ -66,7 +66,10 @@ static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* 
         const bool fits = strip_fits(p, elem);
         const bool lds = (v.lds < 0) ? fits : (v.lds != 0 && fits);
         const int nt = v.nt > 0 ? v.nt : 256;
-        LaunchFn f = f64 ? pick_f64(D, deg, nt, lds) : pick_f32(D, deg, nt, lds);
+        LaunchFn f = nullptr;
+        if (v.kind == 1) f = f64 ? pick_split_f64(D, deg, nt, lds) : pick_split_f32(D, deg, nt, lds);
+        if (!f) f = f64 ? pick_f64(D, deg, nt, lds) : pick_f32(D, deg, nt, lds);
+        if (!f) f = f64 ? pick_split_f64(D, deg, nt == 128 ? 256 : nt, lds) : pick_split_f32(D, deg, nt == 128 ? 256 : nt, lds);
         if (f) {
             size_t strip = 0;
             for (int k = 0; k < D; ++k) strip += size_t(p->nb[k]) * deg * deg;
@@ -188,15 +191,19 @@ extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
     for (int prec : {KMX_PREC_FP64, KMX_PREC_FP32}) {
         if (prec == KMX_PREC_FP32 && !p->d_blk32) continue;
         const size_t elem = prec == KMX_PREC_FP64 ? sizeof(double) : sizeof(float);
-        if (!(prec == KMX_PREC_FP64 ? pick_f64(p->D, p->deg, 256, false) : pick_f32(p->D, p->deg, 256, false)))
-            continue;  // generic kernel only: nothing to choose
+        const bool f64 = prec == KMX_PREC_FP64;
+        const bool has_reg = f64 ? pick_f64(p->D, p->deg, 256, false) : pick_f32(p->D, p->deg, 256, false);
+        const bool has_split = f64 ? pick_split_f64(p->D, p->deg, 256, false) : pick_split_f32(p->D, p->deg, 256, false);
+        if (!has_reg && !has_split) continue;  // generic kernel only: nothing to choose
         const bool fits = strip_fits(p, elem);
         float best = 1e30f;
         kmx_variant bestv = p->var[prec_slot(prec)];
+        for (int kind : {0, 1})
         for (int nt : {128, 256, 512})
             for (int lds : {1, 0}) {
                 if (lds && !fits) continue;
-                kmx_variant v{nt, lds};
+                if ((kind == 0 && !has_reg) || (kind == 1 && (!has_split || nt == 128))) continue;
+                kmx_variant v{nt, lds, kind};
                 for (int w = 0; w < 2 && rc == KMX_OK; ++w) rc = launch_variant(p, false, x, y, prec, v, s);
                 float ms = 1e30f;
                 for (int round = 0; round < 2 && rc == KMX_OK; ++round) {  // best of two rounds
@@ -228,35 +235,113 @@ extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
 }
 
 extern "C" int kmx_plan_get_variant(const kmx_plan* p, int prec, int* nt, int* lds) {
+    // (the lane-split kind is reported through kmx_plan_get_variant's nt sign: negative = split)
     if (!p || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
         set_error("kmx_plan_get_variant: bad arguments");
         return KMX_E_ARG;
     }
     const kmx_variant v = p->var[prec_slot(prec)];
     const bool fits = strip_fits(p, prec == KMX_PREC_FP64 ? sizeof(double) : sizeof(float));
-    if (nt) *nt = v.nt > 0 ? v.nt : 256;
+    if (nt) *nt = (v.nt > 0 ? v.nt : 256) * (v.kind == 1 ? -1 : 1);
     if (lds) *lds = (v.lds < 0) ? int(fits) : int(v.lds != 0 && fits);
     return KMX_OK;
 }
 
 // ------------------------------------------------------------------ batches
-struct kmx_batch {
-    std::vector<kmx_plan*> plans;
+// Per precision: the plans whose (d, deg) the persistent kernel's shape switch
+// holds in registers go into one work list (instance, row) sorted by cost;
+// the others are launched on their own after it.
+struct BatchSet {
     std::vector<int> in_kernel;  // plan indices served by the persistent kernel (slot order)
-    std::vector<int> solo;       // plans launched on their own (shape not in the switch)
-    KArgs<double>* d_args64 = nullptr;
-    KArgs<float>* d_args32 = nullptr;
+    std::vector<int> solo;       // plans launched on their own
+    void* d_args = nullptr;      // KArgs<T>[slots]
     int* d_shape = nullptr;
     int2* d_items = nullptr;
     int nitems = 0;
     int maxB = 1;
-    int grid64 = 0, grid32 = 0;
+    int grid = 0;
+};
+
+struct kmx_batch {
+    std::vector<kmx_plan*> plans;
+    BatchSet set[2];  // [0] FP64, [1] FP32
     bool has32 = true;
 };
 
+// The switch's register budget is that of its largest case, so only blocks of
+// up to 27 values share the persistent launch; larger blocks (which want the
+// lane-split kernels) run as their own autotuned launches.
 static bool batch_shape_ok(const kmx_plan* p, bool fp64) {
+    (void)fp64;
     if (p->D > 3 || p->deg > 4) return false;
-    return p->B * int(fp64 ? sizeof(double) : sizeof(float)) <= 256;
+    return p->B <= 27;
+}
+
+template <typename T>
+static hipError_t build_set(kmx_batch* b, BatchSet& bs, bool fp64) {
+    std::vector<KArgs<T>> args;
+    std::vector<int> shape;
+    struct Item {
+        double cost;
+        int slot, row;
+    };
+    std::vector<Item> items;
+    std::vector<long long> rp;
+    for (int i = 0; i < int(b->plans.size()); ++i) {
+        kmx_plan* p = b->plans[i];
+        if (!fp64 && !p->d_blk32) continue;
+        if (!batch_shape_ok(p, fp64) || int(bs.in_kernel.size()) >= KMX_BATCH_MAX) {
+            bs.solo.push_back(i);
+            continue;
+        }
+        const int slot = int(bs.in_kernel.size());
+        bs.in_kernel.push_back(i);
+        KArgs<T> a;
+        fill_args(a, p, false, nullptr, nullptr);
+        args.push_back(a);
+        shape.push_back(p->D * 16 + p->deg);
+        bs.maxB = std::max(bs.maxB, p->B);
+        rp.assign(size_t(p->N) + 1, 0);
+        if (p->all_pairs) {
+            for (int r = 0; r <= p->N; ++r) rp[r] = (long long)r * p->N;
+        } else if (p->N > 0) {
+            hipError_t e = hipMemcpy(rp.data(), p->d_row_ptr, sizeof(long long) * (p->N + 1), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return e;
+        }
+        const double unit = double(p->T) * p->B * p->D * p->deg;
+        for (int r = 0; r < p->N; ++r) items.push_back({double(rp[r + 1] - rp[r]) * unit, slot, r});
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
+    std::vector<int2> it(items.size());
+    for (size_t q = 0; q < items.size(); ++q) it[q] = make_int2(items[q].slot, items[q].row);
+    bs.nitems = int(it.size());
+    auto up = [](const void* h, size_t bytes, void** d) {
+        if (bytes == 0) return hipSuccess;
+        hipError_t e = hipMalloc(d, bytes);
+        if (e != hipSuccess) return e;
+        return hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice);
+    };
+    hipError_t e = hipSuccess;
+    if (!args.empty()) {
+        e = up(args.data(), args.size() * sizeof(KArgs<T>), &bs.d_args);
+        if (e == hipSuccess) e = up(shape.data(), shape.size() * sizeof(int), reinterpret_cast<void**>(&bs.d_shape));
+        if (e == hipSuccess) e = up(it.data(), it.size() * sizeof(int2), reinterpret_cast<void**>(&bs.d_items));
+    }
+    if (e != hipSuccess) return e;
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const size_t smem = size_t(4 * bs.maxB) * sizeof(T);
+    const int occ = fp64 ? batch_occupancy_f64(smem) : batch_occupancy_f32(smem);
+    bs.grid = std::max(1, std::min(bs.nitems, cus * std::max(1, occ)));
+    return hipSuccess;
+}
+
+static void free_set(BatchSet& bs) {
+    if (bs.d_args) hipFree(bs.d_args);
+    if (bs.d_shape) hipFree(bs.d_shape);
+    if (bs.d_items) hipFree(bs.d_items);
+    bs = BatchSet();
 }
 
 extern "C" int kmx_batch_create(kmx_plan* const* plans, int count, kmx_batch** out) {
@@ -266,78 +351,22 @@ extern "C" int kmx_batch_create(kmx_plan* const* plans, int count, kmx_batch** o
     }
     *out = nullptr;
     kmx_batch* b = new kmx_batch();
-    b->plans.assign(plans, plans + count);
-    std::vector<KArgs<double>> a64;
-    std::vector<KArgs<float>> a32;
-    std::vector<int> shape;
-    struct Item {
-        double cost;
-        int slot, row;
-    };
-    std::vector<Item> items;
-    std::vector<long long> rp;
     for (int i = 0; i < count; ++i) {
-        kmx_plan* p = plans[i];
-        if (!p) {
+        if (!plans[i]) {
             delete b;
             set_error("kmx_batch_create: null plan");
             return KMX_E_ARG;
         }
-        if (!p->d_blk32) b->has32 = false;
-        // a plan joins the persistent kernel only if both precisions fit its switch
-        if (!batch_shape_ok(p, true) || !batch_shape_ok(p, false) || int(b->in_kernel.size()) >= KMX_BATCH_MAX) {
-            b->solo.push_back(i);
-            continue;
-        }
-        const int slot = int(b->in_kernel.size());
-        b->in_kernel.push_back(i);
-        KArgs<double> x64;
-        fill_args(x64, p, false, nullptr, nullptr);
-        a64.push_back(x64);
-        KArgs<float> x32;
-        fill_args(x32, p, false, nullptr, nullptr);
-        a32.push_back(x32);
-        shape.push_back(p->D * 16 + p->deg);
-        b->maxB = std::max(b->maxB, p->B);
-        // per-row pair counts from the device row_ptr are not needed: recompute
-        rp.assign(size_t(p->N) + 1, 0);
-        if (p->all_pairs) {
-            for (int r = 0; r <= p->N; ++r) rp[r] = (long long)r * p->N;
-        } else if (p->N > 0) {
-            hipMemcpy(rp.data(), p->d_row_ptr, sizeof(long long) * (p->N + 1), hipMemcpyDeviceToHost);
-        }
-        const double unit = double(p->T) * p->B * p->D * p->deg;
-        for (int r = 0; r < p->N; ++r) items.push_back({double(rp[r + 1] - rp[r]) * unit, slot, r});
+        b->plans.push_back(plans[i]);
+        if (!plans[i]->d_blk32) b->has32 = false;
     }
-    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
-    std::vector<int2> it(items.size());
-    for (size_t q = 0; q < items.size(); ++q) it[q] = make_int2(items[q].slot, items[q].row);
-    b->nitems = int(it.size());
-    auto up = [](const void* h, size_t bytes, void** d) {
-        if (bytes == 0) return hipSuccess;
-        hipError_t e = hipMalloc(d, bytes);
-        if (e != hipSuccess) return e;
-        return hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice);
-    };
-    hipError_t e = hipSuccess;
-    if (!a64.empty()) {
-        e = up(a64.data(), a64.size() * sizeof(KArgs<double>), reinterpret_cast<void**>(&b->d_args64));
-        if (e == hipSuccess) e = up(a32.data(), a32.size() * sizeof(KArgs<float>), reinterpret_cast<void**>(&b->d_args32));
-        if (e == hipSuccess) e = up(shape.data(), shape.size() * sizeof(int), reinterpret_cast<void**>(&b->d_shape));
-        if (e == hipSuccess) e = up(it.data(), it.size() * sizeof(int2), reinterpret_cast<void**>(&b->d_items));
-    }
+    hipError_t e = build_set<double>(b, b->set[0], true);
+    if (e == hipSuccess) e = build_set<float>(b, b->set[1], false);
     if (e != hipSuccess) {
         set_error(std::string("kmx_batch_create: ") + hipGetErrorString(e));
         kmx_batch_destroy(b);
         return KMX_E_HIP;
     }
-    int dev = 0, cus = 256;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int occ64 = batch_occupancy_f64(size_t(4 * b->maxB) * sizeof(double));
-    const int occ32 = batch_occupancy_f32(size_t(4 * b->maxB) * sizeof(float));
-    b->grid64 = std::max(1, std::min(b->nitems, cus * std::max(1, occ64)));
-    b->grid32 = std::max(1, std::min(b->nitems, cus * std::max(1, occ32)));
     *out = b;
     return KMX_OK;
 }
@@ -357,28 +386,30 @@ extern "C" int kmx_batch_apply(kmx_batch* b, const double* const* xs, double* co
         set_error("kmx_batch_apply: unknown precision");
         return KMX_E_ARG;
     }
-    if (b->nitems > 0) {
+    for (size_t i = 0; i < b->plans.size(); ++i)
+        if (b->plans[i]->N > 0 && (!xs[i] || !ys[i])) {
+            set_error("kmx_batch_apply: null vector");
+            return KMX_E_ARG;
+        }
+    BatchSet& bs = b->set[prec == KMX_PREC_FP64 ? 0 : 1];
+    if (bs.nitems > 0) {
         BatchIO io;
         std::memset(&io, 0, sizeof(io));
-        for (size_t q = 0; q < b->in_kernel.size(); ++q) {
-            io.x[q] = xs[b->in_kernel[q]];
-            io.y[q] = ys[b->in_kernel[q]];
-            if (!io.x[q] || !io.y[q]) {
-                set_error("kmx_batch_apply: null vector");
-                return KMX_E_ARG;
-            }
+        for (size_t q = 0; q < bs.in_kernel.size(); ++q) {
+            io.x[q] = xs[bs.in_kernel[q]];
+            io.y[q] = ys[bs.in_kernel[q]];
         }
         hipError_t e = (prec == KMX_PREC_FP64)
-                           ? launch_batch_f64(b->d_args64, b->d_items, b->nitems, b->d_shape, io, b->grid64,
-                                              size_t(4 * b->maxB) * sizeof(double), s)
-                           : launch_batch_f32(b->d_args32, b->d_items, b->nitems, b->d_shape, io, b->grid32,
-                                              size_t(4 * b->maxB) * sizeof(float), s);
+                           ? launch_batch_f64(reinterpret_cast<const KArgs<double>*>(bs.d_args), bs.d_items, bs.nitems,
+                                              bs.d_shape, io, bs.grid, size_t(4 * bs.maxB) * sizeof(double), s)
+                           : launch_batch_f32(reinterpret_cast<const KArgs<float>*>(bs.d_args), bs.d_items, bs.nitems,
+                                              bs.d_shape, io, bs.grid, size_t(4 * bs.maxB) * sizeof(float), s);
         if (e != hipSuccess) {
             set_error(std::string("kmx_batch_apply: ") + hipGetErrorString(e));
             return KMX_E_HIP;
         }
     }
-    for (int i : b->solo) {
+    for (int i : bs.solo) {
         int rc = kmx_launch_apply(b->plans[i], false, xs[i], ys[i], prec, s);
         if (rc != KMX_OK) return rc;
     }
@@ -387,10 +418,8 @@ extern "C" int kmx_batch_apply(kmx_batch* b, const double* const* xs, double* co
 
 extern "C" int kmx_batch_destroy(kmx_batch* b) {
     if (!b) return KMX_OK;
-    if (b->d_args64) hipFree(b->d_args64);
-    if (b->d_args32) hipFree(b->d_args32);
-    if (b->d_shape) hipFree(b->d_shape);
-    if (b->d_items) hipFree(b->d_items);
+    free_set(b->set[0]);
+    free_set(b->set[1]);
     delete b;
     return KMX_OK;
 }

@@ -50,6 +50,7 @@ struct KArgs {
 struct kmx_variant {
     int nt = 0;
     int lds = -1;
+    int kind = 0;  // 0: one lane per pair, 1: pair split over deg lanes
 };
 
 struct kmx_plan {

@@ -110,6 +110,27 @@ struct Chain<T, D, DEG, USE_LDS, -1> {
                                                const RegArr<int, D>&) {}
 };
 
+// Contractions of dims D-1..1 of a lane's dim-1 slice (a (D-1)-dim block of
+// DEG^(D-1) values): dim k of the block is dim k-1 of the slice.
+template <typename T, int D, int DEG, bool USE_LDS, int K>
+struct SliceChain {
+    __device__ __forceinline__ static void run(RegArr<T, Pow<DEG, D - 1>::v>& v, const RowCtx<T, D, DEG, USE_LDS>& c,
+                                               int t, const RegArr<int, D>& J) {
+        T m[DEG * DEG];
+        c.load_block(t, K, J[K], m);
+        contract_dim<T, D - 1, DEG, K - 1>(v, m);
+        SliceChain<T, D, DEG, USE_LDS, K - 1>::run(v, c, t, J);
+    }
+};
+template <typename T, int D, int DEG, bool USE_LDS>
+struct SliceChain<T, D, DEG, USE_LDS, 0> {
+    __device__ __forceinline__ static void run(RegArr<T, Pow<DEG, D - 1>::v>&, const RowCtx<T, D, DEG, USE_LDS>&,
+                                               int, const RegArr<int, D>&) {}
+};
+
+template <typename T, int G, int BS, int SRC>
+struct GatherAll;
+
 // Copy `len` elements global -> LDS with 16-byte accesses when aligned.
 template <typename T, int NT>
 __device__ __forceinline__ void stage(T* __restrict__ dst, const T* __restrict__ src, int len, int tid) {
@@ -258,6 +279,207 @@ __global__ void __launch_bounds__(NT) k_apply_reg(KArgs<T> a) {
     apply_row<T, D, DEG, NT, USE_LDS, KEEPX>(a, a.x, a.y, row, smem);
 }
 
+// --------------------------------------------------------------- lane split
+// Blocks too large for one lane (64 values in FP64): G = DEG lanes share a
+// pair, lane r of the group holding the dim-1 slice b_1 = r (DEG^(D-1)
+// contiguous values, dim 1 being the slowest).  Dims D..2 contract inside the
+// slice; the dim-1 contraction reads the other slices of the group through
+// DPP quad permutes (G = 2 or 4 lanes of one quad), so no LDS traffic.
+// Value of lane (group base + SRC) for groups of G lanes inside a DPP quad:
+// quad_perm [SRC,SRC,SRC,SRC] for G = 4, [SRC,SRC,2+SRC,2+SRC] for G = 2.
+template <int G, int SRC>
+constexpr int quad_ctrl() {
+    return G == 4 ? SRC * 85 : (SRC | (SRC << 2) | ((2 + SRC) << 4) | ((2 + SRC) << 6));
+}
+
+template <typename T, int G, int SRC>
+__device__ __forceinline__ T quad_from(T v) {
+    constexpr int ctrl = quad_ctrl<G, SRC>();
+    if constexpr (sizeof(T) == 8) {
+        const long long bits = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_update_dpp(0, int(bits), ctrl, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, int(bits >> 32), ctrl, 0xF, 0xF, false);
+        return __builtin_bit_cast(T, (long long)(unsigned)lo | ((long long)hi << 32));
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), ctrl, 0xF, 0xF, false));
+    }
+}
+
+template <typename T, int G, int BS, int SRC>
+struct GatherFMA {
+    // out[q] += m[r][SRC] * (slice SRC)[q]
+    __device__ __forceinline__ static void run(RegArr<T, BS>& out, const RegArr<T, BS>& v, T coef) {
+#pragma unroll
+        for (int q = 0; q < BS; ++q) out[q] = madd(coef, quad_from<T, G, SRC>(v[q]), out[q]);
+    }
+};
+
+template <typename T, int G, int BS, int SRC>
+struct GatherAll {
+    __device__ __forceinline__ static void run(RegArr<T, BS>& out, const RegArr<T, BS>& v, const RegArr<T, G>& coef) {
+        GatherFMA<T, G, BS, SRC>::run(out, v, coef[SRC]);
+        GatherAll<T, G, BS, SRC + 1>::run(out, v, coef);
+    }
+};
+template <typename T, int BS>
+struct GatherAll<T, 2, BS, 2> {
+    __device__ __forceinline__ static void run(RegArr<T, BS>&, const RegArr<T, BS>&, const RegArr<T, 2>&) {}
+};
+template <typename T, int BS>
+struct GatherAll<T, 4, BS, 4> {
+    __device__ __forceinline__ static void run(RegArr<T, BS>&, const RegArr<T, BS>&, const RegArr<T, 4>&) {}
+};
+
+// Sum acc[BS] over the lanes of the wave that share the low LB lane bits (the
+// slice id) and store slice r's sums at red[r*BS + ..].
+template <typename T, int BP, int S, int K, int LB>
+struct HalveS {
+    __device__ __forceinline__ static void run(RegArr<T, BP>& v, int lane, int& idx) {
+        constexpr int mask = 32 >> S;
+        constexpr int half = BP >> (S + 1);
+        const bool hi = (lane & mask) != 0;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const T send = hi ? v[i] : v[i + half];
+            const T keep = hi ? v[i + half] : v[i];
+            v[i] = keep + __shfl_xor(send, mask, 64);
+        }
+        idx += hi ? half : 0;
+        HalveS<T, BP, S + 1, K, LB>::run(v, lane, idx);
+    }
+};
+template <typename T, int BP, int K, int LB>
+struct HalveS<T, BP, K, K, LB> {
+    __device__ __forceinline__ static void run(RegArr<T, BP>&, int, int&) {}
+};
+
+template <typename T, int BS, int LB>
+__device__ __forceinline__ void wave_reduce_store_slices(const RegArr<T, BS>& acc, T* red, int lane) {
+    constexpr int BP = next_pow2(BS);
+    constexpr int LEVELS = 6 - LB;  // lane bits to reduce over
+    constexpr int K = ilog2(BP) < LEVELS ? ilog2(BP) : LEVELS;
+    T v[BP];
+#pragma unroll
+    for (int e = 0; e < BP; ++e) v[e] = (e < BS) ? acc[e] : T(0);
+    int idx = 0;
+    HalveS<T, BP, 0, K, LB>::run(v, lane, idx);
+    // after K halvings each lane holds BP >> K entries; finish with plain steps
+    constexpr int LIVE = BP >> K;
+#pragma unroll
+    for (int s = K; s < LEVELS; ++s)
+#pragma unroll
+        for (int i = 0; i < LIVE; ++i) v[i] += __shfl_xor(v[i], 32 >> s, 64);
+    constexpr int plain = ((64 >> K) - 1) & ~((1 << LB) - 1);
+    const int r = lane & ((1 << LB) - 1);
+    if ((lane & plain) == 0) {
+#pragma unroll
+        for (int i = 0; i < LIVE; ++i)
+            if (idx + i < BS) red[r * BS + idx + i] = v[i];
+    }
+}
+
+template <typename T, int D, int DEG, int NT, bool USE_LDS>
+__device__ __forceinline__ void apply_row_split(const KArgs<T>& a, const double* __restrict__ x,
+                                                double* __restrict__ y, int row, unsigned char* smem) {
+    constexpr int B = Pow<DEG, D>::v;
+    constexpr int BS = B / DEG;  // slice of one lane
+    constexpr int G = DEG;
+    constexpr int LB = ilog2(G);
+    constexpr int NW = NT / 64;
+    static_assert(G == 2 || G == 4, "lane split needs 2 or 4 lanes per pair");
+    const int tid = threadIdx.x;
+    const int r = tid & (G - 1);
+
+    RowCtx<T, D, DEG, USE_LDS> c;
+    c.a = &a;
+#pragma unroll
+    for (int k = 0; k < D; ++k) c.I[k] = a.idx1d[(long long)row * D + k];
+    if (USE_LDS) {
+        T* strip = reinterpret_cast<T*>(smem);
+        int so = 0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            c.soff[k] = so;
+            so += a.nb[k] * DEG * DEG;
+        }
+        c.sterm = so;
+        for (int t = 0; t < a.nterms; ++t) {
+#pragma unroll
+            for (int k = 0; k < D; ++k)
+                stage<T, NT>(strip + t * so + c.soff[k], a.blk + a.off[t * D + k] + (long long)c.I[k] * a.nb[k] * DEG * DEG,
+                             a.nb[k] * DEG * DEG, tid);
+        }
+        __syncthreads();
+        c.strip = strip;
+    }
+    long long pb, pe;
+    if (a.row_ptr) {
+        pb = a.row_ptr[row];
+        pe = a.row_ptr[row + 1];
+    } else {
+        pb = 0;
+        pe = a.N;
+    }
+    T acc[BS];
+#pragma unroll
+    for (int q = 0; q < BS; ++q) acc[q] = T(0);
+    // every lane of a group runs the same trip count (the DPP reads need the
+    // partner lanes active); lanes past the end work on a clamped pair and
+    // drop the result
+    const long long npairs = pe - pb;
+    const long long iters = (npairs + (NT / G) - 1) / (NT / G);
+    for (long long it = 0; it < iters; ++it) {
+        const long long qq = it * (NT / G) + tid / G;
+        const bool live = qq < npairs;
+        const long long q = pb + (live ? qq : 0);
+        if (npairs == 0) break;
+        const int j = a.row_ptr ? a.cols[q] : int(q);
+        int J[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) J[k] = a.idx1d[(long long)j * D + k];
+        const double* xs = x + (long long)j * B + r * BS;
+        for (int t = 0; t < a.nterms; ++t) {
+            T v[BS];
+#pragma unroll
+            for (int e = 0; e < BS; ++e) v[e] = T(xs[e]);
+            // dims D..2 inside the slice (a (D-1)-dim block)
+            SliceChain<T, D, DEG, USE_LDS, D - 1>::run(v, c, t, J);
+            // dim 1 across the group
+            T m[DEG * DEG];
+            c.load_block(t, 0, J[0], m);
+            // acc[r-slice] += scale_t * sum_b m[r][b] * (slice b): the term scale
+            // is folded into the coefficients so no extra slice buffer is live
+            const T s = live ? a.scale[t] : T(0);
+            T coef[G];
+#pragma unroll
+            for (int b = 0; b < G; ++b) coef[b] = s * m[0 * DEG + b];
+#pragma unroll
+            for (int a0 = 1; a0 < G; ++a0)
+                if (r == a0)
+#pragma unroll
+                    for (int b = 0; b < G; ++b) coef[b] = s * m[a0 * DEG + b];
+            GatherAll<T, G, BS, 0>::run(acc, v, coef);
+        }
+    }
+    if (USE_LDS) __syncthreads();
+    T* red = reinterpret_cast<T*>(smem);
+    const int lane = tid & 63, w = tid >> 6;
+    wave_reduce_store_slices<T, BS, LB>(acc, red + w * B, lane);
+    __syncthreads();
+    for (int e = tid; e < B; e += NT) {
+        T sum = red[e];
+#pragma unroll
+        for (int ww = 1; ww < NW; ++ww) sum += red[ww * B + e];
+        y[(long long)row * B + e] = double(sum);
+    }
+}
+
+template <typename T, int D, int DEG, int NT, bool USE_LDS>
+__global__ void __launch_bounds__(NT) k_apply_split(KArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    apply_row_split<T, D, DEG, NT, USE_LDS>(a, a.x, a.y, a.row_order[blockIdx.x], smem);
+}
+
 // ------------------------------------------------------------ many instances
 // One persistent launch over a work list of (instance, row) items, heaviest
 // first, striding by the grid; each instance keeps its own plan (levels, deg,
@@ -271,7 +493,7 @@ struct BatchIO {
 
 template <typename T, int D, int DEG>
 __device__ __forceinline__ void batch_row(const KArgs<T>& a, const double* x, double* y, int row, unsigned char* sm) {
-    if constexpr (Pow<DEG, D>::v * int(sizeof(T)) <= 256) {
+    if constexpr (Pow<DEG, D>::v <= 27) {
         constexpr bool KX = 3 * Pow<DEG, D>::v * int(sizeof(T)) / 4 <= 144;
         apply_row<T, D, DEG, 256, false, KX>(a, x, y, row, sm);
     }
@@ -298,7 +520,6 @@ __global__ void __launch_bounds__(256) k_apply_batch(const KArgs<T>* __restrict_
             case 0x31: batch_row<T, 3, 1>(a, x, y, w.y, smem); break;
             case 0x32: batch_row<T, 3, 2>(a, x, y, w.y, smem); break;
             case 0x33: batch_row<T, 3, 3>(a, x, y, w.y, smem); break;
-            case 0x34: batch_row<T, 3, 4>(a, x, y, w.y, smem); break;
             default: break;
         }
         __syncthreads();
@@ -426,9 +647,41 @@ LaunchFn pick_impl(int D, int deg, int nt, bool lds) {
     }
 }
 
+template <typename T, int D, int DEG, int NT, bool USE_LDS>
+hipError_t launch_split(const void* args, int nrows, size_t smem, hipStream_t s) {
+    auto kern = k_apply_split<T, D, DEG, NT, USE_LDS>;
+    if (smem > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(nrows), dim3(NT), smem, s, *reinterpret_cast<const KArgs<T>*>(args));
+    return hipGetLastError();
+}
+
+template <typename T, int D, int DEG>
+LaunchFn pick_split_shape(int nt, bool lds) {
+    switch (nt) {
+        case 256: return lds ? &launch_split<T, D, DEG, 256, true> : &launch_split<T, D, DEG, 256, false>;
+        case 512: return lds ? &launch_split<T, D, DEG, 512, true> : &launch_split<T, D, DEG, 512, false>;
+        default: return nullptr;
+    }
+}
+
+// shapes with 32-64 value blocks split over 2 or 4 lanes
+template <typename T>
+LaunchFn pick_split_impl(int D, int deg, int nt, bool lds) {
+    if (D == 6 && deg == 2) return pick_split_shape<T, 6, 2>(nt, lds);
+    if (D == 5 && deg == 2) return pick_split_shape<T, 5, 2>(nt, lds);
+    if (D == 3 && deg == 4) return pick_split_shape<T, 3, 4>(nt, lds);
+    return nullptr;
+}
+
 // per-precision entry points (instantiated in kmx_apply_f64.hip / _f32.hip)
 LaunchFn pick_f64(int D, int deg, int nt, bool lds);
 LaunchFn pick_f32(int D, int deg, int nt, bool lds);
+LaunchFn pick_split_f64(int D, int deg, int nt, bool lds);
+LaunchFn pick_split_f32(int D, int deg, int nt, bool lds);
 LaunchFn pick_grouped(int D, int deg, int T, int nt);
 hipError_t launch_generic_f64(const KArgs<double>& a, int D, int deg, int nrows, size_t smem, hipStream_t s);
 hipError_t launch_generic_f32(const KArgs<float>& a, int D, int deg, int nrows, size_t smem, hipStream_t s);

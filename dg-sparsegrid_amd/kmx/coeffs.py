@@ -425,19 +425,22 @@ def diffusion_nd(nd=2, lev=3):
     return PDE(dims, terms, [], sol, sol, {"parameter1": 0, "parameter2": 1}, set_dt, f"diffusion{nd}")
 
 
-def advection_diffusion_varcoeff(nd=3, lev=3):
-    """Variable-coefficient advection-diffusion: per dim an upwind
-    DIV(g_i, LF=-1, P/P) term and an LDG diffusion chain, with
-    g_i(x) = 1 + 0.5 sin(pi x_i) in the DIV/GRAD pterms (coeff_matrix.m:146)."""
+def advection_diffusion_varcoeff(nd=3, lev=3, nu=0.02):
+    """Variable-coefficient advection-diffusion on [-1,1]^nd, periodic: per dim
+    an upwind DIV(-g_i, LF=-1) term and an LDG diffusion chain
+    DIV(nu g_i, LF=+1) * GRAD(1, LF=-1), with g_i(x) = 1 + 0.5 sin(pi x_i) in
+    the pterms (coeff_matrix.m:146).  nu keeps the explicit RK3 step of the
+    advective CFL stable (|lambda_diff| ~ nu * 2.8e6 at lev 5, deg 3)."""
     levs = np.broadcast_to(np.asarray(lev), (nd,))
     dims = [Dimension(-1, 1, int(levs[d]), name=f"x{d + 1}") for d in range(nd)]
     gvar = lambda x, p=None, t=None, dat=None: 1.0 + 0.5 * np.sin(np.pi * np.asarray(x, dtype=np.float64))  # noqa: E731
     gneg = lambda x, p=None, t=None, dat=None: -gvar(x)  # noqa: E731
+    gdif = lambda x, p=None, t=None, dat=None: nu * gvar(x)  # noqa: E731
     terms = []
     for k in range(nd):
         terms.append([[DIV(gneg, -1, "P", "P")] if d == k else [MASS()] for d in range(nd)])
     for k in range(nd):
-        terms.append([[DIV(gvar, +1, "P", "P"), GRAD(_const(1), -1, "P", "P")] if d == k else [MASS(), MASS()]
+        terms.append([[DIV(gdif, +1, "P", "P"), GRAD(_const(1), -1, "P", "P")] if d == k else [MASS(), MASS()]
                       for d in range(nd)])
     bump = lambda x, p, t: np.exp(-4 * np.asarray(x) ** 2)  # noqa: E731
     ic = [[bump] * nd + [lambda t: 1.0]]

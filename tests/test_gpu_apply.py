@@ -132,6 +132,24 @@ def test_autotune_keeps_results(kmx_gpu):
     assert plan2.variant("fp64") == chosen["fp64"]
 
 
+@pytest.mark.parametrize("nd,lev,deg,conn", [(6, 2, 2, True), (5, 2, 2, True), (3, 2, 4, False)])
+def test_lane_split_variants(kmx_gpu, nd, lev, deg, conn):
+    """Blocks split over deg lanes (DPP exchange): every variant the autotuner
+    can pick, both precisions, forward and transpose, against the oracle."""
+    import torch
+
+    mats, idx, rp, cols, x = make_case(nd, lev, deg, 3, conn, seed=60 + nd)
+    yo = oracle_y(mats, idx, deg, rp, cols, x)
+    plan = kmx_gpu.Plan(mats, idx, deg, rp, cols, adjoint=True)
+    chosen = plan.autotune()
+    xd = torch.from_numpy(x).cuda()
+    assert rel(plan.apply(xd, prec="fp64").cpu().numpy(), yo) <= TOL64, chosen
+    assert rel(plan.apply(xd, prec="fp32").cpu().numpy(), yo) <= TOL32, chosen
+    A = O.build_A(O.ApplyInputs(mats, idx, deg, rp, cols)) if plan.dof <= 6000 else None
+    if A is not None:
+        assert rel(plan.apply_transpose(xd, prec="fp64").cpu().numpy(), A.T @ x) <= TOL64
+
+
 @pytest.mark.parametrize("nd,lev,deg,conn", [(2, 3, 2, True), (3, 3, 3, True), (2, 3, 3, False), (4, 2, 3, True)])
 def test_transpose_matches_assembled(kmx_gpu, nd, lev, deg, conn):
     mats, idx, rp, cols, x = make_case(nd, lev, deg, 2, conn, seed=7 + nd)
@@ -287,3 +305,18 @@ def test_batch_matches_single_plans(kmx_gpu):
         torch.cuda.synchronize()
         for y, r in zip(ys, refs):
             assert rel(y.cpu().numpy(), r) <= tol, prec
+
+
+def test_plain_c_caller(kmx_gpu):
+    """dg-sparsegrid_amd/examples/kmx_demo.c: a C program (the shape of a MEX
+    gateway) drives the ABI with host arrays and checks against its own
+    direct sum."""
+    import os
+    import subprocess
+
+    from conftest import PKG
+
+    exe = os.path.join(PKG, "lib", "kmx_demo")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "kmx_demo ok" in r.stdout
