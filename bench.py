@@ -29,6 +29,20 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
+# HBM bytes per launch of the pair-by-pair kernel on this workload, by
+# (precision, workgroup size, LDS-staged strips): rocprofv3 FETCH_SIZE and
+# WRITE_SIZE counter runs of `bench.py --pmc-pass`, fetch scaled by 2.0 and
+# write by 1.0 (what the same counters read for a 256 MiB device copy).
+# Source: profiles/apply_cfg2_hbm_traffic.md.
+MEASURED_TRAFFIC = {
+    ("fp64", 128, False): 18585550, ("fp64", 128, True): 31532832,
+    ("fp64", 256, False): 18546674, ("fp64", 256, True): 31547371,
+    ("fp64", 512, False): 18522944, ("fp64", 512, True): 31535202,
+    ("fp32", 128, False): 15928128, ("fp32", 128, True): 21634979,
+    ("fp32", 256, False): 15885653, ("fp32", 256, True): 21634832,
+    ("fp32", 512, False): 15846711, ("fp32", 512, True): 21612037,
+}
+
 
 def build_operator(seed=2):
     """cfg2-sized operator in the reference's conventions (old hash + connect_nD)."""
@@ -136,6 +150,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched leg")
+    ap.add_argument("--pmc-pass", action="store_true",
+                    help="for counter runs: one 256 MiB device copy (calibration), then 3 applies per precision")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {a.gpus} requested); the operator apply is not "
@@ -148,37 +164,54 @@ def main():
 
     torch.cuda.set_device(0)
     op = build_operator()
-    plan = kmx.Plan(op["mats"], op["table"].idx1d, op["deg"], op["rp"], op["cols"], index_base=1, fp32=True)
     xd = torch.from_numpy(op["x"]).cuda()
     yd = torch.empty_like(xd)
-    variants = plan.autotune()  # workgroup size / LDS staging per precision, timed on this plan
+    # one Kronecker chain per element pair, the reference's order of operations
+    plan = kmx.Plan(op["mats"], op["table"].idx1d, op["deg"], op["rp"], op["cols"], index_base=1, fp32=True)
+    tuned = plan.autotune()  # workgroup size / LDS staging / lane split per precision, timed on this plan
+    if a.pmc_pass:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        for prec in ("fp64", "fp32"):
+            for _ in range(3):
+                plan.apply(xd, yd, prec=prec)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_pass": True, "copy_bytes": src.numel() * 8, "tuned": tuned}), flush=True)
+        return
 
-    res = {}
-    for prec in ("fp64", "fp32", "fp32_grouped"):
-        res[prec] = time_apply(plan, xd, yd, prec, a.warmup, a.steps)
+    runs = {"fp64": "fp64", "fp32": "fp32", "fp32_grouped": "fp32_grouped"}
+    res, err, flops = {}, {}, {}
+    for name, prec in runs.items():
+        res[name] = time_apply(plan, xd, yd, prec, a.warmup, a.steps)
     y64 = plan.apply(xd, prec="fp64").cpu().numpy()
-    err = {}
-    for prec in ("fp32", "fp32_grouped"):
+    fl_pairs, _ = work_model(op, plan.P, "fp64")
+    for name, prec in runs.items():
         yp = plan.apply(xd, prec=prec).cpu().numpy()
-        err[prec] = float(np.linalg.norm(yp - y64) / np.linalg.norm(y64))
-    best = min(("fp32", "fp32_grouped"), key=lambda p: res[p])
+        err[name] = float(np.linalg.norm(yp - y64) / np.linalg.norm(y64))
+        flops[name] = fl_pairs
+    mixed = ("fp32", "fp32_grouped")
+    best = min(mixed, key=lambda n: res[n])
     ms = res[best]
     value = 1000.0 / ms
 
-    flops, _ = work_model(op, plan.P, best)
-    flops64, streamed64 = work_model(op, plan.P, "fp64")
-    _, streamed_best = work_model(op, plan.P, best)
-    achieved = flops / (ms * 1e-3) / 1e12
+    achieved = flops[best] / (ms * 1e-3) / 1e12
+    traffic = None
+    if best == "fp32":
+        v = tuned["fp32"]
+        traffic = MEASURED_TRAFFIC.get(("fp32", abs(v["threads"]), v["lds_staged"])) if v["threads"] > 0 else None
+    kern = {"fp32": "k_apply_reg", "fp32_grouped": "k_apply_grouped"}[best]
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved, 3),
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-        "traffic": None,
-        "kernel": f"k_apply_{'grouped' if best == 'fp32_grouped' else 'reg'} (fp32, d=3, deg=3)",
-        "streamed_model_GBs": round(streamed_best / (ms * 1e-3) / 1e9, 1),
-        "streamed_model_frac_hbm": round(streamed_best / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "traffic_GBs": round(traffic / (ms * 1e-3) / 1e9, 1) if traffic else None,
+        "kernel": f"{kern} (fp32, d=3, deg=3): 2 d deg^(d+1) flops per pair and term / apply time",
+        "flops_per_apply": flops[best],
     }
     out = {
         "metric": METRIC,
@@ -194,13 +227,10 @@ def main():
         "dtype": "f32",
         "data": f"synthetic: x ~ N(0,1) seed 2; 1D operators {op['source']}",
         "config": {"workload": "3D SG lev7 deg3, 3 terms, connect_nD pairs (P=%d, N=%d)" % (plan.P, plan.N),
-                   "variant": best, "block": plan.B, "tuned": variants},
-        "fp64_plain": {"applies_per_s": round(1000.0 / res["fp64"], 2), "ms_per_apply": round(res["fp64"], 5),
-                       "achieved_TFLOPs": round(flops64 / (res["fp64"] * 1e-3) / 1e12, 3),
-                       "frac_fp64_peak": round(flops64 / (res["fp64"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
-                       "streamed_model_GBs": round(streamed64 / (res["fp64"] * 1e-3) / 1e9, 1)},
-        "mixed": {p: {"applies_per_s": round(1000.0 / res[p], 2), "rel_err_vs_fp64": err[p]}
-                  for p in ("fp32", "fp32_grouped")},
+                   "variant": best, "block": plan.B, "tuned": tuned},
+        "runs": {n: {"applies_per_s": round(1000.0 / res[n], 2), "ms_per_apply": round(res[n], 5),
+                     "rel_err_vs_fp64": err[n], "flops_per_apply": flops[n],
+                     "TFLOPs": round(flops[n] / (res[n] * 1e-3) / 1e12, 3)} for n in runs},
         "speedup_vs_fp64": round(res["fp64"] / ms, 3),
         "roofline": roofline,
     }

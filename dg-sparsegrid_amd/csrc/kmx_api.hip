@@ -66,6 +66,7 @@ void free_plan(kmx_plan* p) {
     dfree(p->d_blkT32);
     dfree(p->d_x_host_stage);
     dfree(p->d_y_host_stage);
+    kmx_blocks_free(p);
 }
 
 // Block-row-major copy of one column-major n x n matrix (see KArgs in
@@ -295,7 +296,13 @@ int kmx_plan_create(const kmx_plan_desc* desc, kmx_plan** out) {
                 p->d_rows = upload(rows.data(), rows.size());
             }
         }
+        if (desc->flags & KMX_PLAN_LEVEL_BLOCKS) {
+            std::vector<const double*> mats(desc->mats, desc->mats + size_t(T) * D);
+            kmx_blocks_build(p, idx, std::move(rp), std::move(cols), mats.data());
+        }
         kmx_plan_apply_cached_variants(p);
+        const int rc = kmx_plan_init_splits(p);
+        if (rc != KMX_OK) throw HipError(kmx_last_error());
         *out = p;
         return KMX_OK;
     } catch (const std::exception& e) {
@@ -340,6 +347,7 @@ int kmx_plan_update_term(kmx_plan* p, int term, int dim, const double* mat, void
         }
         // the float staging vectors die here: wait for the copies that read them
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        kmx_blocks_update_term(p, term, dim, mat, s);
         return KMX_OK;
     } catch (const std::exception& e) {
         set_error(std::string("kmx_plan_update_term: ") + e.what());

@@ -5,6 +5,8 @@
 #include "kmx_kernels.h"
 
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -21,7 +23,8 @@ static size_t lds_limit() {
 }
 
 template <typename T>
-static void fill_args(KArgs<T>& a, const kmx_plan* p, bool transpose, const double* x, double* y) {
+static void fill_args(KArgs<T>& a, const kmx_plan* p, bool transpose, const double* x, double* y,
+                      const kmx_pairs* pl = nullptr) {
     std::memset(&a, 0, sizeof(a));
     if (std::is_same<T, double>::value)
         a.blk = reinterpret_cast<const T*>(transpose ? p->d_blkT64 : p->d_blk64);
@@ -34,6 +37,11 @@ static void fill_args(KArgs<T>& a, const kmx_plan* p, bool transpose, const doub
     a.row_ptr = transpose ? p->d_col_ptr : p->d_row_ptr;
     a.cols = transpose ? p->d_rows : p->d_cols;
     a.row_order = transpose ? p->d_col_order : p->d_row_order;
+    if (pl) {
+        a.row_ptr = pl->rp;
+        a.cols = pl->cols;
+        a.row_order = pl->order;
+    }
     a.x = x;
     a.y = y;
     a.N = p->N;
@@ -53,7 +61,7 @@ static bool strip_fits(const kmx_plan* p, size_t elem) {
 
 // One launch of the pair-by-pair kernel family with an explicit variant.
 static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* y, int prec, kmx_variant v,
-                          hipStream_t s) {
+                          hipStream_t s, const kmx_pairs* pl = nullptr) {
     const int D = p->D, deg = p->deg, B = p->B;
     hipError_t err = hipSuccess;
     if (prec == KMX_PREC_FP64 || prec == KMX_PREC_FP32) {
@@ -77,11 +85,11 @@ static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* 
             const size_t smem = std::max(lds ? strip : size_t(0), size_t(nt / 64) * B * elem);
             if (f64) {
                 KArgs<double> a;
-                fill_args(a, p, transpose, x, y);
+                fill_args(a, p, transpose, x, y, pl);
                 err = f(&a, p->N, smem, s);
             } else {
                 KArgs<float> a;
-                fill_args(a, p, transpose, x, y);
+                fill_args(a, p, transpose, x, y, pl);
                 err = f(&a, p->N, smem, s);
             }
         } else {
@@ -92,11 +100,11 @@ static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* 
             }
             if (f64) {
                 KArgs<double> a;
-                fill_args(a, p, transpose, x, y);
+                fill_args(a, p, transpose, x, y, pl);
                 err = launch_generic_f64(a, D, deg, p->N, smem, s);
             } else {
                 KArgs<float> a;
-                fill_args(a, p, transpose, x, y);
+                fill_args(a, p, transpose, x, y, pl);
                 err = launch_generic_f32(a, D, deg, p->N, smem, s);
             }
         }
@@ -130,8 +138,16 @@ int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, in
         set_error("kmx_apply: unknown precision");
         return KMX_E_ARG;
     }
+    if (!transpose && prec != KMX_PREC_FP32_GROUPED && p->bs && p->bs->ok && p->bs->split[prec_slot(prec)].active)
+        return kmx_blocks_apply(p, prec_slot(prec), x, y, s);
     kmx_variant v = prec == KMX_PREC_FP32_GROUPED ? p->var[1] : p->var[prec_slot(prec)];
     return launch_variant(p, transpose, x, y, prec, v, s);
+}
+
+// The pair-by-pair kernels over an explicit pair list (the level-block path's
+// remainder), with the plan's tuned variant.
+int kmx_launch_pairs(kmx_plan* p, const kmx_pairs* pl, const double* x, double* y, int prec, hipStream_t s) {
+    return launch_variant(p, false, x, y, prec, p->var[prec_slot(prec)], s, pl);
 }
 
 // ---------------------------------------------------------------- autotuner
@@ -161,13 +177,55 @@ TuneKey key_of(const kmx_plan* p, int prec) {
 }
 }  // namespace
 
+// Level-block split of one precision with threshold beta (-2: library default).
+static int set_split(kmx_plan* p, int prec, double beta) {
+    if (!p->bs || !p->bs->ok) return KMX_OK;
+    if (prec == KMX_PREC_FP32 && !p->d_blk32) return KMX_OK;
+    if (beta == -2.0) beta = kmx_blocks_default_beta();
+    kmx_split& sp = p->bs->split[prec_slot(prec)];
+    if (sp.beta == beta && (sp.active || (std::isinf(beta) && beta > 0))) return KMX_OK;
+    return kmx_blocks_split(p, prec_slot(prec), beta);
+}
+
 void kmx_plan_apply_cached_variants(kmx_plan* p) {
     std::lock_guard<std::mutex> g(tune_mutex());
     for (int prec : {KMX_PREC_FP64, KMX_PREC_FP32}) {
         auto it = tune_cache().find(key_of(p, prec));
-        if (it != tune_cache().end()) p->var[prec_slot(prec)] = it->second;
+        if (it != tune_cache().end()) {
+            p->var[prec_slot(prec)] = it->second;
+            set_split(p, prec, it->second.beta);
+        }
     }
 }
+
+int kmx_plan_init_splits(kmx_plan* p) {
+    for (int prec : {KMX_PREC_FP64, KMX_PREC_FP32}) {
+        const int rc = set_split(p, prec, p->var[prec_slot(prec)].beta);
+        if (rc != KMX_OK) return rc;
+    }
+    return KMX_OK;
+}
+
+namespace {
+// Average time of one call of fn on stream s (best of two rounds of 8).
+template <typename F>
+int time_calls(F fn, hipStream_t s, hipEvent_t e0, hipEvent_t e1, float& ms) {
+    int rc = KMX_OK;
+    for (int w = 0; w < 2 && rc == KMX_OK; ++w) rc = fn();
+    ms = 1e30f;
+    for (int round = 0; round < 2 && rc == KMX_OK; ++round) {
+        hipEventRecord(e0, s);
+        const int reps = 8;
+        for (int r = 0; r < reps && rc == KMX_OK; ++r) rc = fn();
+        hipEventRecord(e1, s);
+        hipEventSynchronize(e1);
+        float t = 0.f;
+        hipEventElapsedTime(&t, e0, e1);
+        ms = std::min(ms, t / reps);
+    }
+    return rc;
+}
+}  // namespace
 
 extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
     if (!p) {
@@ -194,36 +252,49 @@ extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
         const bool f64 = prec == KMX_PREC_FP64;
         const bool has_reg = f64 ? pick_f64(p->D, p->deg, 256, false) : pick_f32(p->D, p->deg, 256, false);
         const bool has_split = f64 ? pick_split_f64(p->D, p->deg, 256, false) : pick_split_f32(p->D, p->deg, 256, false);
-        if (!has_reg && !has_split) continue;  // generic kernel only: nothing to choose
-        const bool fits = strip_fits(p, elem);
-        float best = 1e30f;
         kmx_variant bestv = p->var[prec_slot(prec)];
-        for (int kind : {0, 1})
-        for (int nt : {128, 256, 512})
-            for (int lds : {1, 0}) {
-                if (lds && !fits) continue;
-                if ((kind == 0 && !has_reg) || (kind == 1 && (!has_split || nt == 128))) continue;
-                kmx_variant v{nt, lds, kind};
-                for (int w = 0; w < 2 && rc == KMX_OK; ++w) rc = launch_variant(p, false, x, y, prec, v, s);
-                float ms = 1e30f;
-                for (int round = 0; round < 2 && rc == KMX_OK; ++round) {  // best of two rounds
-                    hipEventRecord(e0, s);
-                    const int reps = 8;
-                    for (int r = 0; r < reps && rc == KMX_OK; ++r) rc = launch_variant(p, false, x, y, prec, v, s);
-                    hipEventRecord(e1, s);
-                    hipEventSynchronize(e1);
-                    float t = 0.f;
-                    hipEventElapsedTime(&t, e0, e1);
-                    ms = std::min(ms, t / reps);
-                }
+        // 1. pair-by-pair kernel variant, on the whole pair list
+        if (has_reg || has_split) {
+            const bool fits = strip_fits(p, elem);
+            float best = 1e30f;
+            for (int kind : {0, 1})
+                for (int nt : {128, 256, 512})
+                    for (int lds : {1, 0}) {
+                        if (lds && !fits) continue;
+                        if ((kind == 0 && !has_reg) || (kind == 1 && (!has_split || nt == 128))) continue;
+                        kmx_variant v{nt, lds, kind, bestv.beta};
+                        float ms;
+                        rc = time_calls([&] { return launch_variant(p, false, x, y, prec, v, s); }, s, e0, e1, ms);
+                        if (rc != KMX_OK) break;
+                        if (ms < best) {
+                            best = ms;
+                            bestv = v;
+                        }
+                    }
+            if (rc != KMX_OK) break;
+            p->var[prec_slot(prec)] = bestv;
+        }
+        // 2. level-block split threshold, with that variant for the remainder
+        if (p->bs && p->bs->ok) {
+            float best = 1e30f;
+            double bestb = bestv.beta;
+            for (double beta : {HUGE_VAL, 0.0, 1000.0, 4000.0, 16000.0, 64000.0, 256000.0}) {
+                rc = set_split(p, prec, beta);
+                if (rc != KMX_OK) break;
+                float ms;
+                rc = time_calls([&] { return kmx_launch_apply(p, false, x, y, prec, s); }, s, e0, e1, ms);
                 if (rc != KMX_OK) break;
                 if (ms < best) {
                     best = ms;
-                    bestv = v;
+                    bestb = beta;
                 }
             }
-        if (rc != KMX_OK) break;
-        p->var[prec_slot(prec)] = bestv;
+            if (rc != KMX_OK) break;
+            bestv.beta = bestb;
+            p->var[prec_slot(prec)] = bestv;
+            rc = set_split(p, prec, bestb);
+            if (rc != KMX_OK) break;
+        }
         std::lock_guard<std::mutex> g(tune_mutex());
         tune_cache()[key_of(p, prec)] = bestv;
     }
@@ -232,6 +303,44 @@ extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
     hipFree(x);
     hipFree(y);
     return rc;
+}
+
+extern "C" int kmx_plan_set_split(kmx_plan* p, int prec, double beta) {
+    if (!p || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32) || std::isnan(beta)) {
+        set_error("kmx_plan_set_split: bad arguments");
+        return KMX_E_ARG;
+    }
+    if (beta < 0) beta = -1.0;
+    p->var[prec_slot(prec)].beta = beta;
+    if (!p->bs || !p->bs->ok) return KMX_OK;
+    if (prec == KMX_PREC_FP32 && !p->d_blk32) {
+        set_error("kmx_plan_set_split: plan has no FP32 blocks");
+        return KMX_E_ARG;
+    }
+    p->bs->split[prec_slot(prec)].beta = std::numeric_limits<double>::quiet_NaN();  // force a rebuild
+    return set_split(p, prec, beta);
+}
+
+extern "C" int kmx_plan_get_split(const kmx_plan* p, int prec, int* structured, double* beta, int64_t* block_pairs,
+                                  int64_t* pairs_in_blocks, int64_t* remaining_pairs, double* flops) {
+    if (!p || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
+        set_error("kmx_plan_get_split: bad arguments");
+        return KMX_E_ARG;
+    }
+    const bool ok = p->bs && p->bs->ok;
+    const kmx_split* sp = ok ? &p->bs->split[prec_slot(prec)] : nullptr;
+    const bool act = sp && sp->active;
+    if (structured) *structured = ok ? 1 : 0;
+    if (beta) *beta = ok ? sp->beta : HUGE_VAL;
+    if (block_pairs) *block_pairs = act ? sp->block_pairs : 0;
+    if (pairs_in_blocks) *pairs_in_blocks = act ? sp->pairs_in_blocks : 0;
+    if (remaining_pairs) *remaining_pairs = act ? sp->sub_pairs : p->P;
+    if (flops) {
+        double chain = 2.0 * p->D;
+        for (int k = 0; k <= p->D; ++k) chain *= p->deg;
+        *flops = double(act ? sp->sub_pairs : p->P) * p->T * chain + (act ? sp->fact_flops : 0.0);
+    }
+    return KMX_OK;
 }
 
 extern "C" int kmx_plan_get_variant(const kmx_plan* p, int prec, int* nt, int* lds) {

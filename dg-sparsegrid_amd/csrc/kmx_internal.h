@@ -50,7 +50,65 @@ struct KArgs {
 struct kmx_variant {
     int nt = 0;
     int lds = -1;
-    int kind = 0;  // 0: one lane per pair, 1: pair split over deg lanes
+    int kind = 0;          // 0: one lane per pair, 1: pair split over deg lanes
+    double beta = -2.0;    // level-block split threshold (-2: library default)
+};
+
+// One precision's division of the pair list between the level-block kernel
+// (block pairs listed in items/tasks) and the pair-by-pair kernels (the rest,
+// as a CSR sub-list).  beta is the per-stage overhead, in flops, charged to a
+// factorized block pair when the two costs are compared.
+struct kmx_split {
+    bool active = false;
+    double beta = 0;
+    long long sub_pairs = 0;        // pairs left to the pair-by-pair kernel
+    long long block_pairs = 0;      // level-block pairs run factorized
+    long long pairs_in_blocks = 0;  // element pairs they cover
+    double fact_flops = 0;          // flops of the factorized block pairs per apply (all terms)
+    int ntasks = 0;
+    int maxX = 0, maxI = 0, maxY = 0;  // LDS buffer sizes (elements)
+    long long* d_rp = nullptr;
+    int* d_cols = nullptr;
+    int* d_order = nullptr;
+    int4* d_tasks = nullptr;   // (row block, first item, end item, partial offset)
+    int2* d_items = nullptr;   // (column block, contraction order)
+    int* d_btask = nullptr;    // per row block: first task (nblk + 1)
+    void* d_part = nullptr;    // partial row-block results
+    long long part_elems = 0;
+};
+
+// Level-block structure of a plan: elements grouped by their level
+// multi-index (one dense cell tensor per group), per-dimension 1D connection
+// masks, and the masked 1D matrices.  ok is set only when every connected
+// pair of blocks holds exactly the tensor product of the per-dimension masks,
+// so the factorized sum equals the pair-by-pair sum term for term.
+struct kmx_blockset {
+    bool ok = false;
+    std::string why;
+    int nblk = 0;
+    std::vector<int> binfo;   // per block: D log2 cell counts, D first 1D indices, cell-table offset
+    std::vector<int> elems;   // per block: element of each cell (row-major, dim 1 slowest), -1 if absent
+    std::vector<int> eblk, ecell;
+    std::vector<std::vector<unsigned char>> mask;  // per dim: nb x nb, [I * nb + J]
+    struct BP {
+        int bi, bj;
+        long long pairs;
+        double fflops;  // one term, factorized, best order
+        int order;      // 3 bits per stage: dim contracted at stage s
+        int maxI;       // largest intermediate of that order
+        bool fits;
+    };
+    std::vector<BP> bps;
+    std::vector<int> bpid;    // nblk x nblk -> index into bps or -1
+    std::vector<long long> rp;  // host copy of the pair list (0-based)
+    std::vector<int> cols;
+    int* d_binfo = nullptr;
+    int* d_elems = nullptr;
+    int* d_eblk = nullptr;
+    int* d_ecell = nullptr;
+    double* d_mat64 = nullptr;  // masked 1D matrices, column-major, offsets off[t*D+k]
+    float* d_mat32 = nullptr;
+    kmx_split split[2];
 };
 
 struct kmx_plan {
@@ -91,8 +149,28 @@ struct kmx_plan {
     std::vector<double> h_blkT64;
     double* d_x_host_stage = nullptr;  // kmx_apply_host staging
     double* d_y_host_stage = nullptr;
+
+    kmx_blockset* bs = nullptr;  // level-block path (null when disabled)
+};
+
+// Pair list handed to the pair-by-pair kernels (null members: the plan's own).
+struct kmx_pairs {
+    const long long* rp = nullptr;
+    const int* cols = nullptr;
+    const int* order = nullptr;
 };
 
 // Launchers (kmx_apply.hip)
 int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, int prec, hipStream_t s);
+int kmx_launch_pairs(kmx_plan* p, const kmx_pairs* pl, const double* x, double* y, int prec, hipStream_t s);
 void kmx_plan_apply_cached_variants(kmx_plan* p);
+int kmx_plan_init_splits(kmx_plan* p);
+
+// Level-block path (kmx_blocks.hip)
+void kmx_blocks_build(kmx_plan* p, const std::vector<int>& idx, std::vector<long long>&& rp,
+                      std::vector<int>&& cols, const double* const* mats);
+int kmx_blocks_split(kmx_plan* p, int slot, double beta);
+int kmx_blocks_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s);
+void kmx_blocks_update_term(kmx_plan* p, int term, int dim, const double* mat, hipStream_t s);
+void kmx_blocks_free(kmx_plan* p);
+double kmx_blocks_default_beta();

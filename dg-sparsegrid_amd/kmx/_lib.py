@@ -22,6 +22,7 @@ PREC_FP32_GROUPED = 2
 
 PLAN_FP32 = 1
 PLAN_ADJOINT = 2
+PLAN_LEVEL_BLOCKS = 4
 
 BUF_X, BUF_Y, BUF_W0, BUF_W1, BUF_FACTOR = 0, 1, 2, 3, 4
 
@@ -88,6 +89,10 @@ PROTOTYPES = {
     "kmx_plan_destroy": (C.c_int, [C.c_void_p]),
     "kmx_plan_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kmx_plan_get_variant": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "kmx_plan_set_split": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "kmx_plan_get_split": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_double)]),
     "kmx_plan_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
     "kmx_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "kmx_apply_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),

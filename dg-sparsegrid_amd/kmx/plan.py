@@ -61,9 +61,13 @@ class Plan:
     index_base 1 if idx1d / cols are 1-based (MATLAB), 0 otherwise
     fp32      keep FP32 block copies (KMX_PLAN_FP32)
     adjoint   keep the transposed pair list (KMX_PLAN_ADJOINT)
+    level_blocks  also build the level-block structure (KMX_PLAN_LEVEL_BLOCKS):
+              block pairs of the pair list may then run as mode products
+              (set_split / autotune); off, every apply runs pair by pair
     """
 
-    def __init__(self, mats, idx1d, deg, row_ptr=None, cols=None, index_base=1, fp32=True, adjoint=False):
+    def __init__(self, mats, idx1d, deg, row_ptr=None, cols=None, index_base=1, fp32=True, adjoint=False,
+                 level_blocks=False):
         _torch().cuda.current_device()  # make sure the HIP context exists
         self.T = len(mats)
         self.ndim = len(mats[0])
@@ -95,7 +99,8 @@ class Plan:
             desc.row_ptr = rp.ctypes.data_as(C.POINTER(C.c_int64))
             desc.cols = cc.ctypes.data_as(C.POINTER(C.c_int))
         desc.index_base = int(index_base)
-        desc.flags = (_lib.PLAN_FP32 if fp32 else 0) | (_lib.PLAN_ADJOINT if adjoint else 0)
+        desc.flags = ((_lib.PLAN_FP32 if fp32 else 0) | (_lib.PLAN_ADJOINT if adjoint else 0)
+                      | (_lib.PLAN_LEVEL_BLOCKS if level_blocks else 0))
         h = C.c_void_p()
         check(lib().kmx_plan_create(C.byref(desc), C.byref(h)))
         self._h = h
@@ -148,12 +153,25 @@ class Plan:
     def autotune(self, stream=None):
         """Time the kernel variants on this plan and keep the fastest (kmx_plan_autotune)."""
         check(lib().kmx_plan_autotune(self.handle, _stream(stream)))
-        return {p: self.variant(p) for p in ("fp64", "fp32")}
+        return {p: dict(self.variant(p), **self.split(p)) for p in ("fp64", "fp32")}
 
     def variant(self, prec="fp64"):
         nt, lds = C.c_int(), C.c_int()
         check(lib().kmx_plan_get_variant(self.handle, _prec(prec), C.byref(nt), C.byref(lds)))
         return {"threads": nt.value, "lds_staged": bool(lds.value)}
+
+    def set_split(self, prec, beta):
+        """Level-block split threshold (kmx_plan_set_split): beta < 0 runs every
+        block pair that fits factorized, beta = inf none (pair by pair)."""
+        check(lib().kmx_plan_set_split(self.handle, _prec(prec), float(beta)))
+
+    def split(self, prec="fp64"):
+        st, beta = C.c_int(), C.c_double()
+        bp, pb, rem, fl = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
+        check(lib().kmx_plan_get_split(self.handle, _prec(prec), C.byref(st), C.byref(beta), C.byref(bp),
+                                       C.byref(pb), C.byref(rem), C.byref(fl)))
+        return {"structured": bool(st.value), "beta": beta.value, "block_pairs": bp.value,
+                "pairs_in_blocks": pb.value, "remaining_pairs": rem.value, "flops": fl.value}
 
     def set_term_scale(self, t, alpha):
         check(lib().kmx_plan_set_term_scale(self.handle, int(t), float(alpha)))

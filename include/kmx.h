@@ -46,6 +46,9 @@ extern "C" {
 /* ---- plan flags */
 #define KMX_PLAN_FP32 1u    /* also keep FP32 copies of the coefficient blocks */
 #define KMX_PLAN_ADJOINT 2u /* also keep the transposed pair list / blocks (kmx_apply_transpose) */
+#define KMX_PLAN_LEVEL_BLOCKS 4u /* also build the level-block structure (kmx_plan_set_split);
+                                    without it every apply runs pair by pair, in the reference's
+                                    order of operations (apply_A.m:100-237) */
 
 typedef struct kmx_plan kmx_plan;
 
@@ -93,6 +96,27 @@ int kmx_plan_autotune(kmx_plan* plan, void* stream);
 
 /* The variant a plan currently uses for prec (KMX_PREC_FP64 / KMX_PREC_FP32). */
 int kmx_plan_get_variant(const kmx_plan* plan, int prec, int* threads, int* lds_staged);
+
+/* Level-block path.  Elements sharing a level multi-index form a dense cell
+ * tensor; when the pairs between two such blocks are the tensor product of
+ * per-dimension 1D connections (connect_nD.m:63-106 builds them that way, an
+ * all-pairs list trivially), the plan may apply that block pair as d mode
+ * products of masked 1D sub-matrices instead of one Kronecker chain per
+ * element pair -- the same terms, summed in another order.  A block pair runs
+ * factorized when its flops plus beta per contraction stage are below its
+ * pair-by-pair flops.  beta < 0: every block pair that fits; beta = +inf:
+ * none (pure pair-by-pair, the default).  Needs KMX_PLAN_LEVEL_BLOCKS;
+ * kmx_plan_autotune then also times a few thresholds.  No counterpart in the
+ * reference. */
+int kmx_plan_set_split(kmx_plan* plan, int prec, double beta);
+
+/* structured = 1 when the pair list has the level-block structure; the
+ * current beta, the block pairs run factorized, the element pairs they cover,
+ * the element pairs left to the pair-by-pair kernel, and the flops one apply
+ * performs (factorized mode products + 2 d deg^(d+1) per remaining pair and
+ * term), for prec.  Any output pointer may be null. */
+int kmx_plan_get_split(const kmx_plan* plan, int prec, int* structured, double* beta, int64_t* block_pairs,
+                       int64_t* pairs_in_blocks, int64_t* remaining_pairs, double* flops);
 
 /* num_pairs = P, block = deg^d, dof = N*deg^d. */
 int kmx_plan_info(const kmx_plan* plan, int64_t* num_pairs, int* block, int64_t* dof);

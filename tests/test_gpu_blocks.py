@@ -1,0 +1,145 @@
+"""GPU parity of the level-block (sum-factorized) apply against the oracle.
+
+The level-block path (opt-in, KMX_PLAN_LEVEL_BLOCKS) sums exactly the terms
+apply_A.m:100-237 sums (the plan verifies that every connected pair of level blocks is the tensor
+product of per-dimension 1D connections before enabling it), so the FP64 bar
+is the plain one: 1e-13 relative (2-norm).  FP32: 5e-6.  Each case is run
+with every fitting block pair factorized (beta < 0), with a mixed split, and
+with none (beta = inf), all against the same oracle result.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-13
+TOL32 = 5e-6
+SPLITS = (-1.0, 4000.0, float("inf"))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def case(nd, lev, deg, T, conn, seed):
+    rng = np.random.default_rng(seed)
+    if conn:
+        h = O.hash_table_nd(nd, lev, "SG")
+        rp, cols = O.connect_nd(nd, lev, "SG", lev, lev)
+    else:
+        h = O.hash_table_sparse_nd([lev] * nd, max(lev, 1), "SG")
+        rp = cols = None
+    n = deg * 2 ** lev
+    mats = [[rng.standard_normal((n, n)) for _ in range(nd)] for _ in range(T)]
+    x = rng.standard_normal(h["idx1d"].shape[0] * deg ** nd)
+    return mats, h["idx1d"], rp, cols, x
+
+
+def oracle_y(mats, idx, deg, rp, cols, x):
+    return O.apply_A(O.ApplyInputs(mats, idx, deg, rp, cols), x, nthreads=8)
+
+
+def check_all_splits(kmx, mats, idx, deg, rp, cols, x, want_structured=True, tol64=TOL64):
+    import torch
+
+    yo = oracle_y(mats, idx, deg, rp, cols, x)
+    plan = kmx.Plan(mats, idx, deg, rp, cols, index_base=1, fp32=True, level_blocks=True)
+    xd = torch.from_numpy(x).cuda()
+    info = []
+    for beta in SPLITS:
+        for prec, tol in (("fp64", tol64), ("fp32", TOL32)):
+            plan.set_split(prec, beta)
+            y = plan.apply(xd, prec=prec).cpu().numpy()
+            sp = plan.split(prec)
+            assert sp["structured"] == want_structured
+            if not want_structured or beta == float("inf"):
+                assert sp["block_pairs"] == 0 and sp["remaining_pairs"] == plan.P
+            else:
+                assert sp["pairs_in_blocks"] + sp["remaining_pairs"] == plan.P
+            e = rel(y, yo)
+            info.append((beta, prec, sp["block_pairs"], e))
+            assert e <= tol, (beta, prec, sp, e)
+    plan.destroy()
+    return info
+
+
+@pytest.mark.parametrize(
+    "nd,lev,deg,T,conn",
+    [
+        (2, 3, 2, 2, False), (2, 3, 2, 2, True),   # cfg1 shape: new hash all pairs / old hash connect_nD
+        (2, 5, 1, 2, True), (2, 5, 4, 2, False), (2, 4, 3, 3, True),
+        (3, 4, 2, 3, True), (3, 3, 3, 3, False), (3, 4, 4, 2, True),
+        (4, 3, 2, 4, True), (5, 2, 2, 2, True), (6, 3, 2, 6, True),   # 6D lev 3: cfg3's shape at a lower level
+    ],
+)
+def test_level_blocks_match_oracle(kmx_gpu, nd, lev, deg, T, conn):
+    mats, idx, rp, cols, x = case(nd, lev, deg, T, conn, seed=7 + nd * 100 + lev * 10 + deg)
+    check_all_splits(kmx_gpu, mats, idx, deg, rp, cols, x)
+
+
+def test_cfg2_size_level_blocks(kmx_gpu):
+    """3D lev 7 deg 3 with the maximal connect_nD list (P = 1,903,000)."""
+    mats, idx, rp, cols, x = case(3, 7, 3, 3, True, seed=2)
+    info = check_all_splits(kmx_gpu, mats, idx, 3, rp, cols, x)
+    assert any(b > 0 for _, _, b, _ in info)
+
+
+def test_adapted_grid_all_pairs(kmx_gpu):
+    """Incomplete level blocks (refined grid, all pairs): absent cells of a
+    block are zero columns and discarded rows."""
+    from kmx import grid as G
+
+    rng = np.random.default_rng(11)
+    base = G.hash_table_sparse_nd([4, 4, 4], 6, "SG")
+    table = G.refine_random(base, 6, 0.6, rng)
+    lev = int(table.lev.max())
+    n = 2 * 2 ** lev
+    mats = [[rng.standard_normal((n, n)) for _ in range(3)] for _ in range(2)]
+    x = rng.standard_normal(table.num_elem * 8)
+    check_all_splits(kmx_gpu, mats, table.idx1d, 2, None, None, x)
+
+
+def test_unstructured_pair_list_falls_back(kmx_gpu):
+    """A pair list that is not a per-block tensor product (random subset of
+    connect_nD) keeps the pair-by-pair path and still matches."""
+    mats, idx, rp, cols, x = case(3, 4, 2, 2, True, seed=5)
+    rng = np.random.default_rng(3)
+    keep = rng.random(len(cols)) < 0.7
+    nrp = np.zeros_like(rp)
+    nrp[1:] = np.cumsum([keep[rp[i]:rp[i + 1]].sum() for i in range(len(rp) - 1)])
+    check_all_splits(kmx_gpu, mats, idx, 2, nrp, cols[keep], x, want_structured=False)
+
+
+def test_plans_are_pair_by_pair_by_default(kmx_gpu):
+    import torch
+
+    mats, idx, rp, cols, x = case(3, 4, 2, 2, True, seed=9)
+    plan = kmx_gpu.Plan(mats, idx, 2, rp, cols, index_base=1, fp32=True)
+    sp = plan.split("fp64")
+    assert not sp["structured"] and sp["remaining_pairs"] == plan.P
+    y = plan.apply(torch.from_numpy(x).cuda(), prec="fp64").cpu().numpy()
+    assert rel(y, oracle_y(mats, idx, 2, rp, cols, x)) <= TOL64
+
+
+def test_update_term_and_scale_reach_level_blocks(kmx_gpu):
+    import torch
+
+    mats, idx, rp, cols, x = case(3, 5, 2, 3, True, seed=13)
+    plan = kmx_gpu.Plan(mats, idx, 2, rp, cols, index_base=1, fp32=True, level_blocks=True)
+    xd = torch.from_numpy(x).cuda()
+    rng = np.random.default_rng(1)
+    new = rng.standard_normal(mats[1][2].shape)
+    plan.update_term(1, 2, new)
+    plan.set_term_scale(0, -0.5)
+    mats2 = [list(m) for m in mats]
+    mats2[1][2] = new
+    mats2[0] = [mats[0][0] * -0.5, mats[0][1], mats[0][2]]
+    yo = oracle_y(mats2, idx, 2, rp, cols, x)
+    for beta in (-1.0, 4000.0):
+        for prec, tol in (("fp64", TOL64), ("fp32", TOL32)):
+            plan.set_split(prec, beta)
+            assert plan.split(prec)["block_pairs"] > 0
+            y = plan.apply(xd, prec=prec).cpu().numpy()
+            assert rel(y, yo) <= tol, (beta, prec)

@@ -1,0 +1,65 @@
+"""Precision study on the 6D diffusion operator: pdes/diffusion2.m's LDG
+chain generalised to 6 dims (6 terms, one per dim), sparse grid lev 4
+(hash_table_nD + connect_nD: N = 501, P = 247,785), deg 2 (64-value blocks).
+
+* FP64 apply vs the oracle: 1e-13 relative (2-norm).
+* FP32 apply (FP32 operands and accumulation, FP64 x / y) vs the oracle:
+  the error is recorded; bar 5e-6 (FP32 unit roundoff 6e-8 times the
+  accumulation depth).
+* Backward Euler (time_advance.m:328-349's system) with FP64 GMRES residuals
+  around FP32 inner applies, refined to an FP64 relative residual of 1e-10,
+  against the same system solved with FP64 applies only (to 1e-13).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from kmx import coeffs as K
+from kmx import grid as G
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def diffusion6(kmx_gpu):
+    pde = K.diffusion_nd(6, 4)
+    mats = K.get_coeff_mats(pde, 2)
+    tab = G.hash_table_nd([4] * 6, "SG")
+    rp, cols = G.connect_nd(tab, 4, 4, "SG")
+    plan = kmx_gpu.Plan(mats, tab.idx1d, 2, rp, cols, index_base=1, fp32=True)
+    assert (plan.N, plan.P, plan.B, plan.T) == (501, 247785, 64, 6)
+    return pde, mats, tab, rp, cols, plan
+
+
+def test_6d_apply_fp64_and_fp32_vs_oracle(diffusion6):
+    import torch
+
+    pde, mats, tab, rp, cols, plan = diffusion6
+    x = np.random.default_rng(4).standard_normal(plan.dof)
+    yo = O.apply_A(O.ApplyInputs(mats, tab.idx1d, 2, rp, cols), x, nthreads=8)
+    xd = torch.from_numpy(x).cuda()
+    e64 = rel(plan.apply(xd, prec="fp64").cpu().numpy(), yo)
+    e32 = rel(plan.apply(xd, prec="fp32").cpu().numpy(), yo)
+    print(f"6D lev4 deg2 diffusion: FP64 rel err {e64:.2e}, FP32 rel err {e32:.2e}")
+    assert e64 <= 1e-13
+    assert e32 <= 5e-6
+
+
+def test_6d_backward_euler_fp32_inner_refined_to_fp64(diffusion6):
+    import torch
+
+    from kmx import timestep as TS
+
+    pde, mats, tab, rp, cols, plan = diffusion6
+    f0 = torch.from_numpy(np.random.default_rng(5).standard_normal(plan.dof)).cuda()
+    dt = 100 * pde.set_dt(pde, 0.01)  # far beyond the explicit limit
+    f1, hist, inner = TS.backward_euler(plan, f0, dt, tol=1e-10, inner_prec="fp32", inner_tol=1e-5)
+    print(f"BE 6D: FP64 residual history {hist}, {inner} FP32 inner iterations")
+    assert hist[-1] <= 1e-10, hist
+    ref, hist64, _ = TS.backward_euler(plan, f0, dt, tol=1e-13, inner_prec="fp64", inner_tol=1e-13)
+    assert hist64[-1] <= 1e-13, hist64
+    assert rel(f1.cpu().numpy(), ref.cpu().numpy()) <= 1e-8
