@@ -134,6 +134,23 @@ static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* 
 int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, int prec, hipStream_t s) {
     if (p->N == 0) return KMX_OK;
     if (prec == KMX_PREC_FP32_GROUPED && (transpose || !p->d_gcols)) prec = KMX_PREC_FP32;  // same sum pair by pair
+    if (prec == KMX_PREC_BF16) {
+        if (transpose || !p->bs || !p->bs->ok || !p->d_blk32) {
+            set_error("kmx_apply: BF16 runs on the level-block path only (forward apply of a "
+                      "KMX_PLAN_LEVEL_BLOCKS | KMX_PLAN_FP32 plan with level-block structure)");
+            return KMX_E_UNSUPPORTED;
+        }
+        kmx_split& sp = p->bs->split[2];
+        if (!sp.active) {
+            const int rc = kmx_blocks_split(p, 2, -1.0);
+            if (rc != KMX_OK) return rc;
+            if (!sp.active) {
+                set_error("kmx_apply: no block pair of this plan fits the BF16 level-block kernel");
+                return KMX_E_UNSUPPORTED;
+            }
+        }
+        return kmx_blocks_apply(p, 2, x, y, s);
+    }
     if (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32 && prec != KMX_PREC_FP32_GROUPED) {
         set_error("kmx_apply: unknown precision");
         return KMX_E_ARG;

@@ -209,6 +209,113 @@ __device__ __forceinline__ void mode_mfma(const T* __restrict__ in, T* __restric
     }
 }
 
+// BF16 matrix-core product, FP32 accumulation: 16x16x16 per wave.  Lane l
+// feeds A[l % 16][4 (l / 16) + e] and B[4 (l / 16) + e][l % 16], e = 0..3;
+// result register r of lane l holds D[4 (l / 16) + r][l % 16] (checked on
+// gfx950).  Operands are rounded to nearest-even from FP32.
+typedef short svec4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short to_bf16(float f) {
+    unsigned u = __builtin_bit_cast(unsigned, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return short((u >> 16) | 0x40u);  // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return short(u >> 16);
+}
+
+__device__ __forceinline__ void mode_mfma_bf16(const float* __restrict__ in, float* __restrict__ out, int pre, int K,
+                                               int post, int M, const float* __restrict__ A, int lda, bool last,
+                                               float scale, float* __restrict__ As) {
+    constexpr int AC = a_chunk<float>();
+    constexpr int PT = AC / kNT;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int F = pre * post;
+    const int Mp = (M + 15) & ~15;
+    const int tm = Mp >> 4, tf = (F + 15) >> 4;
+    const int li = lane & 15, lk = lane >> 4;
+    int KC = (AC / Mp) & ~15;
+    const int K16 = (K + 15) & ~15;
+    if (KC > K16) KC = K16;
+    const int nch = (K + KC - 1) / KC;
+    const int ntile = tm * tf;
+    for (int g0 = 0; g0 < ntile; g0 += 16) {
+        fvec4 acc[4];
+        int pb[4], qb[4], ib[4];
+        bool on[4], bok[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            acc[u] = fvec4{0.f, 0.f, 0.f, 0.f};
+            const int tile = g0 + wave + 4 * u;
+            on[u] = tile < ntile;
+            const int tt = on[u] ? tile : 0;
+            ib[u] = (tt % tm) * 16;
+            const int fb = (tt / tm) * 16 + li;
+            bok[u] = on[u] && fb < F;
+            pb[u] = bok[u] ? fb / post : 0;
+            qb[u] = bok[u] ? fb - pb[u] * post : 0;
+        }
+        float reg[PT];
+        auto fetch = [&](int kc0) {
+#pragma unroll
+            for (int m = 0; m < PT; ++m) {
+                const int v = threadIdx.x + m * kNT;
+                const int jj = v / Mp;
+                const int i = v - jj * Mp;
+                const int j = kc0 + jj;
+                const bool ok = jj < KC && i < M && j < K;
+                const float val = A[(ok ? i : 0) + (long long)(ok ? j : 0) * lda];
+                reg[m] = ok ? val : 0.f;
+            }
+        };
+        fetch(0);
+        for (int c = 0; c < nch; ++c) {
+            const int kc0 = c * KC;
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < PT; ++m) {
+                const int v = threadIdx.x + m * kNT;
+                if (v < Mp * KC) As[v] = reg[m];
+            }
+            __syncthreads();
+            if (c + 1 < nch) fetch(kc0 + KC);
+            const int kn = min(KC, K16 - kc0);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (!on[u]) continue;  // wave-uniform
+                const float* bp = in + pb[u] * K * post + qb[u];
+                const float* ap = As + ib[u] + li;
+                for (int s16 = 0; s16 < kn; s16 += 16) {
+                    svec4 av, bv;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int jj = s16 + 4 * lk + e;
+                        const int j = kc0 + jj;
+                        av[e] = to_bf16(ap[jj * Mp]);
+                        const float b = bp[(j < K ? j : 0) * post];
+                        bv[e] = to_bf16((bok[u] && j < K) ? b : 0.f);
+                    }
+                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[u], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (!bok[u]) continue;
+            float* op = out + pb[u] * M * post + qb[u];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = ib[u] + 4 * (lane >> 4) + r;
+                if (i < M) {
+                    if (last)
+                        op[i * post] += scale * acc[u][r];
+                    else
+                        op[i * post] = acc[u][r];
+                }
+            }
+        }
+    }
+}
+
 // Stages with at most 8 output rows: a thread per fibre holds all M = R
 // outputs, A(:, j) is the same for every thread of the workgroup.
 template <typename T, int R>
@@ -262,7 +369,7 @@ __device__ __forceinline__ void mode_product(const T* in, T* out, int pre, int K
     }
 }
 
-template <typename T, int D, int DEG>
+template <typename T, int D, int DEG, bool BF16>
 __global__ void __launch_bounds__(kNT) k_blocks(BArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int BE = IPow<DEG, D>::v;
@@ -331,7 +438,10 @@ __global__ void __launch_bounds__(kNT) k_blocks(BArgs<T> a) {
                 const T* A = a.mat + a.off[t * D + k] + bi[D + k] * DEG + (long long)(bj[D + k] * DEG) * a.ld[k];
                 const bool last = s == D - 1;
                 T* dst = last ? Y : (src == I0 ? I1 : I0);
-                mode_product<T, DEG>(src, dst, pre, dims[k], post, rd[k], A, a.ld[k], last, a.scale[t], As);
+                if constexpr (BF16)
+                    mode_mfma_bf16(src, dst, pre, dims[k], post, rd[k], A, a.ld[k], last, a.scale[t], As);
+                else
+                    mode_product<T, DEG>(src, dst, pre, dims[k], post, rd[k], A, a.ld[k], last, a.scale[t], As);
                 __syncthreads();
                 dims[k] = rd[k];
                 src = dst;
@@ -376,9 +486,9 @@ __global__ void __launch_bounds__(256) k_blocks_reduce(RArgs<T> a) {
 using BlocksFn = hipError_t (*)(const void* args, int ntasks, size_t smem, hipStream_t s);
 using ReduceFn = hipError_t (*)(const void* args, long long nval, hipStream_t s);
 
-template <typename T, int D, int DEG>
+template <typename T, int D, int DEG, bool BF16 = false>
 hipError_t launch_blocks(const void* args, int ntasks, size_t smem, hipStream_t s) {
-    auto kern = k_blocks<T, D, DEG>;
+    auto kern = k_blocks<T, D, DEG, BF16>;
     if (smem > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
@@ -397,24 +507,24 @@ hipError_t launch_reduce(const void* args, long long nval, hipStream_t s) {
 }
 
 // deg 1..4, ndim 2..6
-template <typename T, int D>
+template <typename T, bool BF16, int D>
 BlocksFn pick_blocks_d(int deg) {
     switch (deg) {
-        case 1: return &launch_blocks<T, D, 1>;
-        case 2: return &launch_blocks<T, D, 2>;
-        case 3: return &launch_blocks<T, D, 3>;
-        case 4: return &launch_blocks<T, D, 4>;
+        case 1: return &launch_blocks<T, D, 1, BF16>;
+        case 2: return &launch_blocks<T, D, 2, BF16>;
+        case 3: return &launch_blocks<T, D, 3, BF16>;
+        case 4: return &launch_blocks<T, D, 4, BF16>;
     }
     return nullptr;
 }
-template <typename T>
+template <typename T, bool BF16 = false>
 BlocksFn pick_blocks(int D, int deg) {
     switch (D) {
-        case 2: return pick_blocks_d<T, 2>(deg);
-        case 3: return pick_blocks_d<T, 3>(deg);
-        case 4: return pick_blocks_d<T, 4>(deg);
-        case 5: return pick_blocks_d<T, 5>(deg);
-        case 6: return pick_blocks_d<T, 6>(deg);
+        case 2: return pick_blocks_d<T, BF16, 2>(deg);
+        case 3: return pick_blocks_d<T, BF16, 3>(deg);
+        case 4: return pick_blocks_d<T, BF16, 4>(deg);
+        case 5: return pick_blocks_d<T, BF16, 5>(deg);
+        case 6: return pick_blocks_d<T, BF16, 6>(deg);
     }
     return nullptr;
 }
@@ -809,6 +919,12 @@ int kmx_blocks_split(kmx_plan* p, int slot, double beta) {
         for (size_t i = 0; i < bs->bps.size(); ++i) {
             const auto& bp = bs->bps[i];
             if (!bp.fits) continue;
+            if (slot == 2) {  // BF16 chunks hold at least 16 columns of the staged coefficients
+                const int* ba = &bs->binfo[size_t(bp.bi) * (2 * D + 1)];
+                bool ok = true;
+                for (int k = 0; k < D; ++k) ok = ok && (((deg << ba[k]) + 15) & ~15) * 16 <= a_chunk<float>();
+                if (!ok) continue;
+            }
             if (beta < 0 || bp.fflops + beta * D < double(bp.pairs) * chain) {
                 fact[i] = 1;
                 ++nf;
@@ -923,7 +1039,11 @@ int kmx_blocks_split(kmx_plan* p, int slot, double beta) {
 int kmx_blocks_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s) {
     kmx_blockset* bs = p->bs;
     kmx_split& sp = bs->split[slot];
-    const bool f64 = slot == 0;
+    const bool f64 = slot == 0, bf = slot == 2;
+    if (bf && sp.sub_pairs > 0) {
+        set_error("kmx_apply: BF16 needs every block pair of the plan on the level-block path");
+        return KMX_E_UNSUPPORTED;
+    }
     if (sp.sub_pairs > 0) {
         kmx_pairs pl;
         pl.rp = sp.d_rp;
@@ -983,7 +1103,7 @@ int kmx_blocks_apply(kmx_plan* p, int slot, const double* x, double* y, hipStrea
         BArgs<float> a;
         fill(a, bs->d_mat32);
         a.part = static_cast<float*>(sp.d_part);
-        err = pick_blocks<float>(p->D, p->deg)(&a, sp.ntasks, smem, s);
+        err = (bf ? pick_blocks<float, true>(p->D, p->deg) : pick_blocks<float>(p->D, p->deg))(&a, sp.ntasks, smem, s);
         if (err == hipSuccess) {
             RArgs<float> r;
             fill_r(r, static_cast<const float*>(sp.d_part));
@@ -1020,6 +1140,7 @@ void kmx_blocks_free(kmx_plan* p) {
     if (!bs) return;
     free_split(bs->split[0]);
     free_split(bs->split[1]);
+    free_split(bs->split[2]);
     dfree(bs->d_binfo);
     dfree(bs->d_elems);
     dfree(bs->d_eblk);

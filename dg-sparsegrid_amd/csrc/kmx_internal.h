@@ -108,7 +108,7 @@ struct kmx_blockset {
     int* d_ecell = nullptr;
     double* d_mat64 = nullptr;  // masked 1D matrices, column-major, offsets off[t*D+k]
     float* d_mat32 = nullptr;
-    kmx_split split[2];
+    kmx_split split[3];  // FP64, FP32, BF16 (every block pair factorized)
 };
 
 struct kmx_plan {

@@ -19,6 +19,7 @@ KMX_E_UNSUPPORTED = -5
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_FP32_GROUPED = 2
+PREC_BF16 = 3
 
 PLAN_FP32 = 1
 PLAN_ADJOINT = 2
@@ -90,6 +91,10 @@ PROTOTYPES = {
     "kmx_plan_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kmx_plan_get_variant": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "kmx_plan_set_split": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "kmx_realspace_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "kmx_realspace_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kmx_realspace_destroy": (C.c_int, [C.c_void_p]),
     "kmx_plan_get_split": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      C.POINTER(C.c_double)]),

@@ -5,6 +5,7 @@ library:
 
     ftmp = apply_A(pde, opts, A_data, f, deg, Vmax, Emax, imex_flag)   apply_A.m:1
     f    = time_advance(pde, opts, A_data, f, t, dt, deg, hash_table)   time_advance.m:1
+    fr   = wavelet_to_realspace(pde, opts, Meval_D, f, hash_table)      wavelet_to_realspace.m:1
     err, fval = asgard(pde_fn, **opts)                                  asgard.m:3 (RK3/BE subset)
 
 `pde` is a kmx.coeffs.PDE whose 1D matrices have been built with
@@ -20,7 +21,7 @@ import numpy as np
 from . import coeffs as C
 from . import grid as G
 from . import timestep as TS
-from .plan import Plan
+from .plan import Plan, RealSpace
 
 
 @dataclass
@@ -90,6 +91,23 @@ def time_advance(pde, opts, A_data, f, t, dt, deg, hash_table=None, Vmax=None, E
         y, _, _ = TS.backward_euler(plan, x, dt, rhs, tol=opts.extra.get("be_tol", 1e-12))
     else:
         y = TS.rk3_step(lambda v: apply_A(pde, opts, A_data, v, deg), x, t, dt, src)
+    return y.cpu().numpy() if host else y
+
+
+def wavelet_to_realspace(pde, opts, Meval_D, f_wSpace, hash_table):
+    """wavelet_to_realspace.m:1-79: sum over the elements of hash_table of
+    kron_d Meval_D{d}(:, deg-column block of the element's 1D index) times the
+    element's coefficients; the device transform is cached on the pde object
+    per (Meval, table).  Returns prod(rows of Meval_D{d}) values, dim 1 slowest."""
+    import torch
+
+    key = (tuple(id(m) for m in Meval_D), id(hash_table), opts.deg)
+    cache = pde.__dict__.setdefault("_kmx_realspace", {})
+    if key not in cache:
+        cache[key] = RealSpace([np.asarray(m) for m in Meval_D], hash_table.idx1d, opts.deg, index_base=1)
+    host = isinstance(f_wSpace, np.ndarray)
+    x = torch.from_numpy(np.ascontiguousarray(f_wSpace, dtype=np.float64)).cuda() if host else f_wSpace
+    y = cache[key].apply(x)
     return y.cpu().numpy() if host else y
 
 

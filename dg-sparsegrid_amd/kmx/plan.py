@@ -15,6 +15,7 @@ PREC = {
     "fp64": _lib.PREC_FP64,
     "fp32": _lib.PREC_FP32,
     "fp32_grouped": _lib.PREC_FP32_GROUPED,
+    "bf16": _lib.PREC_BF16,
 }
 
 
@@ -212,6 +213,53 @@ class Batch:
     def destroy(self):
         if getattr(self, "_h", None) is not None:
             lib().kmx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class RealSpace:
+    """Wavelet space -> real space (wavelet_to_realspace.m): f_r = sum_e
+    kron_d Meval_d(:, block J_d(e)) f_w(block e), on the device.
+
+    meval     list (dims) of npts_d x ncols_d arrays (Meval_D{d}, host)
+    idx1d     N x d 1D indices per element; deg block edge"""
+
+    def __init__(self, meval, idx1d, deg, index_base=1):
+        _torch().cuda.current_device()
+        keep = [np.asfortranarray(np.asarray(m, dtype=np.float64)) for m in meval]
+        self.ndim = len(keep)
+        self.deg = int(deg)
+        idx = np.ascontiguousarray(np.asarray(idx1d, dtype=np.int32).reshape(-1, self.ndim))
+        self.N = idx.shape[0]
+        npts = np.asarray([m.shape[0] for m in keep], dtype=np.int32)
+        ncols = np.asarray([m.shape[1] for m in keep], dtype=np.int32)
+        ptrs = (C.c_void_p * self.ndim)(*[m.ctypes.data for m in keep])
+        h = C.c_void_p()
+        check(lib().kmx_realspace_create(self.ndim, self.deg, self.N, idx.ctypes.data_as(C.POINTER(C.c_int)),
+                                         int(index_base), npts.ctypes.data_as(C.POINTER(C.c_int)),
+                                         ncols.ctypes.data_as(C.POINTER(C.c_int)),
+                                         C.cast(ptrs, C.POINTER(C.c_void_p)), C.byref(h)))
+        self._h = h
+        self.npts = [int(v) for v in npts]
+        self.dof = self.N * self.deg ** self.ndim
+
+    def apply(self, f_w, f_r=None, stream=None):
+        torch = _torch()
+        if f_w.numel() != self.dof:
+            raise ValueError(f"f_w must have {self.dof} entries")
+        if f_r is None:
+            f_r = torch.empty(int(np.prod(self.npts)), dtype=torch.float64, device=f_w.device)
+        check(lib().kmx_realspace_apply(self._h, _ptr(f_w), _ptr(f_r), _stream(stream)))
+        return f_r
+
+    def destroy(self):
+        if getattr(self, "_h", None) is not None:
+            lib().kmx_realspace_destroy(self._h)
             self._h = None
 
     def __del__(self):

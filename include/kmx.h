@@ -42,6 +42,10 @@ extern "C" {
 #define KMX_PREC_FP32 1       /* FP32 operands and accumulation, same pair-by-pair algorithm */
 #define KMX_PREC_FP32_GROUPED 2 /* FP32, pairs sharing all but the last 1D index are summed
                                    before the outer contractions (same sum, fewer flops) */
+#define KMX_PREC_BF16 3       /* BF16 operands on the matrix cores, FP32 accumulation: the
+                                 level-block mode products of a KMX_PLAN_LEVEL_BLOCKS |
+                                 KMX_PLAN_FP32 plan whose whole pair list factorizes (else
+                                 KMX_E_UNSUPPORTED).  A precision study: ~1e-3 relative */
 
 /* ---- plan flags */
 #define KMX_PLAN_FP32 1u    /* also keep FP32 copies of the coefficient blocks */
@@ -186,6 +190,19 @@ typedef struct {
  * flops_kron{k} (Kronmult/flops_kron2.m:22-46). */
 int kmx_kronmult_batch_plan(int k, const int* nrow, const int* ncol, int nvec, kmx_gemm_desc* out, int capacity,
                             int* count, int64_t* work_elems);
+
+/* ---- wavelet space -> real space (wavelet_to_realspace.m:1-79).
+ * f_r = sum over elements e of kron(Meval_1(:, block J_1(e)), ..., Meval_d(:, block J_d(e)))
+ *       * f_w(block e), where Meval_d(:, block J) are the deg columns of 1D index J.
+ * create: elem_idx1d N x d (as in kmx_plan_desc), meval[d] HOST column-major
+ * npts[d] x ncols[d] (ncols = deg * cells); copied to the device.  apply: f_w
+ * DEVICE N*deg^d, f_r DEVICE prod(npts), overwritten, Kronecker order (dim 1
+ * slowest), asynchronous on `stream`. */
+typedef struct kmx_realspace kmx_realspace;
+int kmx_realspace_create(int ndim, int deg, int num_elem, const int* elem_idx1d, int index_base, const int* npts,
+                         const int* ncols, const double* const* meval, kmx_realspace** out);
+int kmx_realspace_apply(kmx_realspace* r, const double* f_w, double* f_r, void* stream);
+int kmx_realspace_destroy(kmx_realspace* r);
 
 const char* kmx_last_error(void);
 int kmx_sync(void* stream);

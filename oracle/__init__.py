@@ -23,6 +23,10 @@ Parity status per function (what pins it; tests/test_oracle_gold.py):
       gold RK3 trajectories through the coefficient builder -- pinned
   gemm_vbatched
       per-entry closed form (numpy matmul)               -- pinned
+  wavelet_to_realspace
+      numpy.kron closed form summed over elements        -- pinned;
+      gold/generated-inputs/transformations/wavelet_to_realspace_* need the
+      Meval of matrix_plot_D.m, which is not restated -- gold parity unpinned
 """
 import ctypes as C
 import os
@@ -190,6 +194,19 @@ def krond(mats):
     Y = np.zeros(int(np.prod(nrow)) * int(np.prod(ncol)), np.float64)
     _chk(lib().orc_krond(len(keep), _ip(nrow), _ip(ncol), ptrs, _ip(Y)))
     return Y.reshape((int(np.prod(nrow)), int(np.prod(ncol))), order="F")
+
+
+def wavelet_to_realspace(meval, idx1d, deg, fw):
+    """wavelet_to_realspace.m:44-79: sum over elements of kron(Meval_d(:, block of
+    I_d(e))) * f_w(block e).  meval: list of npts_d x ncols_d arrays; idx1d: N x d
+    1-based; returns prod(npts) values in Kronecker order (dim 1 slowest)."""
+    keep, ptrs, nrow, ncol = _matlist(meval)
+    idx = np.ascontiguousarray(np.asarray(idx1d, np.int32).reshape(-1, len(keep)))
+    fw = np.ascontiguousarray(np.asarray(fw, np.float64))
+    fr = np.zeros(int(np.prod(nrow)), np.float64)
+    _chk(lib().orc_wavelet_to_realspace(len(keep), int(deg), idx.shape[0], _ip(idx), _ip(nrow), _ip(ncol), ptrs,
+                                        _ip(fw), _ip(fr)))
+    return fr
 
 
 class ApplyInputs:

@@ -736,6 +736,36 @@ int orc_gemm_vbatched(char ta, char tb, const int* m, const int* n, const int* k
     })
 }
 
+// wavelet_to_realspace.m:44-79: for each element, the deg-column blocks of
+// every dimension's Meval (npts_d x ncols_d, column-major) at the element's 1D
+// index form a rectangular Kronecker product applied to the element's
+// coefficients (kron_multd); the results are summed.
+int orc_wavelet_to_realspace(int nd, int deg, int N, const int* idx1d, const int* npts, const int* ncols,
+                             const double* const* meval, const double* fw, double* fr) {
+    ORC_TRY({
+        long B = 1, npt = 1;
+        for (int d = 0; d < nd; ++d) {
+            B *= deg;
+            npt *= npts[d];
+        }
+        std::fill(fr, fr + npt, 0.0);
+        std::vector<std::vector<double>> cols(nd);
+        for (int i = 0; i < N; ++i) {
+            std::vector<Mat> ml;
+            for (int d = 0; d < nd; ++d) {
+                const int ID = idx1d[size_t(i) * nd + d];  // 1-based
+                if (ID < 1 || ID * deg > ncols[d]) throw std::runtime_error("wavelet_to_realspace: index out of range");
+                cols[d].assign(meval[d] + size_t(ID - 1) * deg * npts[d], meval[d] + size_t(ID) * deg * npts[d]);
+                ml.push_back(Mat{npts[d], deg, cols[d].data()});
+            }
+            std::vector<double> X(fw + size_t(i) * B, fw + size_t(i + 1) * B);
+            std::vector<double> Y = kron_multd(ml, X);
+            for (long p = 0; p < npt; ++p) fr[p] += Y[p];
+        }
+        return 0;
+    })
+}
+
 int orc_num_threads() {
 #ifdef _OPENMP
     return omp_get_max_threads();

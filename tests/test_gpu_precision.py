@@ -63,3 +63,48 @@ def test_6d_backward_euler_fp32_inner_refined_to_fp64(diffusion6):
     ref, hist64, _ = TS.backward_euler(plan, f0, dt, tol=1e-13, inner_prec="fp64", inner_tol=1e-13)
     assert hist64[-1] <= 1e-13, hist64
     assert rel(f1.cpu().numpy(), ref.cpu().numpy()) <= 1e-8
+
+
+def test_6d_apply_bf16_mfma_vs_oracle(kmx_gpu, diffusion6):
+    """BF16 operands on the matrix cores with FP32 accumulation (level-block
+    path): the error against the FP64 oracle is recorded; bar 3e-2 (BF16 unit
+    roundoff 3.9e-3 on both operands, FP32 sums)."""
+    import torch
+
+    pde, mats, tab, rp, cols, _ = diffusion6
+    plan = kmx_gpu.Plan(mats, tab.idx1d, 2, rp, cols, index_base=1, fp32=True, level_blocks=True)
+    x = np.random.default_rng(4).standard_normal(plan.dof)
+    yo = O.apply_A(O.ApplyInputs(mats, tab.idx1d, 2, rp, cols), x, nthreads=8)
+    xd = torch.from_numpy(x).cuda()
+    e16 = rel(plan.apply(xd, prec="bf16").cpu().numpy(), yo)
+    print(f"6D lev4 deg2 diffusion: BF16-MFMA rel err {e16:.2e}")
+    assert 1e-5 < e16 <= 3e-2  # really rounded to BF16, and still close
+
+
+@pytest.mark.parametrize("nd,lev,deg,conn", [(2, 4, 2, False), (3, 4, 3, True), (3, 3, 4, False)])
+def test_bf16_mfma_shapes_vs_oracle(kmx_gpu, nd, lev, deg, conn):
+    import torch
+
+    rng = np.random.default_rng(20 + nd)
+    if conn:
+        h = O.hash_table_nd(nd, lev, "SG")
+        rp, cols = O.connect_nd(nd, lev, "SG", lev, lev)
+    else:
+        h = O.hash_table_sparse_nd([lev] * nd, lev, "SG")
+        rp = cols = None
+    n = deg * 2 ** lev
+    mats = [[rng.standard_normal((n, n)) for _ in range(nd)] for _ in range(2)]
+    x = rng.standard_normal(h["idx1d"].shape[0] * deg ** nd)
+    yo = O.apply_A(O.ApplyInputs(mats, h["idx1d"], deg, rp, cols), x, nthreads=8)
+    plan = kmx_gpu.Plan(mats, h["idx1d"], deg, rp, cols, index_base=1, fp32=True, level_blocks=True)
+    y = plan.apply(torch.from_numpy(x).cuda(), prec="bf16").cpu().numpy()
+    e16 = rel(y, yo)
+    assert 1e-5 < e16 <= 3e-2, e16
+
+
+def test_bf16_needs_level_blocks(kmx_gpu, diffusion6):
+    import torch
+
+    plan = diffusion6[-1]  # built without KMX_PLAN_LEVEL_BLOCKS
+    with pytest.raises(kmx_gpu.KmxError):
+        plan.apply(torch.zeros(plan.dof, dtype=torch.float64, device="cuda"), prec="bf16")

@@ -216,3 +216,30 @@ def test_gemm_vbatched_closed_form():
     O.gemm_vbatched("T", "N", ms, ns, ks, 2.0, As, Bs, 0.5, Cs)
     for a, b, c, c0 in zip(As, Bs, Cs, C0):
         assert np.allclose(c, 2.0 * a.T @ b + 0.5 * c0, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("nd,lev,deg", [(1, 4, 3), (2, 3, 2), (3, 2, 2)])
+def test_wavelet_to_realspace_closed_form(nd, lev, deg):
+    """wavelet_to_realspace.m:44-79 against numpy: sum over elements of
+    kron(column blocks of Meval_d) times the element's coefficients."""
+    rng = np.random.default_rng(nd * 10 + lev)
+    h = O.hash_table_sparse_nd([lev] * nd, lev, "SG")
+    idx = h["idx1d"]
+    n = deg * 2 ** lev
+    npts = [5 + 2 * d for d in range(nd)]
+    meval = [rng.standard_normal((npts[d], n)) for d in range(nd)]
+    B = deg ** nd
+    fw = rng.standard_normal(idx.shape[0] * B)
+    fr = O.wavelet_to_realspace(meval, idx, deg, fw)
+    ref = np.zeros(int(np.prod(npts)))
+    for e in range(idx.shape[0]):
+        K = np.ones((1, 1))
+        for d in range(nd):
+            I = idx[e, d] - 1
+            K = np.kron(K, meval[d][:, I * deg:(I + 1) * deg])
+        ref += K @ fw[e * B:(e + 1) * B]
+    assert np.linalg.norm(fr - ref) <= 1e-13 * np.linalg.norm(ref)
+    # a planted mistake (one element's block from the neighbouring 1D index) is caught
+    bad = idx.copy()
+    bad[-1, 0] = bad[-1, 0] % (2 ** lev) + 1
+    assert np.linalg.norm(O.wavelet_to_realspace(meval, bad, deg, fw) - ref) > 1e-6 * np.linalg.norm(ref)
