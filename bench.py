@@ -4,11 +4,15 @@
 Workload: 3D sparse grid, lev 7, deg 3 (27-value blocks), 3 terms (one
 deg x deg-block operator per dimension and term), the maximal connect_nD pair
 list (P = 1,903,000 element pairs over N = 1696 elements), x ~ N(0,1) seeded.
-A "step" is one full operator application y = A x.  The plain FP64 kernel and
-the mixed-precision (FP32) kernels are timed on the same resident operator;
-`value` is the faster mixed-precision variant's applies/s.
+A "step" is one full operator application y = A x.  The FP64 kernel and the
+mixed-precision (FP32) kernels are timed on the same resident operator;
+`value` is the faster mixed-precision variant's applies/s.  Extra legs: 64
+adapted instances advanced by a HIP-graph RK3 step (--no-batch skips), the
+6D lev 4 deg 2 precision study with FP32 / BF16 errors and a refined backward
+Euler step (--no-precision skips), and the CPU oracle on the host cores
+(--no-cpu skips).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-precision]
 
 Prints ONE JSON line.
 """
@@ -100,8 +104,12 @@ def cpu_baseline(op, budget_s=12.0):
         if time.perf_counter() - t0 > budget_s or n >= 20:
             break
     dt = (time.perf_counter() - t0) / n
+    t1 = time.perf_counter()
+    oracle.apply_A(inp, op["x"], nthreads=1)  # the reference's serial loop order on one core
+    dt1 = time.perf_counter() - t1
     return y, {"value": 1.0 / dt, "unit": "applies/s", "cores": threads, "kind": "port",
-               "sample": f"{n} full applies of the same workload (oracle/kmx_oracle.cpp, OpenMP over element rows)"}
+               "sample": f"{n} full applies of the same workload (oracle/kmx_oracle.cpp, OpenMP over element rows)",
+               "single_thread_applies_per_s": round(1.0 / dt1, 3)}
 
 
 def batched_leg(steps=20, warmup=3):
@@ -225,7 +233,7 @@ def main():
         return
 
     runs = {"fp64": "fp64", "fp32": "fp32", "fp32_grouped": "fp32_grouped"}
-    res, err, flops = {}, {}, {}
+    res, err, flops, maxabs = {}, {}, {}, {}
     for name, prec in runs.items():
         res[name] = time_apply(plan, xd, yd, prec, a.warmup, a.steps)
     y64 = plan.apply(xd, prec="fp64").cpu().numpy()
@@ -233,6 +241,7 @@ def main():
     for name, prec in runs.items():
         yp = plan.apply(xd, prec=prec).cpu().numpy()
         err[name] = float(np.linalg.norm(yp - y64) / np.linalg.norm(y64))
+        maxabs[name] = float(np.max(np.abs(yp - y64)))
         flops[name] = fl_pairs
     mixed = ("fp32", "fp32_grouped")
     best = min(mixed, key=lambda n: res[n])
@@ -272,7 +281,7 @@ def main():
         "config": {"workload": "3D SG lev7 deg3, 3 terms, connect_nD pairs (P=%d, N=%d)" % (plan.P, plan.N),
                    "variant": best, "block": plan.B, "tuned": tuned},
         "runs": {n: {"applies_per_s": round(1000.0 / res[n], 2), "ms_per_apply": round(res[n], 5),
-                     "rel_err_vs_fp64": err[n], "flops_per_apply": flops[n],
+                     "rel_err_vs_fp64": err[n], "max_abs_err_vs_fp64": maxabs[n], "flops_per_apply": flops[n],
                      "TFLOPs": round(flops[n] / (res[n] * 1e-3) / 1e12, 3)} for n in runs},
         "speedup_vs_fp64": round(res["fp64"] / ms, 3),
         "roofline": roofline,
@@ -285,6 +294,7 @@ def main():
         yo, cb = cpu_baseline(op)
         out["cpu_baseline"] = cb
         out["fp64_rel_err_vs_oracle"] = float(np.linalg.norm(y64 - yo) / np.linalg.norm(yo))
+        out["fp64_max_abs_err_vs_oracle"] = float(np.max(np.abs(y64 - yo)))
     print(json.dumps(out), flush=True)
 
 

@@ -53,11 +53,14 @@ static void fill_args(KArgs<T>& a, const kmx_plan* p, bool transpose, const doub
 
 static int prec_slot(int prec) { return prec == KMX_PREC_FP64 ? 0 : 1; }
 
-static bool strip_fits(const kmx_plan* p, size_t elem) {
+// LDS bytes of a row's coefficient strips (every term and dim).
+static size_t strip_bytes(const kmx_plan* p, size_t elem) {
     size_t se = 0;
     for (int k = 0; k < p->D; ++k) se += size_t(p->nb[k]) * p->deg * p->deg;
-    return se * p->T * elem <= lds_limit();
+    return se * p->T * elem;
 }
+
+static bool strip_fits(const kmx_plan* p, size_t elem) { return strip_bytes(p, elem) <= lds_limit(); }
 
 // One launch of the pair-by-pair kernel family with an explicit variant.
 static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* y, int prec, kmx_variant v,
@@ -79,9 +82,7 @@ static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* 
         if (!f) f = f64 ? pick_f64(D, deg, nt, lds) : pick_f32(D, deg, nt, lds);
         if (!f) f = f64 ? pick_split_f64(D, deg, nt == 128 ? 256 : nt, lds) : pick_split_f32(D, deg, nt == 128 ? 256 : nt, lds);
         if (f) {
-            size_t strip = 0;
-            for (int k = 0; k < D; ++k) strip += size_t(p->nb[k]) * deg * deg;
-            strip *= p->T * elem;
+            const size_t strip = strip_bytes(p, elem);
             const size_t smem = std::max(lds ? strip : size_t(0), size_t(nt / 64) * B * elem);
             if (f64) {
                 KArgs<double> a;
@@ -224,21 +225,24 @@ int kmx_plan_init_splits(kmx_plan* p) {
 }
 
 namespace {
-// Average time of one call of fn on stream s (best of two rounds of 8).
+// Average time of one call of fn on stream s: best of three rounds, each
+// round at least ~1 ms of calls (at least 8) so that short kernels are timed
+// over many launches.
 template <typename F>
 int time_calls(F fn, hipStream_t s, hipEvent_t e0, hipEvent_t e1, float& ms) {
     int rc = KMX_OK;
     for (int w = 0; w < 2 && rc == KMX_OK; ++w) rc = fn();
     ms = 1e30f;
-    for (int round = 0; round < 2 && rc == KMX_OK; ++round) {
+    int reps = 8;
+    for (int round = 0; round < 3 && rc == KMX_OK; ++round) {
         hipEventRecord(e0, s);
-        const int reps = 8;
         for (int r = 0; r < reps && rc == KMX_OK; ++r) rc = fn();
         hipEventRecord(e1, s);
         hipEventSynchronize(e1);
         float t = 0.f;
         hipEventElapsedTime(&t, e0, e1);
         ms = std::min(ms, t / reps);
+        if (round == 0 && t < 1.0f) reps = std::min(200, std::max(8, int(reps * 1.0f / std::max(t, 1e-3f)) + 1));
     }
     return rc;
 }
