@@ -202,7 +202,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched leg")
     ap.add_argument("--no-precision", action="store_true", help="skip the 6D precision leg")
     ap.add_argument("--pmc-pass", action="store_true",
-                    help="for counter runs: one 256 MiB device copy (calibration), then 3 applies per precision")
+                    help="for counter runs: one 256 MiB device copy (calibration), 3 applies per precision, "
+                         "then 3 level-block applies per precision of the 6D plan (unless --no-precision)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {a.gpus} requested); the operator apply is not "
@@ -229,6 +230,22 @@ def main():
             for _ in range(3):
                 plan.apply(xd, yd, prec=prec)
         torch.cuda.synchronize()
+        if not a.no_precision:  # matrix-core kernels: the 6D level-block plan, every block pair factorized
+            from kmx import coeffs as K
+            from kmx import grid as G
+
+            mats6 = K.get_coeff_mats(K.diffusion_nd(6, 4), 2)
+            tab6 = G.hash_table_nd([4] * 6, "SG")
+            rp6, cols6 = G.connect_nd(tab6, 4, 4, "SG")
+            p6 = kmx.Plan(mats6, tab6.idx1d, 2, rp6, cols6, index_base=1, fp32=True, level_blocks=True)
+            x6 = torch.from_numpy(np.random.default_rng(3).standard_normal(p6.dof)).cuda()
+            y6 = torch.empty_like(x6)
+            for prec in ("fp64", "fp32"):
+                p6.set_split(prec, -1.0)
+            for prec in ("bf16", "fp64", "fp32"):
+                for _ in range(3):
+                    p6.apply(x6, y6, prec=prec)
+            torch.cuda.synchronize()
         print(json.dumps({"pmc_pass": True, "copy_bytes": src.numel() * 8, "tuned": tuned}), flush=True)
         return
 
