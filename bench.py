@@ -269,7 +269,8 @@ def main():
     traffic = None
     if best == "fp32":
         v = tuned["fp32"]
-        traffic = MEASURED_TRAFFIC.get(("fp32", abs(v["threads"]), v["lds_staged"])) if v["threads"] > 0 else None
+        traffic = (MEASURED_TRAFFIC.get(("fp32", v["threads"], v["lds_staged"]))
+                   if v["threads"] > 0 and v["lds_mode"] in (0, 1) else None)  # counter runs: modes 0 and 1
     kern = {"fp32": "k_apply_reg", "fp32_grouped": "k_apply_grouped"}[best]
     roofline = {
         "bound": "mfma",

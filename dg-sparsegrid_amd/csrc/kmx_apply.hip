@@ -53,14 +53,17 @@ static void fill_args(KArgs<T>& a, const kmx_plan* p, bool transpose, const doub
 
 static int prec_slot(int prec) { return prec == KMX_PREC_FP64 ? 0 : 1; }
 
-// LDS bytes of a row's coefficient strips (every term and dim).
-static size_t strip_bytes(const kmx_plan* p, size_t elem) {
+// LDS bytes of a row's coefficient strips: every term and dim (mode 1) or
+// every dim but the first (mode 2, whose dim-1 blocks come from L2).
+static size_t strip_bytes(const kmx_plan* p, size_t elem, int mode = 1) {
     size_t se = 0;
-    for (int k = 0; k < p->D; ++k) se += size_t(p->nb[k]) * p->deg * p->deg;
+    for (int k = (mode == 2 ? 1 : 0); k < p->D; ++k) se += size_t(p->nb[k]) * p->deg * p->deg;
     return se * p->T * elem;
 }
 
-static bool strip_fits(const kmx_plan* p, size_t elem) { return strip_bytes(p, elem) <= lds_limit(); }
+static bool strip_fits(const kmx_plan* p, size_t elem, int mode = 1) {
+    return strip_bytes(p, elem, mode) <= lds_limit();
+}
 
 // One launch of the pair-by-pair kernel family with an explicit variant.
 static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* y, int prec, kmx_variant v,
@@ -74,15 +77,19 @@ static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* 
             set_error("kmx_apply: plan has no FP32 blocks (create it with KMX_PLAN_FP32)");
             return KMX_E_ARG;
         }
-        const bool fits = strip_fits(p, elem);
-        const bool lds = (v.lds < 0) ? fits : (v.lds != 0 && fits);
+        // LDS mode: 0 none, 1 every strip, 2 every strip but dim 1's (-1: 1 when it fits)
+        int mode = v.lds < 0 ? (strip_fits(p, elem) ? 1 : 0) : v.lds;
+        if (mode && !strip_fits(p, elem, mode)) mode = 0;
+        if (mode == 2 && (v.kind == 1 || D < 2)) mode = 1;
+        if (mode == 1 && !strip_fits(p, elem, 1)) mode = 0;
+        const bool lds = mode != 0;
         const int nt = v.nt > 0 ? v.nt : 256;
         LaunchFn f = nullptr;
         if (v.kind == 1) f = f64 ? pick_split_f64(D, deg, nt, lds) : pick_split_f32(D, deg, nt, lds);
-        if (!f) f = f64 ? pick_f64(D, deg, nt, lds) : pick_f32(D, deg, nt, lds);
+        if (!f) f = f64 ? pick_f64(D, deg, nt, mode) : pick_f32(D, deg, nt, mode);
         if (!f) f = f64 ? pick_split_f64(D, deg, nt == 128 ? 256 : nt, lds) : pick_split_f32(D, deg, nt == 128 ? 256 : nt, lds);
         if (f) {
-            const size_t strip = strip_bytes(p, elem);
+            const size_t strip = strip_bytes(p, elem, mode);
             const size_t smem = std::max(lds ? strip : size_t(0), size_t(nt / 64) * B * elem);
             if (f64) {
                 KArgs<double> a;
@@ -271,17 +278,17 @@ extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
         if (prec == KMX_PREC_FP32 && !p->d_blk32) continue;
         const size_t elem = prec == KMX_PREC_FP64 ? sizeof(double) : sizeof(float);
         const bool f64 = prec == KMX_PREC_FP64;
-        const bool has_reg = f64 ? pick_f64(p->D, p->deg, 256, false) : pick_f32(p->D, p->deg, 256, false);
+        const bool has_reg = f64 ? pick_f64(p->D, p->deg, 256, 0) : pick_f32(p->D, p->deg, 256, 0);
         const bool has_split = f64 ? pick_split_f64(p->D, p->deg, 256, false) : pick_split_f32(p->D, p->deg, 256, false);
         kmx_variant bestv = p->var[prec_slot(prec)];
         // 1. pair-by-pair kernel variant, on the whole pair list
         if (has_reg || has_split) {
-            const bool fits = strip_fits(p, elem);
             float best = 1e30f;
             for (int kind : {0, 1})
                 for (int nt : {128, 256, 512})
-                    for (int lds : {1, 0}) {
-                        if (lds && !fits) continue;
+                    for (int lds : {1, 2, 0}) {
+                        if (lds && !strip_fits(p, elem, lds)) continue;
+                        if (lds == 2 && (kind == 1 || p->D < 2)) continue;
                         if ((kind == 0 && !has_reg) || (kind == 1 && (!has_split || nt == 128))) continue;
                         kmx_variant v{nt, lds, kind, bestv.beta};
                         float ms;
@@ -371,9 +378,12 @@ extern "C" int kmx_plan_get_variant(const kmx_plan* p, int prec, int* nt, int* l
         return KMX_E_ARG;
     }
     const kmx_variant v = p->var[prec_slot(prec)];
-    const bool fits = strip_fits(p, prec == KMX_PREC_FP64 ? sizeof(double) : sizeof(float));
+    const size_t elem = prec == KMX_PREC_FP64 ? sizeof(double) : sizeof(float);
+    int mode = v.lds < 0 ? (strip_fits(p, elem) ? 1 : 0) : v.lds;
+    if (mode && !strip_fits(p, elem, mode)) mode = 0;
+    if (mode == 2 && (v.kind == 1 || p->D < 2)) mode = strip_fits(p, elem, 1) ? 1 : 0;
     if (nt) *nt = (v.nt > 0 ? v.nt : 256) * (v.kind == 1 ? -1 : 1);
-    if (lds) *lds = (v.lds < 0) ? int(fits) : int(v.lds != 0 && fits);
+    if (lds) *lds = mode;
     return KMX_OK;
 }
 

@@ -4,7 +4,7 @@
 
 namespace kmx {
 
-LaunchFn pick_f32(int D, int deg, int nt, bool lds) { return pick_impl<float>(D, deg, nt, lds); }
+LaunchFn pick_f32(int D, int deg, int nt, int lds) { return pick_impl<float>(D, deg, nt, lds); }
 LaunchFn pick_split_f32(int D, int deg, int nt, bool lds) { return pick_split_impl<float>(D, deg, nt, lds); }
 
 hipError_t launch_generic_f32(const KArgs<float>& a, int D, int deg, int nrows, size_t smem, hipStream_t s) {

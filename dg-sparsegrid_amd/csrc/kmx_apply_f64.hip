@@ -4,7 +4,7 @@
 
 namespace kmx {
 
-LaunchFn pick_f64(int D, int deg, int nt, bool lds) { return pick_impl<double>(D, deg, nt, lds); }
+LaunchFn pick_f64(int D, int deg, int nt, int lds) { return pick_impl<double>(D, deg, nt, lds); }
 LaunchFn pick_split_f64(int D, int deg, int nt, bool lds) { return pick_split_impl<double>(D, deg, nt, lds); }
 
 hipError_t launch_generic_f64(const KArgs<double>& a, int D, int deg, int nrows, size_t smem, hipStream_t s) {

@@ -74,7 +74,7 @@ __device__ __forceinline__ void contract_dim(RegArr<T, Pow<DEG, D>::v>& v, const
 }
 
 // Per-row context shared by the chain steps.
-template <typename T, int D, int DEG, bool USE_LDS>
+template <typename T, int D, int DEG, int USE_LDS>
 struct RowCtx {
     const KArgs<T>* a;
     const T* strip;  // LDS strips of this row (USE_LDS)
@@ -84,7 +84,7 @@ struct RowCtx {
 
     __device__ __forceinline__ void load_block(int t, int k, int J, RegArr<T, DEG * DEG>& m) const {
         const T* p;
-        if (USE_LDS) {
+        if (USE_LDS == 1 || (USE_LDS == 2 && k != 0)) {  // mode 2: dim 1's blocks come from L2
             p = strip + t * sterm + soff[k] + J * (DEG * DEG);
         } else {
             p = a->blk + a->off[t * D + k] + ((long long)I[k] * a->nb[k] + J) * (DEG * DEG);
@@ -94,7 +94,7 @@ struct RowCtx {
     }
 };
 
-template <typename T, int D, int DEG, bool USE_LDS, int K>
+template <typename T, int D, int DEG, int USE_LDS, int K>
 struct Chain {
     __device__ __forceinline__ static void run(RegArr<T, Pow<DEG, D>::v>& v, const RowCtx<T, D, DEG, USE_LDS>& c,
                                                int t, const RegArr<int, D>& J) {
@@ -104,7 +104,7 @@ struct Chain {
         Chain<T, D, DEG, USE_LDS, K - 1>::run(v, c, t, J);
     }
 };
-template <typename T, int D, int DEG, bool USE_LDS>
+template <typename T, int D, int DEG, int USE_LDS>
 struct Chain<T, D, DEG, USE_LDS, -1> {
     __device__ __forceinline__ static void run(RegArr<T, Pow<DEG, D>::v>&, const RowCtx<T, D, DEG, USE_LDS>&, int,
                                                const RegArr<int, D>&) {}
@@ -112,7 +112,7 @@ struct Chain<T, D, DEG, USE_LDS, -1> {
 
 // Contractions of dims D-1..1 of a lane's dim-1 slice (a (D-1)-dim block of
 // DEG^(D-1) values): dim k of the block is dim k-1 of the slice.
-template <typename T, int D, int DEG, bool USE_LDS, int K>
+template <typename T, int D, int DEG, int USE_LDS, int K>
 struct SliceChain {
     __device__ __forceinline__ static void run(RegArr<T, Pow<DEG, D - 1>::v>& v, const RowCtx<T, D, DEG, USE_LDS>& c,
                                                int t, const RegArr<int, D>& J) {
@@ -122,7 +122,7 @@ struct SliceChain {
         SliceChain<T, D, DEG, USE_LDS, K - 1>::run(v, c, t, J);
     }
 };
-template <typename T, int D, int DEG, bool USE_LDS>
+template <typename T, int D, int DEG, int USE_LDS>
 struct SliceChain<T, D, DEG, USE_LDS, 0> {
     __device__ __forceinline__ static void run(RegArr<T, Pow<DEG, D - 1>::v>&, const RowCtx<T, D, DEG, USE_LDS>&,
                                                int, const RegArr<int, D>&) {}
@@ -192,7 +192,7 @@ __device__ __forceinline__ void wave_reduce_store(const RegArr<T, B>& acc, T* re
 }
 
 // One row: NT lanes stride over the pair list; y_i reduced and written once.
-template <typename T, int D, int DEG, int NT, bool USE_LDS, bool KEEPX>
+template <typename T, int D, int DEG, int NT, int USE_LDS, bool KEEPX>
 __device__ __forceinline__ void apply_row(const KArgs<T>& a, const double* __restrict__ x, double* __restrict__ y,
                                           int row, unsigned char* smem) {
     constexpr int B = Pow<DEG, D>::v;
@@ -204,18 +204,19 @@ __device__ __forceinline__ void apply_row(const KArgs<T>& a, const double* __res
 #pragma unroll
     for (int k = 0; k < D; ++k) c.I[k] = a.idx1d[(long long)row * D + k];
 
-    if (USE_LDS) {
+    if (USE_LDS) {  // mode 1: every dim's strip; mode 2: all but dim 1's
         T* strip = reinterpret_cast<T*>(smem);
+        constexpr int K0 = USE_LDS == 2 ? 1 : 0;
         int so = 0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             c.soff[k] = so;
-            so += a.nb[k] * DEG * DEG;
+            if (k >= K0) so += a.nb[k] * DEG * DEG;
         }
         c.sterm = so;
         for (int t = 0; t < a.nterms; ++t) {
 #pragma unroll
-            for (int k = 0; k < D; ++k)
+            for (int k = K0; k < D; ++k)
                 stage<T, NT>(strip + t * so + c.soff[k], a.blk + a.off[t * D + k] + (long long)c.I[k] * a.nb[k] * DEG * DEG,
                              a.nb[k] * DEG * DEG, tid);
         }
@@ -272,7 +273,7 @@ __device__ __forceinline__ void apply_row(const KArgs<T>& a, const double* __res
     }
 }
 
-template <typename T, int D, int DEG, int NT, bool USE_LDS, bool KEEPX>
+template <typename T, int D, int DEG, int NT, int USE_LDS, bool KEEPX>
 __global__ void __launch_bounds__(NT) k_apply_reg(KArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int row = a.row_order[blockIdx.x];
@@ -378,7 +379,7 @@ __device__ __forceinline__ void wave_reduce_store_slices(const RegArr<T, BS>& ac
     }
 }
 
-template <typename T, int D, int DEG, int NT, bool USE_LDS>
+template <typename T, int D, int DEG, int NT, int USE_LDS>
 __device__ __forceinline__ void apply_row_split(const KArgs<T>& a, const double* __restrict__ x,
                                                 double* __restrict__ y, int row, unsigned char* smem) {
     constexpr int B = Pow<DEG, D>::v;
@@ -474,7 +475,7 @@ __device__ __forceinline__ void apply_row_split(const KArgs<T>& a, const double*
     }
 }
 
-template <typename T, int D, int DEG, int NT, bool USE_LDS>
+template <typename T, int D, int DEG, int NT, int USE_LDS>
 __global__ void __launch_bounds__(NT) k_apply_split(KArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem[];
     apply_row_split<T, D, DEG, NT, USE_LDS>(a, a.x, a.y, a.row_order[blockIdx.x], smem);
@@ -580,7 +581,7 @@ __global__ void __launch_bounds__(256) k_apply_generic(KArgs<T> a, int D, int DE
 // ---------------------------------------------------------------- launchers
 using LaunchFn = hipError_t (*)(const void* args, int nrows, size_t smem, hipStream_t s);
 
-template <typename T, int D, int DEG, int NT, bool USE_LDS, bool KEEPX>
+template <typename T, int D, int DEG, int NT, int USE_LDS, bool KEEPX>
 hipError_t launch_reg(const void* args, int nrows, size_t smem, hipStream_t s) {
     auto kern = k_apply_reg<T, D, DEG, NT, USE_LDS, KEEPX>;
     if (smem > 65536) {
@@ -603,13 +604,20 @@ constexpr bool keep_x() {
 
 // Workgroup sizes offered to the autotuner: 128/256/512 threads.
 template <typename T, int D, int DEG>
-LaunchFn pick_reg(int nt, bool lds) {
+LaunchFn pick_reg(int nt, int lds) {
     if constexpr (reg_ok<T, D, DEG>()) {
         constexpr bool KX = keep_x<T, D, DEG>();
+        if (lds == 2 && D < 2) return nullptr;
         switch (nt) {
-            case 128: return lds ? &launch_reg<T, D, DEG, 128, true, KX> : &launch_reg<T, D, DEG, 128, false, KX>;
-            case 256: return lds ? &launch_reg<T, D, DEG, 256, true, KX> : &launch_reg<T, D, DEG, 256, false, KX>;
-            case 512: return lds ? &launch_reg<T, D, DEG, 512, true, KX> : &launch_reg<T, D, DEG, 512, false, KX>;
+            case 128:
+                return lds == 2 ? &launch_reg<T, D, DEG, 128, 2, KX>
+                                : lds ? &launch_reg<T, D, DEG, 128, 1, KX> : &launch_reg<T, D, DEG, 128, 0, KX>;
+            case 256:
+                return lds == 2 ? &launch_reg<T, D, DEG, 256, 2, KX>
+                                : lds ? &launch_reg<T, D, DEG, 256, 1, KX> : &launch_reg<T, D, DEG, 256, 0, KX>;
+            case 512:
+                return lds == 2 ? &launch_reg<T, D, DEG, 512, 2, KX>
+                                : lds ? &launch_reg<T, D, DEG, 512, 1, KX> : &launch_reg<T, D, DEG, 512, 0, KX>;
             default: return nullptr;
         }
     } else {
@@ -620,7 +628,7 @@ LaunchFn pick_reg(int nt, bool lds) {
 }
 
 template <typename T, int D>
-LaunchFn pick_deg(int deg, int nt, bool lds) {
+LaunchFn pick_deg(int deg, int nt, int lds) {
     switch (deg) {
         case 1: return pick_reg<T, D, 1>(nt, lds);
         case 2: return pick_reg<T, D, 2>(nt, lds);
@@ -635,7 +643,7 @@ LaunchFn pick_deg(int deg, int nt, bool lds) {
 }
 
 template <typename T>
-LaunchFn pick_impl(int D, int deg, int nt, bool lds) {
+LaunchFn pick_impl(int D, int deg, int nt, int lds) {
     switch (D) {
         case 1: return pick_deg<T, 1>(deg, nt, lds);
         case 2: return pick_deg<T, 2>(deg, nt, lds);
@@ -647,7 +655,7 @@ LaunchFn pick_impl(int D, int deg, int nt, bool lds) {
     }
 }
 
-template <typename T, int D, int DEG, int NT, bool USE_LDS>
+template <typename T, int D, int DEG, int NT, int USE_LDS>
 hipError_t launch_split(const void* args, int nrows, size_t smem, hipStream_t s) {
     auto kern = k_apply_split<T, D, DEG, NT, USE_LDS>;
     if (smem > 65536) {
@@ -678,8 +686,8 @@ LaunchFn pick_split_impl(int D, int deg, int nt, bool lds) {
 }
 
 // per-precision entry points (instantiated in kmx_apply_f64.hip / _f32.hip)
-LaunchFn pick_f64(int D, int deg, int nt, bool lds);
-LaunchFn pick_f32(int D, int deg, int nt, bool lds);
+LaunchFn pick_f64(int D, int deg, int nt, int lds);
+LaunchFn pick_f32(int D, int deg, int nt, int lds);
 LaunchFn pick_split_f64(int D, int deg, int nt, bool lds);
 LaunchFn pick_split_f32(int D, int deg, int nt, bool lds);
 LaunchFn pick_grouped(int D, int deg, int T, int nt);

@@ -159,7 +159,7 @@ class Plan:
     def variant(self, prec="fp64"):
         nt, lds = C.c_int(), C.c_int()
         check(lib().kmx_plan_get_variant(self.handle, _prec(prec), C.byref(nt), C.byref(lds)))
-        return {"threads": nt.value, "lds_staged": bool(lds.value)}
+        return {"threads": nt.value, "lds_staged": bool(lds.value), "lds_mode": lds.value}
 
     def set_split(self, prec, beta):
         """Level-block split threshold (kmx_plan_set_split): beta < 0 runs every

@@ -99,8 +99,8 @@ int kmx_plan_destroy(kmx_plan* plan);
 int kmx_plan_autotune(kmx_plan* plan, void* stream);
 
 /* The variant a plan currently uses for prec (KMX_PREC_FP64 / KMX_PREC_FP32): threads per
- * workgroup (negative: pair split over deg lanes) and whether the row's coefficient strips
- * are staged in LDS. */
+ * workgroup (negative: pair split over deg lanes) and the LDS staging of the row's
+ * coefficient strips (0 none, 1 every dim, 2 every dim but dim 1). */
 int kmx_plan_get_variant(const kmx_plan* plan, int prec, int* threads, int* lds_staged);
 
 /* Level-block path.  Elements sharing a level multi-index form a dense cell

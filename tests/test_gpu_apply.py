@@ -129,7 +129,7 @@ def test_autotune_keeps_results(kmx_gpu):
         assert rel(after, before[p]) <= tol
     # a new plan of the same shape starts from the tuned choice
     plan2 = kmx_gpu.Plan(mats, idx, 3, rp, cols)
-    assert plan2.variant("fp64") == {k: chosen["fp64"][k] for k in ("threads", "lds_staged")}
+    assert plan2.variant("fp64") == {k: chosen["fp64"][k] for k in ("threads", "lds_staged", "lds_mode")}
 
 
 @pytest.mark.parametrize("nd,lev,deg,conn", [(6, 2, 2, True), (5, 2, 2, True), (3, 2, 4, False)])
