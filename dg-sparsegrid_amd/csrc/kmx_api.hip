@@ -66,6 +66,7 @@ void free_plan(kmx_plan* p) {
     dfree(p->d_blkT32);
     dfree(p->d_x_host_stage);
     dfree(p->d_y_host_stage);
+    kmx_sweep_free(p);
     kmx_blocks_free(p);
 }
 

@@ -163,6 +163,8 @@ int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, in
         set_error("kmx_apply: unknown precision");
         return KMX_E_ARG;
     }
+    if (!transpose && prec != KMX_PREC_FP32_GROUPED && p->sw && p->var[prec_slot(prec)].sweep)
+        return kmx_sweep_apply(p, prec_slot(prec), x, y, s);
     if (!transpose && prec != KMX_PREC_FP32_GROUPED && p->bs && p->bs->ok && p->bs->split[prec_slot(prec)].active)
         return kmx_blocks_apply(p, prec_slot(prec), x, y, s);
     kmx_variant v = prec == KMX_PREC_FP32_GROUPED ? p->var[1] : p->var[prec_slot(prec)];
@@ -290,7 +292,7 @@ extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
                         if (lds && !strip_fits(p, elem, lds)) continue;
                         if (lds == 2 && (kind == 1 || p->D < 2)) continue;
                         if ((kind == 0 && !has_reg) || (kind == 1 && (!has_split || nt == 128))) continue;
-                        kmx_variant v{nt, lds, kind, bestv.beta};
+                        kmx_variant v{nt, lds, kind, 0, bestv.beta};
                         float ms;
                         rc = time_calls([&] { return launch_variant(p, false, x, y, prec, v, s); }, s, e0, e1, ms);
                         if (rc != KMX_OK) break;
@@ -322,6 +324,15 @@ extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
             p->var[prec_slot(prec)] = bestv;
             rc = set_split(p, prec, bestb);
             if (rc != KMX_OK) break;
+            // 3. the dimension-sweep path against the best of the above
+            if (p->sw) {
+                p->var[prec_slot(prec)].sweep = 1;
+                float ms;
+                rc = time_calls([&] { return kmx_launch_apply(p, false, x, y, prec, s); }, s, e0, e1, ms);
+                if (rc != KMX_OK) break;
+                bestv.sweep = ms < best ? 1 : 0;
+                p->var[prec_slot(prec)] = bestv;
+            }
         }
         std::lock_guard<std::mutex> g(tune_mutex());
         tune_cache()[key_of(p, prec)] = bestv;
@@ -347,6 +358,30 @@ extern "C" int kmx_plan_set_split(kmx_plan* p, int prec, double beta) {
     }
     p->bs->split[prec_slot(prec)].beta = std::numeric_limits<double>::quiet_NaN();  // force a rebuild
     return set_split(p, prec, beta);
+}
+
+extern "C" int kmx_plan_set_sweep(kmx_plan* p, int prec, int on) {
+    if (!p || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
+        set_error("kmx_plan_set_sweep: bad arguments");
+        return KMX_E_ARG;
+    }
+    if (on && !p->sw) {
+        set_error("kmx_plan_set_sweep: this plan has no dimension-sweep path");
+        return KMX_E_UNSUPPORTED;
+    }
+    p->var[prec_slot(prec)].sweep = on ? 1 : 0;
+    return KMX_OK;
+}
+
+extern "C" int kmx_plan_get_sweep(const kmx_plan* p, int prec, int* available, int* active, double* flops) {
+    if (!p || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
+        set_error("kmx_plan_get_sweep: bad arguments");
+        return KMX_E_ARG;
+    }
+    if (available) *available = p->sw ? 1 : 0;
+    if (active) *active = (p->sw && p->var[prec_slot(prec)].sweep) ? 1 : 0;
+    if (flops) *flops = kmx_sweep_flops(p);
+    return KMX_OK;
 }
 
 extern "C" int kmx_plan_get_split(const kmx_plan* p, int prec, int* structured, double* beta, int64_t* block_pairs,

@@ -733,6 +733,7 @@ static void build_impl(kmx_plan* p, const std::vector<int>& idx, std::vector<lon
             b = int(blv.size());
             bid.emplace(key, b);
             blv.emplace_back(lev.begin() + size_t(e) * D, lev.begin() + size_t(e + 1) * D);
+            bs->blev.insert(bs->blev.end(), lev.begin() + size_t(e) * D, lev.begin() + size_t(e + 1) * D);
         } else {
             b = it->second;
         }
@@ -852,6 +853,41 @@ static void build_impl(kmx_plan* p, const std::vector<int>& idx, std::vector<lon
             bs->bpid[size_t(a) * nblk + b] = int(bs->bps.size());
             bs->bps.push_back(bp);
         }
+    // unconnected block pairs: is their mask product empty as well?
+    bs->full_product = true;
+    for (int a = 0; a < nblk && bs->full_product; ++a)
+        for (int b = 0; b < nblk && bs->full_product; ++b) {
+            if (cnt[size_t(a) * nblk + b] != 0) continue;
+            if (present[a] == ncell[a] && present[b] == ncell[b]) {
+                const int* ba = &bs->binfo[size_t(a) * (2 * D + 1)];
+                const int* bb = &bs->binfo[size_t(b) * (2 * D + 1)];
+                bool empty = false;
+                for (int k = 0; k < D && !empty; ++k) {
+                    bool any = false;
+                    const int nb = p->nb[k];
+                    for (int I = 0; I < (1 << ba[k]) && !any; ++I)
+                        for (int J = 0; J < (1 << bb[k]) && !any; ++J)
+                            any = bs->mask[k][size_t(ba[D + k] + I) * nb + (bb[D + k] + J)] != 0;
+                    empty = !any;
+                }
+                if (!empty) bs->full_product = false;
+            } else {
+                brute += double(present[a]) * present[b];
+                if (brute > 4e8) {
+                    bs->full_product = false;
+                    break;
+                }
+                for (int i : members[a]) {
+                    for (int j : members[b]) {
+                        bool on = true;
+                        for (int k = 0; k < D && on; ++k)
+                            on = bs->mask[k][size_t(idx[size_t(i) * D + k]) * p->nb[k] + idx[size_t(j) * D + k]] != 0;
+                        if (on) bs->full_product = false;
+                    }
+                    if (!bs->full_product) break;
+                }
+            }
+        }
     (void)Lmax;
     // masked matrices
     std::vector<double> m64(size_t(p->mat_elems));
@@ -878,6 +914,7 @@ void kmx_blocks_build(kmx_plan* p, const std::vector<int>& idx, std::vector<long
     p->bs = new kmx_blockset();
     try {
         build_impl(p, idx, std::move(rp), std::move(cols), mats);
+        if (p->bs->ok) kmx_sweep_build(p);
     } catch (const std::exception& e) {
         kmx_blocks_free(p);
         p->bs = new kmx_blockset();

@@ -51,6 +51,7 @@ struct kmx_variant {
     int nt = 0;
     int lds = -1;
     int kind = 0;          // 0: one lane per pair, 1: pair split over deg lanes
+    int sweep = 0;         // 1: the dimension-sweep path (when the plan has one)
     double beta = -2.0;    // level-block split threshold (-2: library default)
 };
 
@@ -89,6 +90,8 @@ struct kmx_blockset {
     std::vector<int> binfo;   // per block: D log2 cell counts, D first 1D indices, cell-table offset
     std::vector<int> elems;   // per block: element of each cell (row-major, dim 1 slowest), -1 if absent
     std::vector<int> eblk, ecell;
+    std::vector<int> blev;    // per block: D levels
+    bool full_product = false;  // also every unconnected block pair has an empty mask product
     std::vector<std::vector<unsigned char>> mask;  // per dim: nb x nb, [I * nb + J]
     struct BP {
         int bi, bj;
@@ -110,6 +113,8 @@ struct kmx_blockset {
     float* d_mat32 = nullptr;
     kmx_split split[3];  // FP64, FP32, BF16 (every block pair factorized)
 };
+
+struct kmx_sweep;  // dimension-sweep path (kmx_sweep.hip)
 
 struct kmx_plan {
     int D = 0, deg = 0, N = 0, T = 0, B = 0;
@@ -151,6 +156,7 @@ struct kmx_plan {
     double* d_y_host_stage = nullptr;
 
     kmx_blockset* bs = nullptr;  // level-block path (null when disabled)
+    kmx_sweep* sw = nullptr;     // dimension-sweep path (null when the structure does not allow it)
 };
 
 // Pair list handed to the pair-by-pair kernels (null members: the plan's own).
@@ -174,3 +180,9 @@ int kmx_blocks_apply(kmx_plan* p, int slot, const double* x, double* y, hipStrea
 void kmx_blocks_update_term(kmx_plan* p, int term, int dim, const double* mat, hipStream_t s);
 void kmx_blocks_free(kmx_plan* p);
 double kmx_blocks_default_beta();
+
+// Dimension-sweep path (kmx_sweep.hip)
+void kmx_sweep_build(kmx_plan* p);
+int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s);
+void kmx_sweep_free(kmx_plan* p);
+double kmx_sweep_flops(const kmx_plan* p);

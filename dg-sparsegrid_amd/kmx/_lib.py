@@ -91,6 +91,9 @@ PROTOTYPES = {
     "kmx_plan_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kmx_plan_get_variant": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "kmx_plan_set_split": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "kmx_plan_set_sweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "kmx_plan_get_sweep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double)]),
     "kmx_realspace_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
                                        C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "kmx_realspace_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

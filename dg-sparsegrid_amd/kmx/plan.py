@@ -154,7 +154,7 @@ class Plan:
     def autotune(self, stream=None):
         """Time the kernel variants on this plan and keep the fastest (kmx_plan_autotune)."""
         check(lib().kmx_plan_autotune(self.handle, _stream(stream)))
-        return {p: dict(self.variant(p), **self.split(p)) for p in ("fp64", "fp32")}
+        return {p: dict(self.variant(p), **self.split(p), sweep=self.sweep(p)) for p in ("fp64", "fp32")}
 
     def variant(self, prec="fp64"):
         nt, lds = C.c_int(), C.c_int()
@@ -165,6 +165,15 @@ class Plan:
         """Level-block split threshold (kmx_plan_set_split): beta < 0 runs every
         block pair that fits factorized, beta = inf none (pair by pair)."""
         check(lib().kmx_plan_set_split(self.handle, _prec(prec), float(beta)))
+
+    def set_sweep(self, prec, on=True):
+        """Use (or stop using) the dimension-sweep path for prec (kmx_plan_set_sweep)."""
+        check(lib().kmx_plan_set_sweep(self.handle, _prec(prec), int(bool(on))))
+
+    def sweep(self, prec="fp64"):
+        av, ac, fl = C.c_int(), C.c_int(), C.c_double()
+        check(lib().kmx_plan_get_sweep(self.handle, _prec(prec), C.byref(av), C.byref(ac), C.byref(fl)))
+        return {"available": bool(av.value), "active": bool(ac.value), "flops": fl.value}
 
     def split(self, prec="fp64"):
         st, beta = C.c_int(), C.c_double()

@@ -124,6 +124,18 @@ int kmx_plan_set_split(kmx_plan* plan, int prec, double beta);
 int kmx_plan_get_split(const kmx_plan* plan, int prec, int* structured, double* beta, int64_t* block_pairs,
                        int64_t* pairs_in_blocks, int64_t* remaining_pairs, double* flops);
 
+/* Dimension-sweep path of a KMX_PLAN_LEVEL_BLOCKS plan: when every pair of
+ * level groups is the tensor product of the 1D connections (connected or not)
+ * and the level tuples form a downward-closed set (sparse and full grids),
+ * the apply runs as d sweeps, one GEMM per (source prefix, target suffix) of
+ * level groups per dimension, on the matrix cores -- the same terms in
+ * another order.  set: on = 1 uses it for prec (KMX_E_UNSUPPORTED when the
+ * plan has none), 0 goes back; kmx_plan_autotune also times it.  get:
+ * whether the plan has it, whether prec uses it, and its flops per apply.
+ * No counterpart in the reference. */
+int kmx_plan_set_sweep(kmx_plan* plan, int prec, int on);
+int kmx_plan_get_sweep(const kmx_plan* plan, int prec, int* available, int* active, double* flops);
+
 /* num_pairs = P, block = deg^d, dof = N*deg^d. */
 int kmx_plan_info(const kmx_plan* plan, int64_t* num_pairs, int* block, int64_t* dof);
 

@@ -143,3 +143,67 @@ def test_update_term_and_scale_reach_level_blocks(kmx_gpu):
             assert plan.split(prec)["block_pairs"] > 0
             y = plan.apply(xd, prec=prec).cpu().numpy()
             assert rel(y, yo) <= tol, (beta, prec)
+
+
+SWEEP_CASES = [
+    (2, 3, 2, 2, False), (2, 3, 2, 2, True), (2, 5, 3, 3, True), (2, 4, 4, 2, False),
+    (3, 4, 2, 3, True), (3, 3, 3, 3, False), (3, 5, 3, 3, True), (4, 3, 2, 4, True), (6, 3, 2, 6, True),
+]
+
+
+@pytest.mark.parametrize("nd,lev,deg,T,conn", SWEEP_CASES)
+def test_dimension_sweep_matches_oracle(kmx_gpu, nd, lev, deg, T, conn):
+    """The dimension-sweep path (GEMM per level-group prefix/suffix per dim,
+    matrix cores) against the oracle: FP64 1e-13, FP32 5e-6."""
+    import torch
+
+    mats, idx, rp, cols, x = case(nd, lev, deg, T, conn, seed=31 + nd * 10 + lev)
+    yo = oracle_y(mats, idx, deg, rp, cols, x)
+    plan = kmx_gpu.Plan(mats, idx, deg, rp, cols, index_base=1, fp32=True, level_blocks=True)
+    info = plan.sweep("fp64")
+    assert info["available"], plan.split("fp64")
+    xd = torch.from_numpy(x).cuda()
+    for prec, tol in (("fp64", TOL64), ("fp32", TOL32)):
+        plan.set_sweep(prec, True)
+        assert plan.sweep(prec)["active"]
+        y = plan.apply(xd, prec=prec).cpu().numpy()
+        assert rel(y, yo) <= tol, (prec, rel(y, yo))
+        y2 = plan.apply(xd, prec=prec).cpu().numpy()  # a second apply reuses the boxes
+        assert np.array_equal(y, y2)
+        plan.set_sweep(prec, False)
+    plan.destroy()
+
+
+def test_dimension_sweep_cfg2_size_and_scales(kmx_gpu):
+    import torch
+
+    mats, idx, rp, cols, x = case(3, 7, 3, 3, True, seed=2)
+    yo = oracle_y(mats, idx, 3, rp, cols, x)
+    plan = kmx_gpu.Plan(mats, idx, 3, rp, cols, index_base=1, fp32=True, level_blocks=True)
+    assert plan.sweep("fp64")["available"]
+    xd = torch.from_numpy(x).cuda()
+    plan.set_sweep("fp64", True)
+    assert rel(plan.apply(xd, prec="fp64").cpu().numpy(), yo) <= TOL64
+    plan.set_term_scale(1, 0.25)
+    mats2 = [list(m) for m in mats]
+    mats2[1] = [mats[1][0] * 0.25, mats[1][1], mats[1][2]]
+    assert rel(plan.apply(xd, prec="fp64").cpu().numpy(), oracle_y(mats2, idx, 3, rp, cols, x)) <= TOL64
+
+
+def test_dimension_sweep_adapted_grid(kmx_gpu):
+    import torch
+    from kmx import grid as G
+
+    rng = np.random.default_rng(12)
+    base = G.hash_table_sparse_nd([4, 4, 4], 6, "SG")
+    table = G.refine_random(base, 6, 0.6, rng)
+    lev = int(table.lev.max())
+    n = 2 * 2 ** lev
+    mats = [[rng.standard_normal((n, n)) for _ in range(3)] for _ in range(2)]
+    x = rng.standard_normal(table.num_elem * 8)
+    yo = oracle_y(mats, table.idx1d, 2, None, None, x)
+    plan = kmx_gpu.Plan(mats, table.idx1d, 2, None, None, index_base=1, fp32=True, level_blocks=True)
+    if not plan.sweep("fp64")["available"]:
+        pytest.skip("refined level set is not downward closed")
+    plan.set_sweep("fp64", True)
+    assert rel(plan.apply(torch.from_numpy(x).cuda(), prec="fp64").cpu().numpy(), yo) <= TOL64
