@@ -193,6 +193,50 @@ def precision_leg(steps=50):
             "N": plan.N, "P": plan.P, "block": plan.B, "runs": runs, "backward_euler": be}
 
 
+def sweep_study(steps=50):
+    """Pair kernels against the opt-in dimension-sweep path on three
+    structured workloads (3D lev 7 deg 3 connect_nD, 6D lev 4 deg 2
+    connect_nD, 2D lev 8 deg 3 all pairs, random 1D matrices where no PDE is
+    named): time per apply and the sweep's error against the FP64 pair path."""
+    import torch
+
+    import kmx
+    from kmx import coeffs as K
+    from kmx import grid as G
+
+    rng = np.random.default_rng(0)
+    op = build_operator()
+    cases = [("3D lev7 deg3 connect_nD", op["mats"], op["table"].idx1d, 3, op["rp"], op["cols"])]
+    mats6 = K.get_coeff_mats(K.diffusion_nd(6, 4), 2)
+    t6 = G.hash_table_nd([4] * 6, "SG")
+    rp6, c6 = G.connect_nd(t6, 4, 4, "SG")
+    cases.append(("6D lev4 deg2 connect_nD", mats6, t6.idx1d, 2, rp6, c6))
+    t2 = G.hash_table_sparse_nd([8, 8], 8, "SG")
+    n2 = 3 * 256
+    cases.append(("2D lev8 deg3 all pairs", [[rng.standard_normal((n2, n2)) for _ in range(2)] for _ in range(4)],
+                  t2.idx1d, 3, None, None))
+    rows = []
+    for name, mats, idx, deg, rp, cols in cases:
+        plan = kmx.Plan(mats, idx, deg, rp, cols, index_base=1, fp32=True, level_blocks=True)
+        x = torch.from_numpy(rng.standard_normal(plan.dof)).cuda()
+        y = torch.empty_like(x)
+        row = {"workload": name, "P": plan.P, "sweep_available": plan.sweep()["available"],
+               "sweep_gflop": round(plan.sweep()["flops"] / 1e9, 4)}
+        if row["sweep_available"]:
+            y64 = plan.apply(x, prec="fp64").cpu().numpy()
+            for prec in ("fp64", "fp32"):
+                plan.set_sweep(prec, False)
+                row[prec + "_pairs_ms"] = round(time_apply(plan, x, y, prec, 5, steps), 4)
+                plan.set_sweep(prec, True)
+                row[prec + "_sweep_ms"] = round(time_apply(plan, x, y, prec, 5, steps), 4)
+                ys = plan.apply(x, prec=prec).cpu().numpy()
+                row[prec + "_sweep_rel_err"] = float(np.linalg.norm(ys - y64) / np.linalg.norm(y64))
+                plan.set_sweep(prec, False)
+        rows.append(row)
+        plan.destroy()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +245,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched leg")
     ap.add_argument("--no-precision", action="store_true", help="skip the 6D precision leg")
+    ap.add_argument("--sweep-study", action="store_true",
+                    help="only time the pair kernels against the opt-in dimension-sweep path (one JSON line)")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="for counter runs: one 256 MiB device copy (calibration), 3 applies per precision, "
                          "then 3 level-block applies per precision of the 6D plan (unless --no-precision)")
@@ -215,6 +261,9 @@ def main():
     import kmx
 
     torch.cuda.set_device(0)
+    if a.sweep_study:
+        print(json.dumps({"sweep_study": sweep_study()}), flush=True)
+        return
     op = build_operator()
     xd = torch.from_numpy(op["x"]).cuda()
     yd = torch.empty_like(xd)
