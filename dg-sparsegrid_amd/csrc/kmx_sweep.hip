@@ -64,7 +64,7 @@ struct SwGemm {
     int M, N, K;
 };
 
-template <typename T>
+template <typename T, int NC>
 __global__ void __launch_bounds__(256) k_sweep_stage(const T* __restrict__ mat, T* __restrict__ arena,
                                                      const SwGemm* __restrict__ gemms, const int4* __restrict__ tiles,
                                                      int ntiles) {
@@ -83,29 +83,38 @@ __global__ void __launch_bounds__(256) k_sweep_stage(const T* __restrict__ mat, 
     const int fbc = bok ? fb : 0;
     const T* ap = mat + g.a_off + (aok ? ia : 0);
     const T* bp = arena + g.b_off + (fbc % g.bE0) + (long long)(fbc / g.bE0) * g.bs1;
-    typename SwMfma<T>::V acc = {T(0), T(0), T(0), T(0)};
-    // K in steps of 4 (lane group lk takes j = 4 s + lk); a last partial step
-    // (K % 4) reads column / row 0 for j >= K and feeds zeros
+    // K in steps of 4 (lane group lk takes j = 4 s + lk), split into NC
+    // independent chains of consecutive steps so that NC loads and NC matrix
+    // operations are in flight per wave; the partial tiles are added at the end
+    // (fixed order: deterministic).  Steps and columns past K read column / row 0
+    // and feed zeros.  Tiles of short K run with NC = 1.
     const int K = g.K;
-    const int K4 = K & ~3;
-    const T* pa = ap + lk * g.lda;
-    const T* pb = bp + lk * int(g.bk);
-    const int sa = 4 * g.lda, sb = 4 * int(g.bk);
-    int k0 = 0, step = 0;
-#pragma unroll 4
-    for (; k0 < K4; k0 += 4, ++step) {
-        const T av = pa[step * sa];
-        const T bv = pb[step * sb];
-        acc = SwMfma<T>::run(aok ? av : T(0), bok ? bv : T(0), acc);
+    const int nsteps = (K + 3) >> 2;
+    const int Q = (nsteps + NC - 1) / NC;
+    typename SwMfma<T>::V acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = typename SwMfma<T>::V{T(0), T(0), T(0), T(0)};
+    const int bk = int(g.bk);
+#pragma unroll 2
+    for (int u = 0; u < Q; ++u) {
+        T av[NC], bv[NC];
+        bool ok[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int j = 4 * (c * Q + u) + lk;
+            ok[c] = j < K;
+            const int jc = ok[c] ? j : 0;
+            av[c] = ap[jc * g.lda];
+            bv[c] = bp[jc * bk];
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            acc[c] = SwMfma<T>::run((aok && ok[c]) ? av[c] : T(0), (bok && ok[c]) ? bv[c] : T(0), acc[c]);
     }
-    if (k0 < K) {
-        const int j = k0 + lk;
-        const bool jok = j < K;
-        const int jc = jok ? j : 0;
-        const T av = ap[jc * g.lda];
-        const T bv = bp[jc * int(g.bk)];
-        acc = SwMfma<T>::run((aok && jok) ? av : T(0), (bok && jok) ? bv : T(0), acc);
-    }
+#pragma unroll
+    for (int c = 1; c < NC; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[0][r] += acc[c][r];
     if (!bok) return;
     const long long cf = (fb % g.cE0) + (long long)((fb / g.cE0) % g.cE1) * g.cs1 +
                          (long long)(fb / (g.cE0 * g.cE1)) * g.cs2;
@@ -113,7 +122,7 @@ __global__ void __launch_bounds__(256) k_sweep_stage(const T* __restrict__ mat, 
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = m0 + SwMfma<T>::row(lane, r);
-        if (i < g.M) cp[(long long)i * g.cm] = acc[r];
+        if (i < g.M) cp[(long long)i * g.cm] = acc[0][r];
     }
 }
 
@@ -158,6 +167,7 @@ struct kmx_sweep {
     long long xn = 0;     // stage-0 box elements (at arena offset 0)
     long long yoff[KMX_MAX_TERMS] = {0};
     std::vector<int> stage_gemm0, stage_tile0;  // per stage: first gemm / tile (D + 1 entries)
+    std::vector<int> stage_short;               // per stage: first tile with K < 16 (fewer than 4 steps)
     int* d_xmap = nullptr;
     long long* d_ymap = nullptr;
     SwGemm* d_gemms = nullptr;
@@ -318,6 +328,9 @@ void kmx_sweep_build(kmx_plan* p) {
             // longest tiles (largest K) first, so that they start in the first wave of the launch
             std::stable_sort(tiles.begin() + sw->stage_tile0.back(), tiles.end(),
                              [&](const int4& u, const int4& v) { return gemms[u.x].K > gemms[v.x].K; });
+            int sh = sw->stage_tile0.back();
+            while (sh < int(tiles.size()) && gemms[tiles[sh].x].K >= 16) ++sh;
+            sw->stage_short.push_back(sh);
         }
         sw->stage_gemm0.push_back(int(gemms.size()));
         sw->stage_tile0.push_back(int(tiles.size()));
@@ -412,11 +425,13 @@ int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream
         using T = std::remove_pointer_t<decltype(arena)>;
         hipLaunchKernelGGL(k_sweep_gather<T>, dim3(grid_of(sw->xn)), dim3(256), 0, s, x, sw->d_xmap, arena, sw->xn);
         for (int st = 0; st < sw->D; ++st) {
-            const int t0 = sw->stage_tile0[st], t1 = sw->stage_tile0[st + 1];
-            const int nt = t1 - t0;
-            if (nt == 0) continue;
-            hipLaunchKernelGGL(k_sweep_stage<T>, dim3((nt + 3) / 4), dim3(256), 0, s, mat, arena, sw->d_gemms,
-                               sw->d_tiles + t0, nt);
+            const int t0 = sw->stage_tile0[st], ts = sw->stage_short[st], t1 = sw->stage_tile0[st + 1];
+            if (ts > t0)  // long K: four chains per wave
+                hipLaunchKernelGGL((k_sweep_stage<T, 4>), dim3((ts - t0 + 3) / 4), dim3(256), 0, s, mat, arena,
+                                   sw->d_gemms, sw->d_tiles + t0, ts - t0);
+            if (t1 > ts)
+                hipLaunchKernelGGL((k_sweep_stage<T, 1>), dim3((t1 - ts + 3) / 4), dim3(256), 0, s, mat, arena,
+                                   sw->d_gemms, sw->d_tiles + ts, t1 - ts);
         }
         hipLaunchKernelGGL(k_sweep_scatter<T>, dim3(grid_of(nval)), dim3(256), 0, s, arena, sw->d_ymap, y, nval, sa);
     };
