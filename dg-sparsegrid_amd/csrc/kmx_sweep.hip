@@ -1,26 +1,40 @@
-// kmx_sweep.hip -- the operator apply as d dimension sweeps over the sparse grid.
+// kmx_sweep.hip -- the operator apply as dimension sweeps that stay on the sparse grid.
 //
 // apply_A.m:100-237 sums, for every target element and every connected source
 // element, (A_{t,1}[I_1,J_1] x ... x A_{t,d}[I_d,J_d]) x_j.  When the pair
-// list is, between every pair of level groups, exactly the tensor product of
+// list is, between every pair of level tuples, exactly the tensor product of
 // per-dimension 1D connections (verified at plan time, as connect_nD.m:63-106
 // builds it; an all-pairs list trivially) and the set of level tuples is
 // downward closed (every sparse or full grid is), the sum factorizes by
-// dimension: contract dim d for every source prefix (levels of dims 1..d-1)
-// against every target row of dim d, then dim d-1, ..., then dim 1.  Because
-// the levels of one dimension are contiguous ranges of its 1D index, each
-// contraction is a plain GEMM with the masked 1D matrix:
+// dimension.  Each 1D operator splits by level into a "down" part (target
+// level <= source level) and an "up" part (target level > source level).
+// Expanding the product over these parts and applying, for every choice, all
+// down parts before any up part keeps every intermediate on the grid's own
+// level tuples: a down step lowers a level below the source's, an up step
+// raises one towards the target's.  Sharing the common work of the choices
+// gives the recursion
 //
-//   stage j, group (p, q)   p = source levels of dims < j, q = target levels of dims > j
-//     C[r_j, (p-cells, q-cells)] = sum_{r'_j} Ahat_j[r_j, r'_j] * Z[(p-cells), r'_j, (q-cells)]
+//   F(z, d)   = F(Down_d z, d+1) + Up_d(F(z, d+1))     (d < D-1)
+//   F(z, D-1) = A_{D-1} z                              (the whole 1D operator)
 //
-// with r'_j running over every source level present under prefix p and r_j
-// over every target level present above suffix q.  Same terms as the
-// pair-by-pair sum, added in another order (7.5x fewer flops on a 3D level-7
-// sparse grid with degree 3).  Each GEMM is cut into 16x16 output tiles, one
-// wave per tile on the matrix cores (16x16x4 FP64 / FP32); every output is
-// written by exactly one wave (no atomics).  Per apply: a gather of x into
-// the stage-0 boxes, d stage launches, and a scatter summing the terms.
+// run breadth-first: D-1 down launches (branch b of depth d reads slot b and
+// writes slot b | 2^d), one launch of whole operators on the 2^(D-1) branches,
+// and D-1 up launches that halve the branches and add, the last of them
+// summing the terms and writing y.  Same terms as the pair-by-pair sum, added
+// in another order (22x fewer flops on a 3D level-7 sparse grid, degree 3).
+//
+// Data: every intermediate is a set of dense boxes, one per level tuple L,
+// holding prod_k deg*cells(L_k) values with dimension 1 slowest.  An op on
+// dimension d groups the boxes that agree off dimension d (rest tuple R):
+// their dimension-d positions, levels 0 to m(R) in order, are exactly the
+// hierarchical 1D positions 0 .. deg*2^m - 1, so the op is a GEMM with the
+// leading block of the masked 1D matrix, rows and columns cut by level for the
+// down and up parts.  Output tiles of 16x16 run on one wave each (or on up to
+// four waves splitting a long K range, summed in a fixed order) on the matrix
+// cores (16x16x4 FP64 / FP32); every output is written by exactly one wave.
+// Everything a wave needs about its tile (row bases and levels of the output
+// rows, the box offset of every source level) is precomputed into a 512-byte
+// work record that the wave reads in one coalesced load.
 #include "kmx_internal.h"
 
 #include <algorithm>
@@ -54,107 +68,200 @@ struct SwMfma<float> {
     static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-// One GEMM of a stage.  A(m, k) = mat[a_off + m + k * lda]; B(k, f) =
-// arena[b_off + k * bk + (f % bE0) + (f / bE0) * bs1]; C(m, f) =
-// arena[c_off + m * cm + (f % cE0) + ((f / cE0) % cE1) * cs1 + (f / (cE0 * cE1)) * cs2].
-struct SwGemm {
-    long long a_off, b_off, c_off;
-    long long bk, bs1, cs1, cs2;
-    int lda, bE0, cm, cE0, cE1;
-    int M, N, K;
+enum { UD_DOWN = 0, UD_FULL = 1, UD_UP = 2 };
+
+// Work record of one wave (UD_REC ints, stored so that lane j reads ints j and
+// 64 + j as one int2).  The tile: rows [m0, m0+16) of the group's positions
+// along d, columns [f0, f0+16) of its (o, i) pairs (O x I), branch b, output
+// term tout; its K range: source levels [la, lb], sper steps of four
+// positions per term.  This wave's share: steps [slo, ...) of term tlo up to
+// step shi of term tlo + tn - 1.  The gsize waves from wave gbase of the
+// workgroup hold the same tile (gsize 0: empty slot).  Per output row i:
+// box_l + p * I, nl * I and level l (-1 past the group); per source level l:
+// box offset.
+enum {
+    R_M0, R_F0, R_B, R_TOUT, R_O, R_I, R_LA, R_LB, R_SPER, R_GSIZE, R_GBASE, R_MAGIC, R_TLO, R_SLO, R_TN, R_SHI,
+    R_ROWBASE = 16,
+    R_ROWNL = 32,
+    R_ROWLEV = 48,
+    UD_BOX = 64,
+    UD_REC = 128
 };
+constexpr int UD_MAXLEV = UD_REC - UD_BOX - 1;  // highest level a record holds a box for
+constexpr int UD_WAVES = 8;                     // waves per workgroup
 
-template <typename T, int NC>
-__global__ void __launch_bounds__(256) k_sweep_stage(const T* __restrict__ mat, T* __restrict__ arena,
-                                                     const SwGemm* __restrict__ gemms, const int4* __restrict__ tiles,
-                                                     int ntiles) {
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wave >= ntiles) return;  // whole waves leave together
-    const int lane = threadIdx.x & 63;
-    const int li = lane & 15, lk = lane >> 4;
-    const int4 tl = tiles[wave];
-    const SwGemm g = gemms[tl.x];
-    const int m0 = tl.y, f0 = tl.z;
-    // operand rows / fibres this lane feeds, clamped in range (fed as zero when outside)
-    const int ia = m0 + li;
-    const bool aok = ia < g.M;
-    const int fb = f0 + li;
-    const bool bok = fb < g.N;
-    const int fbc = bok ? fb : 0;
-    const T* ap = mat + g.a_off + (aok ? ia : 0);
-    const T* bp = arena + g.b_off + (fbc % g.bE0) + (long long)(fbc / g.bE0) * g.bs1;
-    // K in steps of 4 (lane group lk takes j = 4 s + lk), split into NC
-    // independent chains of consecutive steps so that NC loads and NC matrix
-    // operations are in flight per wave; the partial tiles are added at the end
-    // (fixed order: deterministic).  Steps and columns past K read column / row 0
-    // and feed zeros.  Tiles of short K run with NC = 1.
-    const int K = g.K;
-    const int nsteps = (K + 3) >> 2;
-    const int Q = (nsteps + NC - 1) / NC;
-    typename SwMfma<T>::V acc[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] = typename SwMfma<T>::V{T(0), T(0), T(0), T(0)};
-    const int bk = int(g.bk);
-#pragma unroll 2
-    for (int u = 0; u < Q; ++u) {
-        T av[NC], bv[NC];
-        bool ok[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const int j = 4 * (c * Q + u) + lk;
-            ok[c] = j < K;
-            const int jc = ok[c] ? j : 0;
-            av[c] = ap[jc * g.lda];
-            bv[c] = bp[jc * bk];
-        }
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-            acc[c] = SwMfma<T>::run((aok && ok[c]) ? av[c] : T(0), (bok && ok[c]) ? bv[c] : T(0), acc[c]);
-    }
-#pragma unroll
-    for (int c = 1; c < NC; ++c)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[0][r] += acc[c][r];
-    if (!bok) return;
-    const long long cf = (fb % g.cE0) + (long long)((fb / g.cE0) % g.cE1) * g.cs1 +
-                         (long long)(fb / (g.cE0 * g.cE1)) * g.cs2;
-    T* cp = arena + g.c_off + cf;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int i = m0 + SwMfma<T>::row(lane, r);
-        if (i < g.M) cp[(long long)i * g.cm] = acc[0][r];
-    }
-}
-
-// stage-0 boxes from x (absent cells: 0)
-template <typename T>
-__global__ void __launch_bounds__(256) k_sweep_gather(const double* __restrict__ x, const int* __restrict__ map,
-                                                      T* __restrict__ out, long long n) {
-    for (long long v = blockIdx.x * 256ll + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
-        const int s = map[v];
-        out[v] = s >= 0 ? T(x[s]) : T(0);
-    }
-}
-
-struct ScatterArgs {
-    long long yoff[KMX_MAX_TERMS];
+// Slot b of term t of an intermediate starts at base + (b * T + t) * nb; slot 0
+// of an input with x0 set is x itself, read through the box map.  part: which
+// source levels a row takes (UD_DOWN / UD_FULL / UD_UP, the last adding slot
+// b | add_bit); final: sum the terms, scaled, into y.
+struct UdOp {
+    int d, deg, lda, T;
+    int part, final, x0, out_bit, add_bit;
+    long long nb;
+    long long in_base, out_base, add_base;
+    long long a_off[KMX_MAX_TERMS];  // column-major masked matrix of (term, d)
     double scale[KMX_MAX_TERMS];
-    int T;
 };
 
-// y[v] = sum_t scale_t * Y_t[map[v]]
-template <typename T>
-__global__ void __launch_bounds__(256) k_sweep_scatter(const T* __restrict__ arena, const long long* __restrict__ map,
-                                                       double* __restrict__ y, long long n, ScatterArgs s) {
-    for (long long v = blockIdx.x * 256ll + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
-        const long long m = map[v];
-        double acc = 0.0;
-        for (int t = 0; t < s.T; ++t) acc += s.scale[t] * double(arena[s.yoff[t] + m]);
-        y[v] = acc;
+constexpr int UD_BATCH = 8;  // steps whose loads are issued before their matrix operations
+
+template <typename T, int PART, bool FINAL>
+__global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ mat, T* __restrict__ arena,
+                                                          const int2* __restrict__ recs, UdOp op,
+                                                          const double* __restrict__ x, const int* __restrict__ xmap,
+                                                          double* __restrict__ y, const int* __restrict__ ymap) {
+    using V = typename SwMfma<T>::V;
+    __shared__ T part[UD_WAVES][4][64];
+    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int2 rv = recs[((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane];
+    const int gsize = __builtin_amdgcn_readlane(rv.x, R_GSIZE);
+    const int gbase = __builtin_amdgcn_readlane(rv.x, R_GBASE);
+    const bool leader = gsize > 0 && w == gbase;
+    V acc = V{T(0), T(0), T(0), T(0)};
+    int oaddr[4] = {0, 0, 0, 0};
+    bool ook[4] = {false, false, false, false};
+    T addv[4] = {T(0), T(0), T(0), T(0)};
+    int yi[4] = {-1, -1, -1, -1};
+    const int b = __builtin_amdgcn_readlane(rv.x, R_B);
+    const int tout = __builtin_amdgcn_readlane(rv.x, R_TOUT);
+    if (gsize > 0) {
+        const int m0 = __builtin_amdgcn_readlane(rv.x, R_M0), f0 = __builtin_amdgcn_readlane(rv.x, R_F0);
+        const int O = __builtin_amdgcn_readlane(rv.x, R_O), I = __builtin_amdgcn_readlane(rv.x, R_I);
+        const int li = lane & 15, lk = lane >> 4;
+        const int lr = __shfl(rv.x, R_ROWLEV + li);  // level of A row m0 + li (-1: past the group)
+        const bool rok = lr >= 0;
+        const int r0 = rok ? m0 + li : 0;
+        const int f = f0 + li;
+        const bool fok = f < O * I;
+        const int fc = fok ? f : 0;
+        const unsigned magic = unsigned(__builtin_amdgcn_readlane(rv.x, R_MAGIC));
+        const int fo = I == 1 ? fc : magic ? int(__umulhi(unsigned(fc), magic)) : fc / I;
+        const int fi = fc - fo * I;
+
+        if (leader) {  // output addresses of this lane's four rows; addends and y indices fetched now
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = SwMfma<T>::row(lane, q);
+                const int rb = __shfl(rv.x, R_ROWBASE + row), rn = __shfl(rv.x, R_ROWNL + row);
+                const int rl = __shfl(rv.x, R_ROWLEV + row);
+                ook[q] = fok && rl >= 0;
+                oaddr[q] = ook[q] ? rb + fo * rn + fi : 0;
+            }
+            if (PART == UD_UP) {
+                const int ta = FINAL ? 0 : tout, tb = FINAL ? op.T : tout + 1;
+                for (int t = ta; t < tb; ++t) {
+                    const T* add = arena + op.add_base + ((long long)(b | op.add_bit) * op.T + t) * op.nb;
+                    const T sc = FINAL ? T(op.scale[t]) : T(1);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) addv[q] += sc * add[oaddr[q]];
+                }
+            }
+            if (FINAL)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) yi[q] = ymap[oaddr[q]];
+        }
+
+        const int la = __builtin_amdgcn_readlane(rv.x, R_LA);
+        const int sper = __builtin_amdgcn_readlane(rv.x, R_SPER), tlo = __builtin_amdgcn_readlane(rv.x, R_TLO);
+        const int slo = __builtin_amdgcn_readlane(rv.x, R_SLO), tn = __builtin_amdgcn_readlane(rv.x, R_TN);
+        const int shi = __builtin_amdgcn_readlane(rv.x, R_SHI);
+        const bool xin = op.x0 && b == 0;  // B is x, read through the box map
+        for (int k = 0; k < tn; ++k) {
+            const int t = tlo + k;
+            const int s0 = k == 0 ? slo : 0, s1 = k == tn - 1 ? shi : sper;
+            const T* abase = mat + op.a_off[t];
+            const T* bbase = arena + (xin ? 0 : op.in_base + ((long long)b * op.T + t) * op.nb);
+            const T sc = FINAL ? T(op.scale[t]) : T(1);
+            int l = la, rem = s0;  // level and step of step s0 (wave-uniform)
+            for (;;) {
+                const int ns = (op.deg * (l ? 1 << (l - 1) : 1) + 3) >> 2;
+                if (rem < ns) break;
+                rem -= ns;
+                ++l;
+            }
+            int p = 4 * rem;
+            int nl = 0, aidx = 0, bidx = 0;
+            bool aon = false, newlev = true;
+            for (int q = s0; q < s1; q += UD_BATCH) {
+                // 1. per-lane offsets of the batch (0 where masked); the cursor moves on the scalar unit
+                int ai[UD_BATCH], bi[UD_BATCH];
+                bool ma[UD_BATCH], mb[UD_BATCH];
+#pragma unroll
+                for (int c = 0; c < UD_BATCH; ++c) {
+                    if (newlev) {
+                        const int st = l ? 1 << (l - 1) : 0;
+                        nl = op.deg * (l ? st : 1);
+                        const int box = __builtin_amdgcn_readlane(rv.y, min(l, UD_MAXLEV));
+                        aon = rok && (PART == UD_FULL || (PART == UD_DOWN ? lr <= l : lr > l));
+                        aidx = r0 + (op.deg * st + p + lk) * op.lda;
+                        bidx = box + (fo * nl + p + lk) * I + fi;
+                        newlev = false;
+                    }
+                    const bool on = q + c < s1;
+                    const bool pin = on && p + lk < nl;
+                    ai[c] = pin ? aidx : 0;
+                    bi[c] = pin ? bidx : 0;
+                    ma[c] = pin && aon;  // A element (row li, position): row and level masks
+                    mb[c] = pin && fok;  // B element (position, column li)
+                    if (on) {
+                        p += 4;
+                        aidx += 4 * op.lda;
+                        bidx += 4 * I;
+                        if (p >= nl) {
+                            p = 0;
+                            ++l;
+                            newlev = true;
+                        }
+                    }
+                }
+                // 2. every load of the batch in flight at once
+                T av[UD_BATCH], bv[UD_BATCH];
+                if (xin) {
+                    int xm[UD_BATCH];
+#pragma unroll
+                    for (int c = 0; c < UD_BATCH; ++c) {
+                        av[c] = abase[ai[c]];
+                        xm[c] = xmap[bi[c]];
+                    }
+#pragma unroll
+                    for (int c = 0; c < UD_BATCH; ++c) {
+                        mb[c] = mb[c] && xm[c] >= 0;  // absent cell: x is 0 there
+                        bv[c] = T(x[xm[c] >= 0 ? xm[c] : 0]);
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < UD_BATCH; ++c) {
+                        av[c] = abase[ai[c]];
+                        bv[c] = bbase[bi[c]];
+                    }
+                }
+                // 3. matrix operations (masked steps feed zeros)
+#pragma unroll
+                for (int c = 0; c < UD_BATCH; ++c)
+                    acc = SwMfma<T>::run(ma[c] ? av[c] * sc : T(0), mb[c] ? bv[c] : T(0), acc);
+            }
+        }
+        if (gsize > 1 && !leader)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) part[w][q][lane] = acc[q];
+    }
+    __syncthreads();  // every wave of every workgroup, split tiles or not
+    if (!leader) return;
+    for (int v = 1; v < gsize; ++v)  // the tile's partial sums in wave order
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] += part[w + v][q][lane];
+    if (FINAL) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (ook[q] && yi[q] >= 0) y[yi[q]] = double(acc[q] + addv[q]);
+    } else {
+        T* out = arena + op.out_base + ((long long)(b | op.out_bit) * op.T + tout) * op.nb;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (ook[q]) out[oaddr[q]] = acc[q] + addv[q];
     }
 }
-
-unsigned grid_of(long long n) { return unsigned(std::max<long long>(1, std::min<long long>((n + 255) / 256, 65536))); }
 
 }  // namespace
 }  // namespace kmx
@@ -163,15 +270,18 @@ using namespace kmx;
 
 struct kmx_sweep {
     int D = 0;
+    long long nb = 0;     // values per intermediate (every box of every level tuple)
     long long arena = 0;  // elements
-    long long xn = 0;     // stage-0 box elements (at arena offset 0)
-    long long yoff[KMX_MAX_TERMS] = {0};
-    std::vector<int> stage_gemm0, stage_tile0;  // per stage: first gemm / tile (D + 1 entries)
-    std::vector<int> stage_short;               // per stage: first tile with K < 16 (fewer than 4 steps)
+    struct Op {
+        int part;
+        bool final;
+        UdOp a;
+        int wg0, wg1;  // workgroups (UD_WAVES work records each) of this op
+    };
+    std::vector<Op> ops;
     int* d_xmap = nullptr;
-    long long* d_ymap = nullptr;
-    SwGemm* d_gemms = nullptr;
-    int4* d_tiles = nullptr;
+    int* d_ymap = nullptr;
+    int* d_recs = nullptr;
     void* d_arena[2] = {nullptr, nullptr};  // FP64, FP32
     long long flops = 0;                    // useful flops per apply (all terms)
 };
@@ -180,38 +290,33 @@ namespace {
 
 using Lev = std::vector<int>;
 
-long long cum_cells(int m) { return 1LL << m; }                 // cells in levels 0..m
 long long cells_of(int l) { return l == 0 ? 1 : 1LL << (l - 1); }  // cells of level l
 long long start_of(int l) { return l == 0 ? 0 : 1LL << (l - 1); }  // first 1D index of level l
 
-struct Builder {
-    int D, deg, T;
-    std::set<Lev> S;                 // level tuples present
-    std::vector<std::set<Lev>> P;    // P[j] = {L[0:j]}
-    std::vector<std::set<Lev>> Q;    // Q[j] = {L[j+1:D]}
-    long long arena = 0;
+int host_level(long long s, int deg) {
+    const long long idx = s / deg;
+    int l = 0;
+    while ((1LL << l) <= idx) ++l;
+    return l;  // idx in [2^(l-1), 2^l)
+}
 
-    long long fibres(const Lev& l) const {
-        long long f = 1;
-        for (int v : l) f *= cells_of(v) * deg;
-        return f;
-    }
-    // source levels of dim j present under prefix p (0..max): K rows
-    long long Kof(int j, const Lev& p) const {
-        int m = -1;
-        for (const Lev& L : S)
-            if (std::equal(p.begin(), p.end(), L.begin())) m = std::max(m, L[j]);
-        if (m < 0) throw std::logic_error("empty prefix");
-        return deg * cum_cells(m);
-    }
-    // target levels of dim j present above suffix q (0..max): M rows
-    long long Mof(int j, const Lev& q) const {
-        int m = -1;
-        for (const Lev& L : S)
-            if (std::equal(q.begin(), q.end(), L.begin() + j + 1)) m = std::max(m, L[j]);
-        if (m < 0) throw std::logic_error("empty suffix");
-        return deg * cum_cells(m);
-    }
+// steps of four source positions of levels la..lb
+int steps_of(int la, int lb, int deg) {
+    int n = 0;
+    for (int l = la; l <= lb; ++l) n += int((deg * cells_of(l) + 3) / 4);
+    return n;
+}
+
+// The boxes R u {d: l}, l = 0..m, of one op dimension d: O / I = values of the
+// dimensions before / after d; box = their offsets, level 0 first.
+struct Group {
+    int m, O, I;
+    std::vector<int> box;
+    unsigned magic;  // s / I = umulhi(s, magic) for every column s < O * I (0: divide)
+};
+
+struct HostTile {
+    int m0, f0, b, t, group, la, lb, nterm, sper, steps, gsize;
 };
 
 }  // namespace
@@ -220,186 +325,334 @@ void kmx_sweep_build(kmx_plan* p) {
     kmx_blockset* bs = p->bs;
     if (!bs || !bs->ok || !bs->full_product || p->T > KMX_MAX_TERMS) return;
     const int D = p->D, deg = p->deg, T = p->T;
-    Builder b{D, deg, T, {}, {}, {}, 0};
-    for (int k = 0; k < bs->nblk; ++k) b.S.insert(Lev(bs->blev.begin() + size_t(k) * D, bs->blev.begin() + size_t(k + 1) * D));
+    std::set<Lev> S;
+    for (int k = 0; k < bs->nblk; ++k) S.insert(Lev(bs->blev.begin() + size_t(k) * D, bs->blev.begin() + size_t(k + 1) * D));
     // downward closed: lowering any level keeps the tuple in the set
-    for (const Lev& L : b.S)
+    for (const Lev& L : S)
         for (int k = 0; k < D; ++k)
             if (L[k] > 0) {
                 Lev M = L;
                 --M[k];
-                if (!b.S.count(M)) return;
+                if (!S.count(M)) return;
             }
-    b.P.resize(D + 1);
-    b.Q.resize(D);
-    for (const Lev& L : b.S) {
-        for (int j = 0; j <= D; ++j) b.P[j].insert(Lev(L.begin(), L.begin() + j));
-        for (int j = 0; j < D; ++j) b.Q[j].insert(Lev(L.begin() + j + 1, L.end()));
-    }
     auto* sw = new kmx_sweep();
     sw->D = D;
     try {
-        // ---- boxes.  O[j] (output of stage j, j = D..0; O[D] = gathered x):
-        //   j = D:    key p (levels of dims 0..D-2):  [p fibres] x [r'_{D-1}: K_{D-1}(p)]
-        //   0<j<D:    key (ps, q), ps = dims 0..j-2, q = dims j+1..D-1:
-        //             [ps fibres] x [r'_{j-1}: K_{j-1}(ps)] x [r_j: M_j(q)] x [q fibres]
-        //   j = 0:    key q (dims 1..D-1): [r_0: M_0(q)] x [q fibres]
-        long long off = 0;
-        std::map<Lev, long long> xbox;  // O[D]
-        for (const Lev& pp : b.P[D - 1]) {
-            xbox[pp] = off;
-            off += b.fibres(pp) * b.Kof(D - 1, pp);
+        // ---- boxes: one per level tuple, dimension 1 slowest
+        std::map<Lev, long long> boxoff;
+        long long nb = 0;
+        for (const Lev& L : S) {
+            boxoff[L] = nb;
+            long long sz = 1;
+            for (int k = 0; k < D; ++k) sz *= deg * cells_of(L[k]);
+            nb += sz;
         }
-        sw->xn = off;
-        // per term: O[j] boxes for j = D-1..0
-        std::vector<std::vector<std::map<std::pair<Lev, Lev>, long long>>> obox(
-            T, std::vector<std::map<std::pair<Lev, Lev>, long long>>(D));
-        for (int t = 0; t < T; ++t) {
-            for (int j = D - 1; j >= 1; --j)
-                for (const Lev& ps : b.P[j - 1])
-                    for (const Lev& q : b.Q[j]) {
-                        obox[t][j][{ps, q}] = off;
-                        off += b.fibres(ps) * b.Kof(j - 1, ps) * b.Mof(j, q) * b.fibres(q);
+        if (nb >= (1LL << 31) - 16) throw std::runtime_error("sweep boxes beyond 32-bit offsets");
+        if ((long long)p->N * p->B >= (1LL << 31)) throw std::runtime_error("sweep values beyond 32-bit indices");
+        sw->nb = nb;
+        for (int k = 0; k < D; ++k)
+            if ((long long)p->n[k] * p->n[k] >= (1LL << 31)) throw std::runtime_error("1D matrix beyond 32-bit offsets");
+        // ---- groups per op dimension
+        std::vector<Group> groups;
+        std::vector<int> group0(D + 1);
+        std::vector<long long> fl_full(D, 0), fl_down(D, 0), fl_up(D, 0);  // one branch, one term
+        for (int d = 0; d < D; ++d) {
+            group0[d] = int(groups.size());
+            std::map<Lev, int> rmax;
+            for (const Lev& L : S) {
+                Lev R = L;
+                R.erase(R.begin() + d);
+                auto it = rmax.find(R);
+                if (it == rmax.end()) rmax[R] = L[d];
+                else it->second = std::max(it->second, L[d]);
+            }
+            for (const auto& kv : rmax) {
+                Group g{};
+                g.m = kv.second;
+                if (g.m > UD_MAXLEV) throw std::runtime_error("sweep level beyond the work record");
+                if (((long long)deg << g.m) > p->n[d]) throw std::logic_error("sweep group beyond the 1D matrix");
+                long long O = 1, I = 1;
+                for (int k = 0; k < D - 1; ++k) (k < d ? O : I) *= deg * cells_of(kv.first[k]);
+                if (O * I * ((long long)deg << g.m) >= (1LL << 31)) throw std::runtime_error("sweep group too large");
+                g.O = int(O);
+                g.I = int(I);
+                g.magic = 0;
+                if (I > 1) {  // exact when (largest column) * (magic * I - 2^32) < 2^32
+                    const unsigned long long mg = (1ULL << 32) / (unsigned long long)I + 1;
+                    const unsigned long long e = mg * (unsigned long long)I - (1ULL << 32);
+                    if (mg < (1ULL << 32) && (unsigned long long)(O * I) * e < (1ULL << 32)) g.magic = unsigned(mg);
+                }
+                for (int l = 0; l <= g.m; ++l) {
+                    Lev L = kv.first;
+                    L.insert(L.begin() + d, l);
+                    const long long off = boxoff.at(L);  // present: downward closed
+                    if (off + O * I * deg * cells_of(l) > nb) throw std::logic_error("sweep box outside the boxes");
+                    g.box.push_back(int(off));
+                }
+                for (int lt = 0; lt <= g.m; ++lt)
+                    for (int ls = 0; ls <= g.m; ++ls) {
+                        const long long fl = 2LL * deg * cells_of(lt) * deg * cells_of(ls) * O * I;
+                        fl_full[d] += fl;
+                        (lt <= ls ? fl_down[d] : fl_up[d]) += fl;
                     }
-            sw->yoff[t] = off;
-            for (const Lev& q : b.Q[0]) {
-                obox[t][0][{Lev(), q}] = off;
-                off += b.Mof(0, q) * b.fibres(q);
+                groups.push_back(g);
             }
         }
-        sw->arena = off;
-        // ---- GEMMs and tiles, stage by stage (dim D-1 first)
-        std::vector<SwGemm> gemms;
-        std::vector<int4> tiles;
-        for (int j = D - 1; j >= 0; --j) {
-            sw->stage_gemm0.push_back(int(gemms.size()));
-            sw->stage_tile0.push_back(int(tiles.size()));
-            for (int t = 0; t < T; ++t)
-                for (const Lev& pp : b.P[j])
-                    for (const Lev& q : b.Q[j]) {
-                        SwGemm g{};
-                        const long long K = b.Kof(j, pp), M = b.Mof(j, q);
-                        const long long pf = b.fibres(pp), qf = b.fibres(q);
-                        g.M = int(M);
-                        g.K = int(K);
-                        if (pf * qf > (1LL << 30)) throw std::runtime_error("stage GEMM too wide");
-                        g.N = int(pf * qf);
-                        g.a_off = p->off[t * D + j];
-                        g.lda = p->n[j];
-                        // B: the previous stage's box
-                        if (j == D - 1) {
-                            g.b_off = xbox.at(pp);
-                            g.bk = 1;
-                            g.bE0 = 1;
-                            g.bs1 = K;
-                        } else {
-                            // q = (l_{j+1}, qr): box O[j+1] keyed (pp, qr), rows r_{j+1} of level l_{j+1}
-                            const int l1 = q[0];
-                            const Lev qr(q.begin() + 1, q.end());
-                            const long long Mn = b.Mof(j + 1, qr), qrf = b.fibres(qr);
-                            g.b_off = obox[t][j + 1].at({pp, qr}) + deg * start_of(l1) * qrf;
-                            g.bk = Mn * qrf;
-                            g.bE0 = int(cells_of(l1) * deg * qrf);
-                            g.bs1 = K * Mn * qrf;
+        group0[D] = int(groups.size());
+        for (const Group& g : groups)  // the device column division is exact on every column
+            if (g.magic)
+                for (long long f : {0LL, (long long)g.O * g.I - 1, (long long)g.I - 1, (long long)g.I})
+                    if (f < (long long)g.O * g.I &&
+                        (long long)(((unsigned long long)f * g.magic) >> 32) != f / g.I)
+                        throw std::logic_error("sweep column division");
+        // ---- arena: slots 1..2^(D-1)-1 of every term (down outputs; later the up outputs,
+        // from slot 0 on), then slots 0..2^(D-1)-1 of every term (whole operators).  Slot 0
+        // of the down / whole-operator inputs is x, read through the box map.
+        const long long half = 1LL << (D - 1);
+        const long long zb = 0;
+        const long long gb = zb + std::max<long long>(half - 1, 1) * T * nb;
+        sw->arena = gb + half * T * nb;
+        if (sw->arena > (1LL << 40)) throw std::runtime_error("sweep arena too large");
+        // ---- ops and their work records
+        std::vector<std::vector<int>> recs;  // one per wave, UD_REC ints
+        auto base_op = [&](int d) {
+            UdOp a{};
+            a.d = d;
+            a.deg = deg;
+            a.lda = p->n[d];
+            a.T = T;
+            a.nb = nb;
+            for (int t = 0; t < T; ++t) {
+                a.a_off[t] = p->off[t * D + d];
+                if (a.a_off[t] < 0 || a.a_off[t] + (long long)p->n[d] * p->n[d] > p->mat_elems)
+                    throw std::logic_error("sweep matrix outside the matrices");
+                a.scale[t] = 1.0;
+            }
+            return a;
+        };
+        auto add_op = [&](int part, bool fin, UdOp a, long long nbranch) {
+            a.part = part;
+            a.final = fin ? 1 : 0;
+            kmx_sweep::Op op{part, fin, a, int(recs.size() / UD_WAVES), 0};
+            const int d = a.d;
+            const int nterm = fin ? T : 1;
+            std::vector<HostTile> ot;
+            for (long long b = 0; b < nbranch; ++b)
+                for (int t = 0; t < (fin ? 1 : T); ++t)
+                    for (int gi = group0[d]; gi < group0[d + 1]; ++gi) {
+                        const Group& g = groups[gi];
+                        const int n = deg << g.m;
+                        for (int m0 = 0; m0 < n; m0 += 16) {
+                            // source levels: all (whole operator), >= the rows' lowest (down), < their highest (up)
+                            int la = 0, lb = g.m;
+                            if (part == UD_DOWN) la = host_level(m0, deg);
+                            if (part == UD_UP) lb = host_level(std::min(n, m0 + 16) - 1, deg) - 1;
+                            const int sper = steps_of(la, lb, deg);
+                            const int steps = nterm * sper;
+                            int gsize = 1;  // waves sharing the tile: at most one batch of steps each
+                            while (gsize < UD_WAVES && (steps + gsize - 1) / gsize > UD_BATCH) gsize *= 2;
+                            for (int f0 = 0; f0 < g.O * g.I; f0 += 16)
+                                ot.push_back(HostTile{m0, f0, int(b), t, gi, la, lb, nterm, sper, steps, gsize});
                         }
-                        // C: this stage's box
-                        g.cm = int(qf);
-                        g.cE0 = int(qf);
-                        if (j == 0) {
-                            g.c_off = obox[t][0].at({Lev(), q});
-                            g.cE1 = 1;
-                            g.cs1 = 0;
-                            g.cs2 = 0;
-                        } else {
-                            const Lev ps(pp.begin(), pp.end() - 1);
-                            const int lp = pp.back();
-                            const long long Kp = b.Kof(j - 1, ps);
-                            g.c_off = obox[t][j].at({ps, q}) + deg * start_of(lp) * M * qf;
-                            g.cE1 = int(cells_of(lp) * deg);
-                            g.cs1 = M * qf;
-                            g.cs2 = Kp * M * qf;
-                        }
-                        sw->flops += 2LL * M * K * pf * qf;
-                        const int gi = int(gemms.size());
-                        gemms.push_back(g);
-                        for (long long m0 = 0; m0 < M; m0 += 16)
-                            for (long long f0 = 0; f0 < pf * qf; f0 += 16) tiles.push_back(make_int4(gi, int(m0), int(f0), 0));
                     }
-            // longest tiles (largest K) first, so that they start in the first wave of the launch
-            std::stable_sort(tiles.begin() + sw->stage_tile0.back(), tiles.end(),
-                             [&](const int4& u, const int4& v) { return gemms[u.x].K > gemms[v.x].K; });
-            int sh = sw->stage_tile0.back();
-            while (sh < int(tiles.size()) && gemms[tiles[sh].x].K >= 16) ++sh;
-            sw->stage_short.push_back(sh);
+            // widest tiles first (power-of-two wave counts then never straddle a workgroup), longest first
+            std::stable_sort(ot.begin(), ot.end(), [](const HostTile& u, const HostTile& v) {
+                return u.gsize != v.gsize ? u.gsize > v.gsize : u.steps > v.steps;
+            });
+            for (const HostTile& h : ot) {
+                const int slot = int(recs.size() % UD_WAVES);
+                if (slot % h.gsize) throw std::logic_error("sweep tile straddles a workgroup");
+                const Group& g = groups[h.group];
+                const int n = deg << g.m;
+                const int qn = (h.steps + h.gsize - 1) / h.gsize;
+                for (int w = 0; w < h.gsize; ++w) {
+                    std::vector<int> r(UD_REC, 0);
+                    r[R_M0] = h.m0;
+                    r[R_F0] = h.f0;
+                    r[R_B] = h.b;
+                    r[R_TOUT] = h.t;
+                    r[R_O] = g.O;
+                    r[R_I] = g.I;
+                    r[R_LA] = h.la;
+                    r[R_LB] = h.lb;
+                    r[R_SPER] = h.sper;
+                    r[R_GSIZE] = h.gsize;
+                    r[R_GBASE] = slot;
+                    r[R_MAGIC] = int(g.magic);
+                    const int q0 = std::min(h.steps, w * qn), q1 = std::min(h.steps, (w + 1) * qn);
+                    if (q1 > q0) {
+                        r[R_TLO] = h.t + q0 / h.sper;
+                        r[R_SLO] = q0 % h.sper;
+                        r[R_TN] = (q1 - 1) / h.sper - q0 / h.sper + 1;
+                        r[R_SHI] = (q1 - 1) % h.sper + 1;
+                    } else {
+                        r[R_TLO] = h.t;
+                    }
+                    for (int i = 0; i < 16; ++i) {
+                        const int row = h.m0 + i;
+                        if (row < n) {
+                            const int l = host_level(row, deg);
+                            r[R_ROWBASE + i] = g.box[l] + int(row - deg * start_of(l)) * g.I;
+                            r[R_ROWNL + i] = int(deg * cells_of(l)) * g.I;
+                            r[R_ROWLEV + i] = l;
+                        } else {
+                            r[R_ROWLEV + i] = -1;
+                        }
+                    }
+                    for (int l = 0; l <= g.m; ++l) r[UD_BOX + l] = g.box[l];
+                    recs.push_back(std::move(r));
+                }
+            }
+            while (recs.size() % UD_WAVES) recs.push_back(std::vector<int>(UD_REC, 0));  // empty slots
+            op.wg1 = int(recs.size() / UD_WAVES);
+            sw->ops.push_back(op);
+        };
+        // host checks need each record's tile: rebuilt from the records themselves below
+        long long flops = 0;
+        if (D == 1) {
+            UdOp a = base_op(0);
+            a.x0 = 1;
+            add_op(UD_FULL, true, a, 1);
+            flops += fl_full[0] * T;
+        } else {
+            for (int d = 0; d + 1 < D; ++d) {  // down: slot b -> slot b | 2^d
+                UdOp a = base_op(d);
+                a.x0 = 1;
+                a.in_base = zb - T * nb;
+                a.out_base = zb - T * nb;
+                a.out_bit = 1 << d;
+                add_op(UD_DOWN, false, a, 1LL << d);
+                flops += fl_down[d] * T * (1LL << d);
+            }
+            {  // whole operator on dimension D-1: slot b -> whole-operator slot b
+                UdOp a = base_op(D - 1);
+                a.x0 = 1;
+                a.in_base = zb - T * nb;
+                a.out_base = gb;
+                add_op(UD_FULL, false, a, half);
+                flops += fl_full[D - 1] * T * half;
+            }
+            long long src = gb;  // up: slot b + slot b | 2^d -> slot b of the other array
+            for (int d = D - 2; d >= 0; --d) {
+                const long long dst = src == gb ? zb : gb;
+                UdOp a = base_op(d);
+                a.in_base = src;
+                a.add_base = src;
+                a.add_bit = 1 << d;
+                a.out_base = dst;
+                add_op(UD_UP, d == 0, a, 1LL << d);
+                flops += fl_up[d] * T * (1LL << d);
+                src = dst;
+            }
         }
-        sw->stage_gemm0.push_back(int(gemms.size()));
-        sw->stage_tile0.push_back(int(tiles.size()));
-        // ---- gather map (stage-0 boxes <- x) and scatter map (y <- term boxes, offsets relative to yoff[t])
-        std::vector<int> xmap(size_t(sw->xn), -1);
-        std::vector<long long> ymap(size_t(p->N) * p->B, 0);
+        sw->flops = flops;
+        // ---- box maps: x -> boxes (read side) and boxes -> y
+        std::vector<int> xmap(size_t(nb), -1);
+        std::vector<int> ymap(size_t(nb), -1);
         for (int e = 0; e < p->N; ++e) {
             const int blk = bs->eblk[e];
             const int* bi = &bs->binfo[size_t(blk) * (2 * D + 1)];
             Lev L(bs->blev.begin() + size_t(blk) * D, bs->blev.begin() + size_t(blk + 1) * D);
-            std::vector<long long> cell(D);
-            {
-                long long ci = bs->ecell[e];
-                for (int k = D - 1; k >= 0; --k) {
-                    cell[k] = ci & ((1LL << bi[k]) - 1);
-                    ci >>= bi[k];
-                }
+            std::vector<long long> cell(D), stride(D);
+            long long ci = bs->ecell[e];
+            for (int k = D - 1; k >= 0; --k) {
+                if ((1LL << bi[k]) != cells_of(L[k]) || bi[D + k] != start_of(L[k]))
+                    throw std::logic_error("sweep block layout");
+                cell[k] = ci & ((1LL << bi[k]) - 1);
+                ci >>= bi[k];
             }
-            const Lev pp(L.begin(), L.end() - 1);
-            const long long Kx = b.Kof(D - 1, pp);
-            const Lev q(L.begin() + 1, L.end());
-            const long long qf = b.fibres(q);
+            long long sd = 1;
+            for (int k = D - 1; k >= 0; --k) {
+                stride[k] = sd;
+                sd *= deg * cells_of(L[k]);
+            }
+            const long long b0 = boxoff.at(L);
             for (int a = 0; a < p->B; ++a) {
-                std::vector<int> av(D);
+                long long pos = b0;
                 int rest = a;
                 for (int k = D - 1; k >= 0; --k) {
-                    av[k] = rest % deg;
+                    pos += (cell[k] * deg + rest % deg) * stride[k];
                     rest /= deg;
                 }
-                // x box: p fibre (dims 0..D-2) x r'_{D-1}
-                long long pfi = 0;
-                for (int k = 0; k < D - 1; ++k) pfi = pfi * (cells_of(L[k]) * deg) + cell[k] * deg + av[k];
-                const long long rx = (start_of(L[D - 1]) + cell[D - 1]) * deg + av[D - 1];
-                xmap[size_t(xbox.at(pp) + pfi * Kx + rx)] = e * p->B + a;
-                // y box: r_0 x q fibre (dims 1..D-1)
-                long long qfi = 0;
-                for (int k = 1; k < D; ++k) qfi = qfi * (cells_of(L[k]) * deg) + cell[k] * deg + av[k];
-                const long long r0 = (start_of(L[0]) + cell[0]) * deg + av[0];
-                ymap[size_t(e) * p->B + a] = (obox[0][0].at({Lev(), q}) - sw->yoff[0]) + r0 * qf + qfi;
+                if (pos < 0 || pos >= nb || xmap[size_t(pos)] != -1) throw std::logic_error("sweep map collision");
+                xmap[size_t(pos)] = e * p->B + a;
+                ymap[size_t(pos)] = e * p->B + a;
             }
         }
-        // every address a stage can touch stays inside its array (checked here, on the host)
-        for (const SwGemm& g : gemms) {
-            const long long amax = g.a_off + (g.M - 1) + (long long)(g.K - 1) * g.lda;
-            const long long bmax = g.b_off + (long long)(g.K - 1) * g.bk + (std::min(g.N, g.bE0) - 1) +
-                                   (long long)((g.N - 1) / g.bE0) * g.bs1;
-            const long long cmax = g.c_off + (long long)(g.M - 1) * g.cm + (std::min(g.N, g.cE0) - 1) +
-                                   (long long)(std::min((g.N - 1) / g.cE0, g.cE1 - 1)) * g.cs1 +
-                                   (long long)((g.N - 1) / (g.cE0 * g.cE1)) * g.cs2;
-            if (g.bk * 4LL * g.K >= (1LL << 31) || 4LL * g.lda * g.K >= (1LL << 31))
-                throw std::logic_error("sweep strides beyond 32-bit offsets");
-            if (g.M < 1 || g.N < 1 || g.K < 1 || g.bE0 < 1 || g.cE0 < 1 || g.cE1 < 1 || g.a_off < 0 || g.b_off < 0 ||
-                g.c_off < 0 || amax >= p->mat_elems || bmax >= sw->arena || cmax >= sw->arena)
-                throw std::logic_error("sweep GEMM outside its arrays");
+        // ---- every record: its tile inside a group of its op, its slots inside the arena,
+        // its rows and boxes as the group defines them (checked here, on the host)
+        auto slot_ok = [&](long long off) { return off >= 0 && off + nb <= sw->arena; };
+        for (const auto& op : sw->ops) {
+            const long long nbr = op.final ? 1 : (op.part == UD_FULL ? half : (1LL << op.a.d));
+            for (int wg = op.wg0; wg < op.wg1; ++wg)
+                for (int w = 0; w < UD_WAVES; ++w) {
+                    const std::vector<int>& r = recs[size_t(wg) * UD_WAVES + w];
+                    if (r[R_GSIZE] == 0) continue;
+                    const int gbase = r[R_GBASE], gsize = r[R_GSIZE];
+                    if (gsize < 1 || gsize > UD_WAVES || gbase < 0 || gbase > w || w >= gbase + gsize ||
+                        gbase + gsize > UD_WAVES)
+                        throw std::logic_error("sweep record wave group");
+                    for (int v = gbase; v < gbase + gsize; ++v) {  // the waves of one tile agree on it
+                        const std::vector<int>& o = recs[size_t(wg) * UD_WAVES + v];
+                        for (int k : {int(R_M0), int(R_F0), int(R_B), int(R_TOUT), int(R_O), int(R_I), int(R_LA),
+                                      int(R_LB), int(R_SPER), int(R_GSIZE), int(R_GBASE)})
+                            if (o[k] != r[k]) throw std::logic_error("sweep record wave group");
+                    }
+                    // the group: same O, I and level-0 box
+                    const Group* gp = nullptr;
+                    for (int gi = group0[op.a.d]; gi < group0[op.a.d + 1] && !gp; ++gi)
+                        if (groups[gi].O == r[R_O] && groups[gi].I == r[R_I] && groups[gi].box[0] == r[UD_BOX])
+                            gp = &groups[gi];
+                    if (!gp) throw std::logic_error("sweep record group");
+                    const Group& g = *gp;
+                    const int n = deg << g.m;
+                    const int sper = steps_of(r[R_LA], r[R_LB], deg);
+                    if (r[R_M0] < 0 || r[R_M0] >= n || r[R_M0] % 16 || r[R_F0] < 0 || r[R_F0] >= g.O * g.I ||
+                        r[R_LA] < 0 || r[R_LB] > g.m || r[R_SPER] != sper || r[R_B] < 0 || r[R_B] >= nbr ||
+                        r[R_TOUT] < 0 || r[R_TOUT] >= T || r[R_MAGIC] != int(g.magic))
+                        throw std::logic_error("sweep record outside its group");
+                    if (r[R_TN] < 0 || (r[R_TN] > 0 && (r[R_TLO] < 0 || r[R_TLO] + r[R_TN] > T || r[R_SLO] < 0 ||
+                                                        r[R_SLO] >= sper || r[R_SHI] < 1 || r[R_SHI] > sper ||
+                                                        (!op.final && (r[R_TN] != 1 || r[R_TLO] != r[R_TOUT])))))
+                        throw std::logic_error("sweep record steps");
+                    for (int i = 0; i < 16; ++i) {
+                        const int row = r[R_M0] + i;
+                        const int l = row < n ? host_level(row, deg) : -1;
+                        if (r[R_ROWLEV + i] != l ||
+                            (l >= 0 && (r[R_ROWBASE + i] != g.box[l] + int(row - deg * start_of(l)) * g.I ||
+                                        r[R_ROWNL + i] != int(deg * cells_of(l)) * g.I)))
+                            throw std::logic_error("sweep record rows");
+                    }
+                    for (int l = 0; l <= g.m; ++l)
+                        if (r[UD_BOX + l] != g.box[l]) throw std::logic_error("sweep record boxes");
+                    for (int t = r[R_TLO]; t < r[R_TLO] + r[R_TN]; ++t)
+                        if (!(op.a.x0 && r[R_B] == 0) && !slot_ok(op.a.in_base + ((long long)r[R_B] * T + t) * nb))
+                            throw std::logic_error("sweep input slot outside the arena");
+                    for (int t = 0; t < T; ++t) {
+                        if (!op.final && t != r[R_TOUT]) continue;
+                        if (op.part == UD_UP &&
+                            !slot_ok(op.a.add_base + ((long long)(r[R_B] | op.a.add_bit) * T + t) * nb))
+                            throw std::logic_error("sweep add slot outside the arena");
+                        if (!op.final && !slot_ok(op.a.out_base + ((long long)(r[R_B] | op.a.out_bit) * T + t) * nb))
+                            throw std::logic_error("sweep output slot outside the arena");
+                    }
+                }
         }
-        for (int v : xmap)
-            if (v < -1 || v >= p->N * p->B) throw std::logic_error("sweep gather map out of range");
-        for (long long v : ymap)
-            for (int t = 0; t < T; ++t)
-                if (v < 0 || sw->yoff[t] + v >= sw->arena) throw std::logic_error("sweep scatter map out of range");
+        // records interleaved so that lane j reads ints j and 64 + j as one int2
+        std::vector<int> flat(recs.size() * UD_REC);
+        for (size_t i = 0; i < recs.size(); ++i)
+            for (int j = 0; j < UD_REC / 2; ++j) {
+                flat[i * UD_REC + 2 * j] = recs[i][j];
+                flat[i * UD_REC + 2 * j + 1] = recs[i][UD_REC / 2 + j];
+            }
         auto up = [](const auto* h, size_t n, auto*& d) {
-            if (hipMalloc(&d, n * sizeof(*h)) != hipSuccess) throw std::bad_alloc();
-            if (hipMemcpy(d, h, n * sizeof(*h), hipMemcpyHostToDevice) != hipSuccess)
+            if (hipMalloc(&d, std::max<size_t>(1, n) * sizeof(*h)) != hipSuccess) throw std::bad_alloc();
+            if (n && hipMemcpy(d, h, n * sizeof(*h), hipMemcpyHostToDevice) != hipSuccess)
                 throw std::runtime_error("hipMemcpy failed");
         };
         up(xmap.data(), xmap.size(), sw->d_xmap);
         up(ymap.data(), ymap.size(), sw->d_ymap);
-        up(gemms.data(), gemms.size(), sw->d_gemms);
-        up(tiles.data(), tiles.size(), sw->d_tiles);
+        up(flat.data(), flat.size(), sw->d_recs);
         if (hipMalloc(&sw->d_arena[0], size_t(sw->arena) * sizeof(double)) != hipSuccess) throw std::bad_alloc();
         if (bs->d_mat32 && hipMalloc(&sw->d_arena[1], size_t(sw->arena) * sizeof(float)) != hipSuccess)
             throw std::bad_alloc();
@@ -413,29 +666,21 @@ void kmx_sweep_build(kmx_plan* p) {
 int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s) {
     kmx_sweep* sw = p->sw;
     kmx_blockset* bs = p->bs;
-    const bool f64 = slot == 0;
-    ScatterArgs sa;
-    sa.T = p->T;
-    for (int t = 0; t < p->T; ++t) {
-        sa.yoff[t] = sw->yoff[t];
-        sa.scale[t] = p->scale[t];
-    }
-    const long long nval = (long long)p->N * p->B;
     auto run = [&](auto* arena, const auto* mat) {
         using T = std::remove_pointer_t<decltype(arena)>;
-        hipLaunchKernelGGL(k_sweep_gather<T>, dim3(grid_of(sw->xn)), dim3(256), 0, s, x, sw->d_xmap, arena, sw->xn);
-        for (int st = 0; st < sw->D; ++st) {
-            const int t0 = sw->stage_tile0[st], ts = sw->stage_short[st], t1 = sw->stage_tile0[st + 1];
-            if (ts > t0)  // long K: four chains per wave
-                hipLaunchKernelGGL((k_sweep_stage<T, 4>), dim3((ts - t0 + 3) / 4), dim3(256), 0, s, mat, arena,
-                                   sw->d_gemms, sw->d_tiles + t0, ts - t0);
-            if (t1 > ts)
-                hipLaunchKernelGGL((k_sweep_stage<T, 1>), dim3((t1 - ts + 3) / 4), dim3(256), 0, s, mat, arena,
-                                   sw->d_gemms, sw->d_tiles + ts, t1 - ts);
+        for (const auto& op : sw->ops) {
+            UdOp a = op.a;
+            if (op.final)
+                for (int t = 0; t < p->T; ++t) a.scale[t] = p->scale[t];
+            const dim3 grid(unsigned(op.wg1 - op.wg0)), block(UD_WAVES * 64);
+            const int2* rc = reinterpret_cast<const int2*>(sw->d_recs) + size_t(op.wg0) * UD_WAVES * (UD_REC / 2);
+            auto* kern = op.part == UD_DOWN ? k_ud_op<T, UD_DOWN, false>
+                         : op.part == UD_FULL ? (op.final ? k_ud_op<T, UD_FULL, true> : k_ud_op<T, UD_FULL, false>)
+                                              : (op.final ? k_ud_op<T, UD_UP, true> : k_ud_op<T, UD_UP, false>);
+            hipLaunchKernelGGL(kern, grid, block, 0, s, mat, arena, rc, a, x, sw->d_xmap, y, sw->d_ymap);
         }
-        hipLaunchKernelGGL(k_sweep_scatter<T>, dim3(grid_of(nval)), dim3(256), 0, s, arena, sw->d_ymap, y, nval, sa);
     };
-    if (f64) {
+    if (slot == 0) {
         run(static_cast<double*>(sw->d_arena[0]), bs->d_mat64);
     } else {
         if (!sw->d_arena[1] || !bs->d_mat32) {
@@ -457,8 +702,7 @@ void kmx_sweep_free(kmx_plan* p) {
     if (!sw) return;
     if (sw->d_xmap) hipFree(sw->d_xmap);
     if (sw->d_ymap) hipFree(sw->d_ymap);
-    if (sw->d_gemms) hipFree(sw->d_gemms);
-    if (sw->d_tiles) hipFree(sw->d_tiles);
+    if (sw->d_recs) hipFree(sw->d_recs);
     for (auto& a : sw->d_arena)
         if (a) hipFree(a);
     delete sw;
@@ -466,3 +710,4 @@ void kmx_sweep_free(kmx_plan* p) {
 }
 
 double kmx_sweep_flops(const kmx_plan* p) { return p->sw ? double(p->sw->flops) : 0.0; }
+

@@ -153,8 +153,9 @@ SWEEP_CASES = [
 
 @pytest.mark.parametrize("nd,lev,deg,T,conn", SWEEP_CASES)
 def test_dimension_sweep_matches_oracle(kmx_gpu, nd, lev, deg, T, conn):
-    """The dimension-sweep path (GEMM per level-group prefix/suffix per dim,
-    matrix cores) against the oracle: FP64 1e-13, FP32 5e-6."""
+    """The dimension-sweep path (each 1D operator split into its down and up
+    parts by level, intermediates on the grid's own level tuples, matrix cores)
+    against the oracle: FP64 1e-13, FP32 5e-6."""
     import torch
 
     mats, idx, rp, cols, x = case(nd, lev, deg, T, conn, seed=31 + nd * 10 + lev)
@@ -207,3 +208,30 @@ def test_dimension_sweep_adapted_grid(kmx_gpu):
         pytest.skip("refined level set is not downward closed")
     plan.set_sweep("fp64", True)
     assert rel(plan.apply(torch.from_numpy(x).cuda(), prec="fp64").cpu().numpy(), yo) <= TOL64
+
+
+ADAPTED_SWEEP_CASES = [  # (dims, base level, refined max level, degree, terms, seed)
+    (2, 3, 4, 2, 2, 1), (2, 3, 5, 3, 1, 2), (3, 3, 4, 2, 1, 3), (2, 4, 5, 2, 1, 4),
+]
+
+
+@pytest.mark.parametrize("nd,base_lev,ref_lev,deg,T,seed", ADAPTED_SWEEP_CASES)
+def test_dimension_sweep_adapted_shapes(kmx_gpu, nd, base_lev, ref_lev, deg, T, seed):
+    """Partially filled level blocks (absent cells read as zero through the box
+    map) on the dimension-sweep path, FP64 and FP32, against the oracle."""
+    import torch
+    from kmx import grid as G
+
+    rng = np.random.default_rng(seed)
+    table = G.refine_random(G.hash_table_sparse_nd([base_lev] * nd, base_lev + 2, "SG"), ref_lev, 0.6, rng)
+    n = deg * 2 ** int(table.lev.max())
+    mats = [[rng.standard_normal((n, n)) for _ in range(nd)] for _ in range(T)]
+    x = rng.standard_normal(table.num_elem * deg ** nd)
+    yo = oracle_y(mats, table.idx1d, deg, None, None, x)
+    plan = kmx_gpu.Plan(mats, table.idx1d, deg, None, None, index_base=1, fp32=True, level_blocks=True)
+    assert plan.sweep("fp64")["available"]
+    xd = torch.from_numpy(x).cuda()
+    for prec, tol in (("fp64", TOL64), ("fp32", TOL32)):
+        plan.set_sweep(prec, True)
+        assert rel(plan.apply(xd, prec=prec).cpu().numpy(), yo) <= tol, prec
+    plan.destroy()
