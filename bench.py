@@ -6,7 +6,10 @@ deg x deg-block operator per dimension and term), the maximal connect_nD pair
 list (P = 1,903,000 element pairs over N = 1696 elements), x ~ N(0,1) seeded.
 A "step" is one full operator application y = A x.  The FP64 kernel and the
 mixed-precision (FP32) kernels are timed on the same resident operator;
-`value` is the faster mixed-precision variant's applies/s.  Extra legs: 64
+`value` is the faster pair-by-pair mixed-precision variant's applies/s.  The
+same operator also runs through the dimension sweep (a level-block plan: same
+terms in another summation order) as the runs "fp64_sweep" / "fp32_sweep".
+Extra legs: 64
 adapted instances advanced by a HIP-graph RK3 step (--no-batch skips), the
 6D lev 4 deg 2 precision study with FP32 / BF16 errors and a refined backward
 Euler step (--no-precision skips), and the CPU oracle on the host cores
@@ -34,17 +37,18 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
 # HBM bytes per launch of the pair-by-pair kernel on this workload, by
-# (precision, workgroup size, LDS-staged strips): rocprofv3 FETCH_SIZE and
-# WRITE_SIZE counter runs of `bench.py --pmc-pass`, fetch scaled by 2.0 and
-# write by 1.0 (what the same counters read for a 256 MiB device copy).
+# (precision, workgroup size, strip mode: 0 blocks from L2, 1 every strip in
+# LDS, 2 strips of dims 2..d in LDS): rocprofv3 FETCH_SIZE and WRITE_SIZE
+# counter runs of `bench.py --pmc-pass`, fetch scaled by 2.0 and write by 1.0
+# (what the same counters read for a 256 MiB device copy).
 # Source: profiles/apply_cfg2_hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    ("fp64", 128, False): 18585550, ("fp64", 128, True): 31532832,
-    ("fp64", 256, False): 18546674, ("fp64", 256, True): 31547371,
-    ("fp64", 512, False): 18522944, ("fp64", 512, True): 31535202,
-    ("fp32", 128, False): 15928128, ("fp32", 128, True): 21634979,
-    ("fp32", 256, False): 15885653, ("fp32", 256, True): 21634832,
-    ("fp32", 512, False): 15846711, ("fp32", 512, True): 21612037,
+    ("fp64", 128, 0): 18585395, ("fp64", 128, 1): 31547699, ("fp64", 128, 2): 28021555,
+    ("fp64", 256, 0): 18546790, ("fp64", 256, 1): 31549850, ("fp64", 256, 2): 28034560,
+    ("fp64", 512, 0): 18522624, ("fp64", 512, 1): 31550669, ("fp64", 512, 2): 28005888,
+    ("fp32", 128, 0): 15928013, ("fp32", 128, 1): 21643776, ("fp32", 128, 2): 20102246,
+    ("fp32", 256, 0): 15885619, ("fp32", 256, 1): 21643264, ("fp32", 256, 2): 20082586,
+    ("fp32", 512, 0): 15846912, ("fp32", 512, 1): 21621043, ("fp32", 512, 2): 20045107,
 }
 
 
@@ -309,7 +313,23 @@ def main():
         err[name] = float(np.linalg.norm(yp - y64) / np.linalg.norm(y64))
         maxabs[name] = float(np.max(np.abs(yp - y64)))
         flops[name] = fl_pairs
-    mixed = ("fp32", "fp32_grouped")
+    # the same operator through the dimension sweep (level-block plan: same terms, summed in
+    # another order; the FP64 baseline above stays the reference's pair-by-pair order)
+    sp = kmx.Plan(op["mats"], op["table"].idx1d, op["deg"], op["rp"], op["cols"], index_base=1, fp32=True,
+                  level_blocks=True)
+    sweep_runs = []
+    if sp.sweep("fp64")["available"]:
+        for prec in ("fp64", "fp32"):
+            name = prec + "_sweep"
+            sp.set_sweep(prec, True)
+            res[name] = time_apply(sp, xd, yd, prec, a.warmup, a.steps)
+            ys = sp.apply(xd, prec=prec).cpu().numpy()
+            err[name] = float(np.linalg.norm(ys - y64) / np.linalg.norm(y64))
+            maxabs[name] = float(np.max(np.abs(ys - y64)))
+            flops[name] = sp.sweep(prec)["flops"]
+            sweep_runs.append(name)
+    sp.destroy()
+    mixed = ("fp32", "fp32_grouped")  # the headline: pair-by-pair FP32 (the sweep is reported beside it)
     best = min(mixed, key=lambda n: res[n])
     ms = res[best]
     value = 1000.0 / ms
@@ -318,8 +338,7 @@ def main():
     traffic = None
     if best == "fp32":
         v = tuned["fp32"]
-        traffic = (MEASURED_TRAFFIC.get(("fp32", v["threads"], v["lds_staged"]))
-                   if v["threads"] > 0 and v["lds_mode"] in (0, 1) else None)  # counter runs: modes 0 and 1
+        traffic = MEASURED_TRAFFIC.get(("fp32", v["threads"], v["lds_mode"])) if v["threads"] > 0 else None
     kern = {"fp32": "k_apply_reg", "fp32_grouped": "k_apply_grouped"}[best]
     roofline = {
         "bound": "mfma",
@@ -349,8 +368,9 @@ def main():
                    "variant": best, "block": plan.B, "tuned": tuned},
         "runs": {n: {"applies_per_s": round(1000.0 / res[n], 2), "ms_per_apply": round(res[n], 5),
                      "rel_err_vs_fp64": err[n], "max_abs_err_vs_fp64": maxabs[n], "flops_per_apply": flops[n],
-                     "TFLOPs": round(flops[n] / (res[n] * 1e-3) / 1e12, 3)} for n in runs},
+                     "TFLOPs": round(flops[n] / (res[n] * 1e-3) / 1e12, 3)} for n in list(runs) + sweep_runs},
         "speedup_vs_fp64": round(res["fp64"] / ms, 3),
+        "sweep_speedup_vs_fp64": {n: round(res["fp64"] / res[n], 3) for n in sweep_runs},
         "roofline": roofline,
     }
     if not a.no_batch:
