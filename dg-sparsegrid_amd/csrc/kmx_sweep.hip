@@ -38,9 +38,11 @@
 #include "kmx_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <set>
 #include <stdexcept>
+#include <type_traits>
 #include <vector>
 
 namespace kmx {
@@ -103,27 +105,47 @@ struct UdOp {
     double scale[KMX_MAX_TERMS];
 };
 
-constexpr int UD_BATCH = 8;  // steps whose loads are issued before their matrix operations
+// Steps per load batch: one A value and NBR B values per step stay in registers (FP64
+// values take two).  Tiles are split over waves by the FP64 batch, so FP32 waves finish
+// theirs in fewer batches.
+template <typename T, int NBR>
+constexpr int ud_batch() {
+    return sizeof(T) == 4 || NBR == 1 ? 8 : NBR == 2 ? 6 : 4;
+}
+int ud_split_batch(int nbr) {
+    return nbr == 1 ? ud_batch<double, 1>() : nbr == 2 ? ud_batch<double, 2>() : ud_batch<double, 4>();
+}
 
-template <typename T, int PART, bool FINAL>
+// One wave: one output tile of NBR consecutive branches (same group, rows, columns and
+// source range; only their slots differ), so the A operand, the level cursor and the
+// offsets are shared.  Lane (li, lk) feeds row m0 + li of A and column f0 + li of B at
+// position 4 step + lk of the current level; the cursor is wave-uniform.
+template <typename T, int PART, bool FINAL, int NBR>
 __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ mat, T* __restrict__ arena,
                                                           const int2* __restrict__ recs, UdOp op,
                                                           const double* __restrict__ x, const int* __restrict__ xmap,
                                                           double* __restrict__ y, const int* __restrict__ ymap) {
     using V = typename SwMfma<T>::V;
-    __shared__ T part[UD_WAVES][4][64];
+    constexpr int NB = ud_batch<T, NBR>();
+    __shared__ T part[UD_WAVES - 1][NBR][4][64];  // partial tiles of waves 1.. of a split tile
     const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     const int2 rv = recs[((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane];
     const int gsize = __builtin_amdgcn_readlane(rv.x, R_GSIZE);
     const int gbase = __builtin_amdgcn_readlane(rv.x, R_GBASE);
     const bool leader = gsize > 0 && w == gbase;
-    V acc = V{T(0), T(0), T(0), T(0)};
+    V acc[NBR];
+    T addv[NBR][4];
+#pragma unroll
+    for (int j = 0; j < NBR; ++j) {
+        acc[j] = V{T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) addv[j][q] = T(0);
+    }
     int oaddr[4] = {0, 0, 0, 0};
     bool ook[4] = {false, false, false, false};
-    T addv[4] = {T(0), T(0), T(0), T(0)};
     int yi[4] = {-1, -1, -1, -1};
-    const int b = __builtin_amdgcn_readlane(rv.x, R_B);
+    const int b = __builtin_amdgcn_readlane(rv.x, R_B);  // first branch of the tile
     const int tout = __builtin_amdgcn_readlane(rv.x, R_TOUT);
     if (gsize > 0) {
         const int m0 = __builtin_amdgcn_readlane(rv.x, R_M0), f0 = __builtin_amdgcn_readlane(rv.x, R_F0);
@@ -150,12 +172,14 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ m
             }
             if (PART == UD_UP) {
                 const int ta = FINAL ? 0 : tout, tb = FINAL ? op.T : tout + 1;
-                for (int t = ta; t < tb; ++t) {
-                    const T* add = arena + op.add_base + ((long long)(b | op.add_bit) * op.T + t) * op.nb;
-                    const T sc = FINAL ? T(op.scale[t]) : T(1);
+                for (int t = ta; t < tb; ++t)
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) addv[q] += sc * add[oaddr[q]];
-                }
+                    for (int j = 0; j < NBR; ++j) {
+                        const T* add = arena + op.add_base + ((long long)((b + j) | op.add_bit) * op.T + t) * op.nb;
+                        const T sc = FINAL ? T(op.scale[t]) : T(1);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) addv[j][q] += sc * add[oaddr[q]];
+                    }
             }
             if (FINAL)
 #pragma unroll
@@ -166,12 +190,15 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ m
         const int sper = __builtin_amdgcn_readlane(rv.x, R_SPER), tlo = __builtin_amdgcn_readlane(rv.x, R_TLO);
         const int slo = __builtin_amdgcn_readlane(rv.x, R_SLO), tn = __builtin_amdgcn_readlane(rv.x, R_TN);
         const int shi = __builtin_amdgcn_readlane(rv.x, R_SHI);
-        const bool xin = op.x0 && b == 0;  // B is x, read through the box map
+        const bool xin0 = op.x0 && b == 0;  // branch b (j = 0) is x, read through the box map
         for (int k = 0; k < tn; ++k) {
             const int t = tlo + k;
             const int s0 = k == 0 ? slo : 0, s1 = k == tn - 1 ? shi : sper;
             const T* abase = mat + op.a_off[t];
-            const T* bbase = arena + (xin ? 0 : op.in_base + ((long long)b * op.T + t) * op.nb);
+            const T* bbase[NBR];
+#pragma unroll
+            for (int j = 0; j < NBR; ++j)
+                bbase[j] = arena + ((j == 0 && xin0) ? 0 : op.in_base + ((long long)(b + j) * op.T + t) * op.nb);
             const T sc = FINAL ? T(op.scale[t]) : T(1);
             int l = la, rem = s0;  // level and step of step s0 (wave-uniform)
             for (;;) {
@@ -183,12 +210,12 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ m
             int p = 4 * rem;
             int nl = 0, aidx = 0, bidx = 0;
             bool aon = false, newlev = true;
-            for (int q = s0; q < s1; q += UD_BATCH) {
+            for (int q = s0; q < s1; q += NB) {
                 // 1. per-lane offsets of the batch (0 where masked); the cursor moves on the scalar unit
-                int ai[UD_BATCH], bi[UD_BATCH];
-                bool ma[UD_BATCH], mb[UD_BATCH];
+                int ai[NB], bi[NB];
+                bool ma[NB], mb[NB];
 #pragma unroll
-                for (int c = 0; c < UD_BATCH; ++c) {
+                for (int c = 0; c < NB; ++c) {
                     if (newlev) {
                         const int st = l ? 1 << (l - 1) : 0;
                         nl = op.deg * (l ? st : 1);
@@ -216,50 +243,59 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ m
                     }
                 }
                 // 2. every load of the batch in flight at once
-                T av[UD_BATCH], bv[UD_BATCH];
-                if (xin) {
-                    int xm[UD_BATCH];
+                T av[NB], bv[NBR][NB];
 #pragma unroll
-                    for (int c = 0; c < UD_BATCH; ++c) {
-                        av[c] = abase[ai[c]];
-                        xm[c] = xmap[bi[c]];
-                    }
+                for (int c = 0; c < NB; ++c) av[c] = abase[ai[c]];
 #pragma unroll
-                    for (int c = 0; c < UD_BATCH; ++c) {
-                        mb[c] = mb[c] && xm[c] >= 0;  // absent cell: x is 0 there
-                        bv[c] = T(x[xm[c] >= 0 ? xm[c] : 0]);
-                    }
-                } else {
+                for (int j = 0; j < NBR; ++j) {
+                    if (j == 0 && xin0) {
+                        int xm[NB];
 #pragma unroll
-                    for (int c = 0; c < UD_BATCH; ++c) {
-                        av[c] = abase[ai[c]];
-                        bv[c] = bbase[bi[c]];
+                        for (int c = 0; c < NB; ++c) xm[c] = xmap[bi[c]];
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) {
+                            const double v = x[xm[c] >= 0 ? xm[c] : 0];
+                            bv[0][c] = xm[c] >= 0 ? T(v) : T(0);  // absent cell: x is 0 there
+                        }
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) bv[j][c] = bbase[j][bi[c]];
                     }
                 }
                 // 3. matrix operations (masked steps feed zeros)
 #pragma unroll
-                for (int c = 0; c < UD_BATCH; ++c)
-                    acc = SwMfma<T>::run(ma[c] ? av[c] * sc : T(0), mb[c] ? bv[c] : T(0), acc);
+                for (int c = 0; c < NB; ++c) {
+                    const T a = ma[c] ? av[c] * sc : T(0);
+#pragma unroll
+                    for (int j = 0; j < NBR; ++j) acc[j] = SwMfma<T>::run(a, mb[c] ? bv[j][c] : T(0), acc[j]);
+                }
             }
         }
         if (gsize > 1 && !leader)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) part[w][q][lane] = acc[q];
+            for (int j = 0; j < NBR; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) part[w - 1][j][q][lane] = acc[j][q];
     }
     __syncthreads();  // every wave of every workgroup, split tiles or not
     if (!leader) return;
     for (int v = 1; v < gsize; ++v)  // the tile's partial sums in wave order
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] += part[w + v][q][lane];
+        for (int j = 0; j < NBR; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[j][q] += part[w + v - 1][j][q][lane];
     if (FINAL) {
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (ook[q] && yi[q] >= 0) y[yi[q]] = double(acc[q] + addv[q]);
+            if (ook[q] && yi[q] >= 0) y[yi[q]] = double(acc[0][q] + addv[0][q]);
     } else {
-        T* out = arena + op.out_base + ((long long)(b | op.out_bit) * op.T + tout) * op.nb;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (ook[q]) out[oaddr[q]] = acc[q] + addv[q];
+        for (int j = 0; j < NBR; ++j) {
+            T* out = arena + op.out_base + ((long long)((b + j) | op.out_bit) * op.T + tout) * op.nb;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (ook[q]) out[oaddr[q]] = acc[j][q] + addv[j][q];
+        }
     }
 }
 
@@ -277,6 +313,7 @@ struct kmx_sweep {
         bool final;
         UdOp a;
         int wg0, wg1;  // workgroups (UD_WAVES work records each) of this op
+        int nbr;       // branches per tile
     };
     std::vector<Op> ops;
     int* d_xmap = nullptr;
@@ -415,6 +452,14 @@ void kmx_sweep_build(kmx_plan* p) {
         if (sw->arena > (1LL << 40)) throw std::runtime_error("sweep arena too large");
         // ---- ops and their work records
         std::vector<std::vector<int>> recs;  // one per wave, UD_REC ints
+        // branches per tile at most (1, 2 or 4; KMX_SWEEP_BRANCHES overrides): sharing a
+        // tile between more branches pays where there are many (d >= 4); on 3D lev 7
+        // deg 3 two and one time alike, four is 10 % slower (longer per-wave chains)
+        int max_nbr = D >= 4 ? 4 : 2;
+        if (const char* e = std::getenv("KMX_SWEEP_BRANCHES")) {
+            const int v = std::atoi(e);
+            if (v == 1 || v == 2 || v == 4) max_nbr = v;
+        }
         auto base_op = [&](int d) {
             UdOp a{};
             a.d = d;
@@ -433,11 +478,13 @@ void kmx_sweep_build(kmx_plan* p) {
         auto add_op = [&](int part, bool fin, UdOp a, long long nbranch) {
             a.part = part;
             a.final = fin ? 1 : 0;
-            kmx_sweep::Op op{part, fin, a, int(recs.size() / UD_WAVES), 0};
+            const int nbr = int(std::min<long long>(nbranch, max_nbr));  // branches per tile (a power of two)
+            kmx_sweep::Op op{part, fin, a, int(recs.size() / UD_WAVES), 0, nbr};
             const int d = a.d;
             const int nterm = fin ? T : 1;
+            const int batch = ud_split_batch(nbr);
             std::vector<HostTile> ot;
-            for (long long b = 0; b < nbranch; ++b)
+            for (long long b = 0; b < nbranch; b += nbr)
                 for (int t = 0; t < (fin ? 1 : T); ++t)
                     for (int gi = group0[d]; gi < group0[d + 1]; ++gi) {
                         const Group& g = groups[gi];
@@ -450,7 +497,7 @@ void kmx_sweep_build(kmx_plan* p) {
                             const int sper = steps_of(la, lb, deg);
                             const int steps = nterm * sper;
                             int gsize = 1;  // waves sharing the tile: at most one batch of steps each
-                            while (gsize < UD_WAVES && (steps + gsize - 1) / gsize > UD_BATCH) gsize *= 2;
+                            while (gsize < UD_WAVES && (steps + gsize - 1) / gsize > batch) gsize *= 2;
                             for (int f0 = 0; f0 < g.O * g.I; f0 += 16)
                                 ot.push_back(HostTile{m0, f0, int(b), t, gi, la, lb, nterm, sper, steps, gsize});
                         }
@@ -583,7 +630,9 @@ void kmx_sweep_build(kmx_plan* p) {
         // its rows and boxes as the group defines them (checked here, on the host)
         auto slot_ok = [&](long long off) { return off >= 0 && off + nb <= sw->arena; };
         for (const auto& op : sw->ops) {
-            const long long nbr = op.final ? 1 : (op.part == UD_FULL ? half : (1LL << op.a.d));
+            const long long nbranch = op.final ? 1 : (op.part == UD_FULL ? half : (1LL << op.a.d));
+            if (op.nbr < 1 || op.nbr > 4 || nbranch % op.nbr || (op.final && op.nbr != 1))
+                throw std::logic_error("sweep branches per tile");
             for (int wg = op.wg0; wg < op.wg1; ++wg)
                 for (int w = 0; w < UD_WAVES; ++w) {
                     const std::vector<int>& r = recs[size_t(wg) * UD_WAVES + w];
@@ -608,7 +657,8 @@ void kmx_sweep_build(kmx_plan* p) {
                     const int n = deg << g.m;
                     const int sper = steps_of(r[R_LA], r[R_LB], deg);
                     if (r[R_M0] < 0 || r[R_M0] >= n || r[R_M0] % 16 || r[R_F0] < 0 || r[R_F0] >= g.O * g.I ||
-                        r[R_LA] < 0 || r[R_LB] > g.m || r[R_SPER] != sper || r[R_B] < 0 || r[R_B] >= nbr ||
+                        r[R_LA] < 0 || r[R_LB] > g.m || r[R_SPER] != sper || r[R_B] < 0 || r[R_B] % op.nbr ||
+                        r[R_B] + op.nbr > nbranch ||
                         r[R_TOUT] < 0 || r[R_TOUT] >= T || r[R_MAGIC] != int(g.magic))
                         throw std::logic_error("sweep record outside its group");
                     if (r[R_TN] < 0 || (r[R_TN] > 0 && (r[R_TLO] < 0 || r[R_TLO] + r[R_TN] > T || r[R_SLO] < 0 ||
@@ -625,16 +675,18 @@ void kmx_sweep_build(kmx_plan* p) {
                     }
                     for (int l = 0; l <= g.m; ++l)
                         if (r[UD_BOX + l] != g.box[l]) throw std::logic_error("sweep record boxes");
-                    for (int t = r[R_TLO]; t < r[R_TLO] + r[R_TN]; ++t)
-                        if (!(op.a.x0 && r[R_B] == 0) && !slot_ok(op.a.in_base + ((long long)r[R_B] * T + t) * nb))
-                            throw std::logic_error("sweep input slot outside the arena");
-                    for (int t = 0; t < T; ++t) {
-                        if (!op.final && t != r[R_TOUT]) continue;
-                        if (op.part == UD_UP &&
-                            !slot_ok(op.a.add_base + ((long long)(r[R_B] | op.a.add_bit) * T + t) * nb))
-                            throw std::logic_error("sweep add slot outside the arena");
-                        if (!op.final && !slot_ok(op.a.out_base + ((long long)(r[R_B] | op.a.out_bit) * T + t) * nb))
-                            throw std::logic_error("sweep output slot outside the arena");
+                    for (int bj = r[R_B]; bj < r[R_B] + op.nbr; ++bj) {  // every branch of the tile
+                        for (int t = r[R_TLO]; t < r[R_TLO] + r[R_TN]; ++t)
+                            if (!(op.a.x0 && bj == 0) && !slot_ok(op.a.in_base + ((long long)bj * T + t) * nb))
+                                throw std::logic_error("sweep input slot outside the arena");
+                        for (int t = 0; t < T; ++t) {
+                            if (!op.final && t != r[R_TOUT]) continue;
+                            if (op.part == UD_UP &&
+                                !slot_ok(op.a.add_base + ((long long)(bj | op.a.add_bit) * T + t) * nb))
+                                throw std::logic_error("sweep add slot outside the arena");
+                            if (!op.final && !slot_ok(op.a.out_base + ((long long)(bj | op.a.out_bit) * T + t) * nb))
+                                throw std::logic_error("sweep output slot outside the arena");
+                        }
                     }
                 }
         }
@@ -674,9 +726,15 @@ int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream
                 for (int t = 0; t < p->T; ++t) a.scale[t] = p->scale[t];
             const dim3 grid(unsigned(op.wg1 - op.wg0)), block(UD_WAVES * 64);
             const int2* rc = reinterpret_cast<const int2*>(sw->d_recs) + size_t(op.wg0) * UD_WAVES * (UD_REC / 2);
-            auto* kern = op.part == UD_DOWN ? k_ud_op<T, UD_DOWN, false>
-                         : op.part == UD_FULL ? (op.final ? k_ud_op<T, UD_FULL, true> : k_ud_op<T, UD_FULL, false>)
-                                              : (op.final ? k_ud_op<T, UD_UP, true> : k_ud_op<T, UD_UP, false>);
+            auto pick = [&](auto nbr_c) {
+                constexpr int NBR = decltype(nbr_c)::value;
+                return op.part == UD_DOWN ? k_ud_op<T, UD_DOWN, false, NBR>
+                       : op.part == UD_FULL ? (op.final ? k_ud_op<T, UD_FULL, true, 1> : k_ud_op<T, UD_FULL, false, NBR>)
+                                            : (op.final ? k_ud_op<T, UD_UP, true, 1> : k_ud_op<T, UD_UP, false, NBR>);
+            };
+            auto* kern = op.nbr == 1   ? pick(std::integral_constant<int, 1>{})
+                         : op.nbr == 2 ? pick(std::integral_constant<int, 2>{})
+                                       : pick(std::integral_constant<int, 4>{});
             hipLaunchKernelGGL(kern, grid, block, 0, s, mat, arena, rc, a, x, sw->d_xmap, y, sw->d_ymap);
         }
     };
