@@ -443,13 +443,13 @@ struct kmx_batch {
     bool has32 = true;
 };
 
-// The switch's register budget is that of its largest case, so only blocks of
-// up to 27 values share the persistent launch; larger blocks (which want the
-// lane-split kernels) run as their own autotuned launches.
+// The switch's register budget is that of its largest case: blocks of up to 27
+// values (one lane per pair) share the persistent launch, and in FP64 so do 3D
+// degree-4 blocks (four lanes per pair, a row body that needs fewer registers
+// than the 27-value one); other shapes run as their own autotuned launches.
 static bool batch_shape_ok(const kmx_plan* p, bool fp64) {
-    (void)fp64;
     if (p->D > 3 || p->deg > 4) return false;
-    return p->B <= 27;
+    return p->B <= 27 || (fp64 && p->D == 3 && p->deg == 4);
 }
 
 template <typename T>

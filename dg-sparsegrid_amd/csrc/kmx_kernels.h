@@ -497,12 +497,18 @@ __device__ __forceinline__ void batch_row(const KArgs<T>& a, const double* x, do
     if constexpr (Pow<DEG, D>::v <= 27) {
         constexpr bool KX = 3 * Pow<DEG, D>::v * int(sizeof(T)) / 4 <= 144;
         apply_row<T, D, DEG, 256, false, KX>(a, x, y, row, sm);
+    } else if constexpr (sizeof(T) == 8 && (DEG == 2 || DEG == 4)) {
+        // larger FP64 blocks: DEG lanes share a pair (its registers fit the FP64 switch's
+        // budget; in FP32 the case would cost the switch occupancy, so those run alone)
+        apply_row_split<T, D, DEG, 256, 0>(a, x, y, row, sm);
     }
 }
 
+// (FP64: at least two waves per SIMD, the register budget that keeps its largest cases unspilled)
 template <typename T>
-__global__ void __launch_bounds__(256) k_apply_batch(const KArgs<T>* __restrict__ args, const int2* __restrict__ items,
-                                                     int nitems, const int* __restrict__ shape, BatchIO io) {
+__global__ void __launch_bounds__(256, sizeof(T) == 8 ? 2 : 1)
+    k_apply_batch(const KArgs<T>* __restrict__ args, const int2* __restrict__ items, int nitems,
+                  const int* __restrict__ shape, BatchIO io) {
     extern __shared__ __align__(16) unsigned char smem[];
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int2 w = items[it];
@@ -521,6 +527,7 @@ __global__ void __launch_bounds__(256) k_apply_batch(const KArgs<T>* __restrict_
             case 0x31: batch_row<T, 3, 1>(a, x, y, w.y, smem); break;
             case 0x32: batch_row<T, 3, 2>(a, x, y, w.y, smem); break;
             case 0x33: batch_row<T, 3, 3>(a, x, y, w.y, smem); break;
+            case 0x34: batch_row<T, 3, 4>(a, x, y, w.y, smem); break;
             default: break;
         }
         __syncthreads();
