@@ -117,8 +117,13 @@ def cpu_baseline(op, budget_s=12.0):
 
 
 def batched_leg(steps=20, warmup=3):
-    """64 adapted 2D/3D advection-diffusion instances, one persistent batched
-    apply per RK3 stage, whole step replayed from a HIP graph; FP32 vs FP64."""
+    """64 adapted 2D/3D advection-diffusion instances (all element pairs), one
+    batched apply per RK3 stage, whole step replayed from a HIP graph.  Runs:
+    "fp64" / "fp32" = every instance pair by pair in the persistent launch (the
+    FP64 one is the plain baseline); "fp32_sweep" / "fp64_sweep" = every instance
+    on the merged dimension sweeps (level-block plans, one launch per phase for
+    all instances; same terms, other summation order).  Errors: final state of
+    each run against the FP64 pair-path state after the same steps."""
     import torch
 
     from kmx import batch as BT
@@ -128,8 +133,9 @@ def batched_leg(steps=20, warmup=3):
     build_s = time.perf_counter() - t0
     out = {"instances": len(inst), "build_s": round(build_s, 1)}
     states = {}
-    for prec in ("fp64", "fp32"):
-        sysm = BT.BatchedSystem(inst)
+    for name, prec, sweep in (("fp64", "fp64", None), ("fp32", "fp32", None),
+                              ("fp32_sweep", "fp32", ("fp32",)), ("fp64_sweep", "fp64", ("fp64",))):
+        sysm = BT.BatchedSystem(inst, sweep=sweep)
         g = BT.BatchRK3Graph(sysm, prec=prec)
         for _ in range(warmup):
             g.step()
@@ -142,15 +148,20 @@ def batched_leg(steps=20, warmup=3):
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / steps
-        states[prec] = sysm.f.cpu().numpy()
-        out[prec] = {"ms_per_rk3_step": round(ms, 4),
+        states[name] = sysm.f.cpu().numpy()
+        out[name] = {"ms_per_rk3_step": round(ms, 4),
                      "instance_applies_per_s": round(3 * len(inst) / (ms * 1e-3), 1),
-                     "TFLOPs": round(3 * sysm.flops / (ms * 1e-3) / 1e12, 3)}
+                     "flops_per_apply": sysm.flops_by_prec[prec],
+                     "TFLOPs": round(3 * sysm.flops_by_prec[prec] / (ms * 1e-3) / 1e12, 3)}
+        if sweep:
+            out[name]["merged_sweep"] = sysm.sweep[prec]
         out["pairs"] = int(sysm.pairs)
         out["dof"] = int(sysm.dof)
-    d = states["fp32"] - states["fp64"]
-    out["rk3_rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(states["fp64"]))
-    out["speedup_fp32_vs_fp64"] = round(out["fp64"]["ms_per_rk3_step"] / out["fp32"]["ms_per_rk3_step"], 3)
+        del g, sysm
+    for name in ("fp32", "fp32_sweep", "fp64_sweep"):
+        d = states[name] - states["fp64"]
+        out[name]["rk3_rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(states["fp64"]))
+        out[name]["speedup_vs_fp64"] = round(out["fp64"]["ms_per_rk3_step"] / out[name]["ms_per_rk3_step"], 3)
     out["steps"] = steps + warmup
     return out
 

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <stdexcept>
 #include <tuple>
 
 using namespace kmx;
@@ -441,6 +442,10 @@ struct kmx_batch {
     std::vector<kmx_plan*> plans;
     BatchSet set[2];  // [0] FP64, [1] FP32
     bool has32 = true;
+    // merged dimension sweeps (kmx_batch_set_sweep): members[q] = plan index of io slot q
+    kmx_sweep_batch* sweep = nullptr;
+    std::vector<int> members;
+    bool sweep_on[2] = {false, false};
 };
 
 // The switch's register budget is that of its largest case: blocks of up to 27
@@ -462,9 +467,13 @@ static hipError_t build_set(kmx_batch* b, BatchSet& bs, bool fp64) {
     };
     std::vector<Item> items;
     std::vector<long long> rp;
+    std::vector<char> skip(b->plans.size(), 0);  // plans served by the merged sweeps
+    if (b->sweep_on[fp64 ? 0 : 1])
+        for (int i : b->members) skip[size_t(i)] = 1;
     for (int i = 0; i < int(b->plans.size()); ++i) {
         kmx_plan* p = b->plans[i];
         if (!fp64 && !p->d_blk32) continue;
+        if (skip[size_t(i)]) continue;
         if (!batch_shape_ok(p, fp64) || int(bs.in_kernel.size()) >= KMX_BATCH_MAX) {
             bs.solo.push_back(i);
             continue;
@@ -566,7 +575,18 @@ extern "C" int kmx_batch_apply(kmx_batch* b, const double* const* xs, double* co
             set_error("kmx_batch_apply: null vector");
             return KMX_E_ARG;
         }
-    BatchSet& bs = b->set[prec == KMX_PREC_FP64 ? 0 : 1];
+    const int slot = prec == KMX_PREC_FP64 ? 0 : 1;
+    if (b->sweep_on[slot]) {
+        BatchIO io;
+        std::memset(&io, 0, sizeof(io));
+        for (size_t q = 0; q < b->members.size(); ++q) {
+            io.x[q] = xs[b->members[q]];
+            io.y[q] = ys[b->members[q]];
+        }
+        const int rc = kmx_sweep_batch_apply(b->sweep, slot, io, s);
+        if (rc != KMX_OK) return rc;
+    }
+    BatchSet& bs = b->set[slot];
     if (bs.nitems > 0) {
         BatchIO io;
         std::memset(&io, 0, sizeof(io));
@@ -595,6 +615,63 @@ extern "C" int kmx_batch_destroy(kmx_batch* b) {
     if (!b) return KMX_OK;
     free_set(b->set[0]);
     free_set(b->set[1]);
+    kmx_sweep_batch_free(b->sweep);
     delete b;
+    return KMX_OK;
+}
+
+extern "C" int kmx_batch_set_sweep(kmx_batch* b, int prec, int on) {
+    if (!b || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
+        set_error("kmx_batch_set_sweep: bad arguments");
+        return KMX_E_ARG;
+    }
+    const int slot = prec_slot(prec);
+    if (bool(on) == b->sweep_on[slot]) return KMX_OK;
+    if (on && !b->sweep) {
+        std::vector<kmx_plan*> mp;
+        std::vector<int> mi;
+        for (int i = 0; i < int(b->plans.size()) && int(mi.size()) < KMX_BATCH_MAX; ++i) {
+            kmx_plan* p = b->plans[i];
+            if (!p->sw || std::find(mp.begin(), mp.end(), p) != mp.end()) continue;  // one arena per plan
+            mp.push_back(p);
+            mi.push_back(i);
+        }
+        if (mp.empty()) {
+            set_error("kmx_batch_set_sweep: no plan of the batch has a dimension-sweep path");
+            return KMX_E_UNSUPPORTED;
+        }
+        try {
+            b->sweep = kmx_sweep_batch_build(mp);
+        } catch (const std::exception& ex) {
+            set_error(std::string("kmx_batch_set_sweep: ") + ex.what());
+            return KMX_E_HIP;
+        }
+        b->members = mi;
+    }
+    if (on && slot == 1)
+        for (int i : b->members)
+            if (!b->plans[size_t(i)]->d_blk32) {
+                set_error("kmx_batch_set_sweep: a plan has no FP32 blocks");
+                return KMX_E_ARG;
+            }
+    b->sweep_on[slot] = on != 0;
+    free_set(b->set[slot]);
+    const hipError_t e = slot == 0 ? build_set<double>(b, b->set[0], true) : build_set<float>(b, b->set[1], false);
+    if (e != hipSuccess) {
+        set_error(std::string("kmx_batch_set_sweep: ") + hipGetErrorString(e));
+        return KMX_E_HIP;
+    }
+    return KMX_OK;
+}
+
+extern "C" int kmx_batch_get_sweep(const kmx_batch* b, int prec, int* members, int* active, int* phases) {
+    if (!b || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
+        set_error("kmx_batch_get_sweep: bad arguments");
+        return KMX_E_ARG;
+    }
+    const bool act = b->sweep_on[prec_slot(prec)];
+    if (members) *members = act ? int(b->members.size()) : 0;
+    if (active) *active = act ? 1 : 0;
+    if (phases) *phases = act ? kmx_sweep_batch_phases(b->sweep) : 0;
     return KMX_OK;
 }

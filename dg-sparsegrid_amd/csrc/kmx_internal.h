@@ -43,6 +43,13 @@ struct KArgs {
     const long long* grp_row_ptr;  // N+1 offsets into grp_ptr
 };
 
+// x / y device pointers of the instances of a batch, by slot (a kernel argument).
+#define KMX_BATCH_MAX 128
+struct BatchIO {
+    const double* x[KMX_BATCH_MAX];
+    double* y[KMX_BATCH_MAX];
+};
+
 }  // namespace kmx
 
 // Launch variant of the pair-by-pair kernels: workgroup size (0 = default
@@ -186,3 +193,13 @@ void kmx_sweep_build(kmx_plan* p);
 int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s);
 void kmx_sweep_free(kmx_plan* p);
 double kmx_sweep_flops(const kmx_plan* p);
+// The sweeps of several plans merged into one launch per phase (kmx_sweep.hip):
+// phase k runs op k - (phases - ops of the plan) of every plan, so each plan's ops
+// keep their order and every plan's last op falls in the last phase.  members:
+// distinct plans, each with a sweep path; io slot q is members[q].  Term scales
+// are taken at build time.  Build throws std::exception on failure.
+struct kmx_sweep_batch;
+kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members);
+int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const kmx::BatchIO& io, hipStream_t s);
+void kmx_sweep_batch_free(kmx_sweep_batch* sb);
+int kmx_sweep_batch_phases(const kmx_sweep_batch* sb);

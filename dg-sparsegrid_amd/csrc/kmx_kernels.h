@@ -485,12 +485,7 @@ __global__ void __launch_bounds__(NT) k_apply_split(KArgs<T> a) {
 // One persistent launch over a work list of (instance, row) items, heaviest
 // first, striding by the grid; each instance keeps its own plan (levels, deg,
 // pair list, coefficient blocks).  The shape switch selects the fully unrolled
-// row body for the instance's (d, deg).
-#define KMX_BATCH_MAX 128
-struct BatchIO {
-    const double* x[KMX_BATCH_MAX];
-    double* y[KMX_BATCH_MAX];
-};
+// row body for the instance's (d, deg).  (BatchIO: kmx_internal.h)
 
 template <typename T, int D, int DEG>
 __device__ __forceinline__ void batch_row(const KArgs<T>& a, const double* x, double* y, int row, unsigned char* sm) {

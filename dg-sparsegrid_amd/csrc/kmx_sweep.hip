@@ -119,18 +119,17 @@ int ud_split_batch(int nbr) {
 // One wave: one output tile of NBR consecutive branches (same group, rows, columns and
 // source range; only their slots differ), so the A operand, the level cursor and the
 // offsets are shared.  Lane (li, lk) feeds row m0 + li of A and column f0 + li of B at
-// position 4 step + lk of the current level; the cursor is wave-uniform.
+// position 4 step + lk of the current level; the cursor is wave-uniform.  rv: this
+// lane's two ints of the wave's work record; part: LDS for the partial tiles of waves
+// 1.. of a split tile, (UD_WAVES - 1) x NBR x 4 x 64 values.  Every wave of the
+// workgroup calls it (it holds one workgroup barrier).
 template <typename T, int PART, bool FINAL, int NBR>
-__global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ mat, T* __restrict__ arena,
-                                                          const int2* __restrict__ recs, UdOp op,
-                                                          const double* __restrict__ x, const int* __restrict__ xmap,
-                                                          double* __restrict__ y, const int* __restrict__ ymap) {
+__device__ __forceinline__ void ud_tile(const T* __restrict__ mat, T* __restrict__ arena, const int2 rv, const int w,
+                                        const int lane, const UdOp& op, const double* __restrict__ x,
+                                        const int* __restrict__ xmap, double* __restrict__ y,
+                                        const int* __restrict__ ymap, T* __restrict__ part) {
     using V = typename SwMfma<T>::V;
     constexpr int NB = ud_batch<T, NBR>();
-    __shared__ T part[UD_WAVES - 1][NBR][4][64];  // partial tiles of waves 1.. of a split tile
-    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    const int lane = threadIdx.x & 63;
-    const int2 rv = recs[((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane];
     const int gsize = __builtin_amdgcn_readlane(rv.x, R_GSIZE);
     const int gbase = __builtin_amdgcn_readlane(rv.x, R_GBASE);
     const bool leader = gsize > 0 && w == gbase;
@@ -275,7 +274,7 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ m
 #pragma unroll
             for (int j = 0; j < NBR; ++j)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) part[w - 1][j][q][lane] = acc[j][q];
+                for (int q = 0; q < 4; ++q) part[(((w - 1) * NBR + j) * 4 + q) * 64 + lane] = acc[j][q];
     }
     __syncthreads();  // every wave of every workgroup, split tiles or not
     if (!leader) return;
@@ -283,7 +282,7 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ m
 #pragma unroll
         for (int j = 0; j < NBR; ++j)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[j][q] += part[w + v - 1][j][q][lane];
+            for (int q = 0; q < 4; ++q) acc[j][q] += part[(((w + v - 1) * NBR + j) * 4 + q) * 64 + lane];
     if (FINAL) {
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -296,6 +295,87 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ m
             for (int q = 0; q < 4; ++q)
                 if (ook[q]) out[oaddr[q]] = acc[j][q] + addv[j][q];
         }
+    }
+}
+
+// One op of one plan: UD_WAVES work records per workgroup.
+template <typename T, int PART, bool FINAL, int NBR>
+__global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ mat, T* __restrict__ arena,
+                                                          const int2* __restrict__ recs, UdOp op,
+                                                          const double* __restrict__ x, const int* __restrict__ xmap,
+                                                          double* __restrict__ y, const int* __restrict__ ymap) {
+    __shared__ T part[(UD_WAVES - 1) * NBR * 4 * 64];
+    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int2 rv = recs[((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane];
+    ud_tile<T, PART, FINAL, NBR>(mat, arena, rv, w, lane, op, x, xmap, y, ymap, part);
+}
+
+// Op kinds of the batched launch: (part, final, branches per tile).
+enum {
+    UK_DOWN1, UK_DOWN2, UK_DOWN4, UK_FULL1, UK_FULL2, UK_FULL4, UK_UP1, UK_UP2, UK_UP4, UK_FULL_FINAL, UK_UP_FINAL
+};
+int ud_kind(int part, bool fin, int nbr) {
+    const int nb = nbr == 1 ? 0 : nbr == 2 ? 1 : 2;
+    return part == UD_DOWN ? UK_DOWN1 + nb
+           : part == UD_FULL ? (fin ? int(UK_FULL_FINAL) : UK_FULL1 + nb)
+                             : (fin ? int(UK_UP_FINAL) : UK_UP1 + nb);
+}
+int ud_kind_nbr(int kind) {
+    return kind == UK_FULL_FINAL || kind == UK_UP_FINAL ? 1 : 1 << ((kind - UK_DOWN1) % 3);
+}
+
+// One op of one instance of a batch: its parameters (term scales as at batch
+// set-up), its plan's matrices, arena and box maps per precision, and its slot
+// in the batch's x / y pointer arrays.
+struct UdBatchOp {
+    UdOp a;
+    const void* mat[2];  // FP64, FP32 blocks of the plan
+    void* arena[2];
+    const int* xmap;
+    const int* ymap;
+    int inst;
+    int kind;
+};
+
+// One phase of a batch: ops of many instances, each on whole workgroups
+// (wg_op[workgroup] = its op); the workgroup's op kind picks the tile body.
+// MAXNBR bounds the branches per tile of the phase (LDS for the partial tiles).
+template <typename T, int MAXNBR>
+__global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __restrict__ ops,
+                                                             const int* __restrict__ wg_op,
+                                                             const int2* __restrict__ recs, BatchIO io) {
+    __shared__ T part[(UD_WAVES - 1) * MAXNBR * 4 * 64];
+    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int2 rv = recs[((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane];
+    const UdBatchOp& o = ops[wg_op[blockIdx.x]];
+    constexpr int sl = sizeof(T) == 8 ? 0 : 1;
+    const T* mat = static_cast<const T*>(o.mat[sl]);
+    T* arena = static_cast<T*>(o.arena[sl]);
+    const double* x = io.x[o.inst];
+    double* y = io.y[o.inst];
+    switch (o.kind) {
+        case UK_DOWN1: ud_tile<T, UD_DOWN, false, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
+        case UK_DOWN2: ud_tile<T, UD_DOWN, false, 2>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
+        case UK_FULL1: ud_tile<T, UD_FULL, false, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
+        case UK_FULL2: ud_tile<T, UD_FULL, false, 2>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
+        case UK_UP1: ud_tile<T, UD_UP, false, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
+        case UK_UP2: ud_tile<T, UD_UP, false, 2>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
+        case UK_FULL_FINAL:
+            ud_tile<T, UD_FULL, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part);
+            break;
+        case UK_UP_FINAL: ud_tile<T, UD_UP, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
+        default:
+            if constexpr (MAXNBR == 4) {
+                if (o.kind == UK_DOWN4)
+                    ud_tile<T, UD_DOWN, false, 4>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part);
+                else if (o.kind == UK_FULL4)
+                    ud_tile<T, UD_FULL, false, 4>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part);
+                else if (o.kind == UK_UP4)
+                    ud_tile<T, UD_UP, false, 4>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part);
+            }
+            break;
     }
 }
 
@@ -769,3 +849,122 @@ void kmx_sweep_free(kmx_plan* p) {
 
 double kmx_sweep_flops(const kmx_plan* p) { return p->sw ? double(p->sw->flops) : 0.0; }
 
+
+// ------------------------------------------------------------------ batches
+struct kmx_sweep_batch {
+    struct Phase {
+        int wg0, nwg, maxnbr;  // first workgroup, workgroups, most branches per tile
+    };
+    std::vector<Phase> phases;
+    UdBatchOp* d_ops = nullptr;
+    int* d_wgop = nullptr;  // op of every workgroup, phases in order
+    int* d_recs = nullptr;  // UD_WAVES work records per workgroup, copied from the plans
+    bool has32 = true;
+};
+
+kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members) {
+    if (members.empty() || members.size() > KMX_BATCH_MAX) throw std::invalid_argument("sweep batch size");
+    int nph = 0;
+    for (size_t q = 0; q < members.size(); ++q) {
+        const kmx_plan* p = members[q];
+        if (!p || !p->sw || p->sw->ops.empty()) throw std::invalid_argument("sweep batch member without a sweep");
+        for (size_t r = 0; r < q; ++r)
+            if (members[r] == p) throw std::invalid_argument("sweep batch member twice (one arena per plan)");
+        nph = std::max(nph, int(p->sw->ops.size()));
+    }
+    auto* sb = new kmx_sweep_batch();
+    try {
+        std::vector<UdBatchOp> ops;
+        std::vector<int> wgop;
+        std::vector<std::pair<const int*, long long>> copies;  // (source records, workgroups)
+        for (int ph = 0; ph < nph; ++ph) {
+            kmx_sweep_batch::Phase P{int(wgop.size()), 0, 1};
+            for (size_t q = 0; q < members.size(); ++q) {
+                kmx_plan* p = members[q];
+                const kmx_sweep* sw = p->sw;
+                const int k = ph - (nph - int(sw->ops.size()));
+                if (k < 0) continue;
+                const kmx_sweep::Op& op = sw->ops[size_t(k)];
+                UdBatchOp o{};
+                o.a = op.a;
+                if (op.final)
+                    for (int t = 0; t < p->T; ++t) o.a.scale[t] = p->scale[t];
+                o.mat[0] = p->bs->d_mat64;
+                o.mat[1] = p->bs->d_mat32;
+                o.arena[0] = sw->d_arena[0];
+                o.arena[1] = sw->d_arena[1];
+                if (!sw->d_arena[1] || !p->bs->d_mat32) sb->has32 = false;
+                o.xmap = sw->d_xmap;
+                o.ymap = sw->d_ymap;
+                o.inst = int(q);
+                o.kind = ud_kind(op.part, op.final, op.nbr);
+                if (ud_kind_nbr(o.kind) != op.nbr || op.wg1 < op.wg0) throw std::logic_error("sweep batch op");
+                P.maxnbr = std::max(P.maxnbr, op.nbr);
+                const int oi = int(ops.size());
+                ops.push_back(o);
+                for (int wg = op.wg0; wg < op.wg1; ++wg) wgop.push_back(oi);
+                copies.emplace_back(sw->d_recs + size_t(op.wg0) * UD_WAVES * UD_REC, op.wg1 - op.wg0);
+                P.nwg += op.wg1 - op.wg0;
+                if (wgop.size() >= size_t(1) << 30) throw std::runtime_error("sweep batch too large");
+            }
+            sb->phases.push_back(P);
+        }
+        auto alloc = [](auto*& d, size_t n) {
+            if (hipMalloc(&d, std::max<size_t>(1, n) * sizeof(*d)) != hipSuccess) throw std::bad_alloc();
+        };
+        alloc(sb->d_ops, ops.size());
+        alloc(sb->d_wgop, wgop.size());
+        alloc(sb->d_recs, wgop.size() * UD_WAVES * UD_REC);
+        if (hipMemcpy(sb->d_ops, ops.data(), ops.size() * sizeof(UdBatchOp), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(sb->d_wgop, wgop.data(), wgop.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("hipMemcpy failed");
+        size_t at = 0;  // records land in the order their workgroups were listed
+        for (const auto& c : copies) {
+            const size_t n = size_t(c.second) * UD_WAVES * UD_REC;
+            if (n && hipMemcpy(sb->d_recs + at, c.first, n * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess)
+                throw std::runtime_error("hipMemcpy failed");
+            at += n;
+        }
+        if (at != wgop.size() * UD_WAVES * UD_REC) throw std::logic_error("sweep batch records");
+    } catch (...) {
+        kmx_sweep_batch_free(sb);
+        throw;
+    }
+    return sb;
+}
+
+int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io, hipStream_t s) {
+    if (slot == 1 && !sb->has32) {
+        set_error("kmx_batch_apply: a sweep member has no FP32 blocks");
+        return KMX_E_ARG;
+    }
+    for (const auto& P : sb->phases) {
+        if (P.nwg == 0) continue;
+        const int* wgop = sb->d_wgop + P.wg0;
+        const int2* rc = reinterpret_cast<const int2*>(sb->d_recs) + size_t(P.wg0) * UD_WAVES * (UD_REC / 2);
+        const dim3 grid(unsigned(P.nwg)), block(UD_WAVES * 64);
+        if (slot == 0) {
+            if (P.maxnbr <= 2) hipLaunchKernelGGL((k_ud_batch<double, 2>), grid, block, 0, s, sb->d_ops, wgop, rc, io);
+            else hipLaunchKernelGGL((k_ud_batch<double, 4>), grid, block, 0, s, sb->d_ops, wgop, rc, io);
+        } else {
+            if (P.maxnbr <= 2) hipLaunchKernelGGL((k_ud_batch<float, 2>), grid, block, 0, s, sb->d_ops, wgop, rc, io);
+            else hipLaunchKernelGGL((k_ud_batch<float, 4>), grid, block, 0, s, sb->d_ops, wgop, rc, io);
+        }
+    }
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        set_error(std::string("sweep batch launch: ") + hipGetErrorString(err));
+        return KMX_E_HIP;
+    }
+    return KMX_OK;
+}
+
+void kmx_sweep_batch_free(kmx_sweep_batch* sb) {
+    if (!sb) return;
+    if (sb->d_ops) hipFree(sb->d_ops);
+    if (sb->d_wgop) hipFree(sb->d_wgop);
+    if (sb->d_recs) hipFree(sb->d_recs);
+    delete sb;
+}
+
+int kmx_sweep_batch_phases(const kmx_sweep_batch* sb) { return sb ? int(sb->phases.size()) : 0; }

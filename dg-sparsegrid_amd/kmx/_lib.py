@@ -108,6 +108,9 @@ PROTOTYPES = {
     "kmx_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
     "kmx_batch_apply": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, C.c_void_p]),
     "kmx_batch_destroy": (C.c_int, [C.c_void_p]),
+    "kmx_batch_set_sweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "kmx_batch_get_sweep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
     "kmx_kronmult": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.c_void_p,
                                C.c_int, C.c_void_p, C.c_void_p]),
     "kmx_gemm_vbatched": (C.c_int, [C.c_char, C.c_char, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),

@@ -53,24 +53,35 @@ def make_instances(count=64, seed=0, keep_prob=0.6, lev2d=(4, 8), lev3d=(4, 6), 
 
 
 class BatchedSystem:
-    """Plans + concatenated device state for a list of instances."""
+    """Plans + concatenated device state for a list of instances.
 
-    def __init__(self, instances, device="cuda"):
+    sweep: None (default) applies every instance pair by pair (the persistent
+    launch); a tuple of precisions, e.g. ("fp32",), builds level-block plans and
+    runs every instance that has a dimension-sweep path as one merged sweep
+    (one launch per phase for all of them) in those precisions."""
+
+    def __init__(self, instances, device="cuda", sweep=None):
         import torch
 
         self.instances = instances
         self.plans = []
+        sweep = tuple(sweep or ())
         for inst in instances:
-            p = Plan(inst["mats"], inst["table"].idx1d, inst["deg"], None, None, index_base=1, fp32=True)
+            p = Plan(inst["mats"], inst["table"].idx1d, inst["deg"], None, None, index_base=1, fp32=True,
+                     level_blocks=bool(sweep))
             for t, s in enumerate(inst["scales"]):
                 p.set_term_scale(t, float(s))
             self.plans.append(p)
         # blocks above 27 values run outside the persistent launch, each with
-        # the kernel variant timed fastest on its own plan
+        # the kernel variant timed fastest on its own plan (the autotuner of a
+        # level-block plan may also pick its own dimension sweep)
         for p in self.plans:
-            if p.B > 27:
+            if p.B > 27 and not (sweep and p.sweep()["available"]):
                 p.autotune()
         self.batch = Batch(self.plans)
+        for prec in sweep:
+            self.batch.set_sweep(prec, True)
+        self.sweep = {prec: self.batch.sweep(prec) for prec in ("fp64", "fp32")}
         sizes = [p.dof for p in self.plans]
         self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
         self.dof = int(self.offsets[-1])
@@ -79,6 +90,13 @@ class BatchedSystem:
                                                    zip(self.plans, instances)])).to(device)
         self.pairs = sum(p.P for p in self.plans)
         self.flops = sum(p.P * p.T * 2 * p.ndim * p.deg ** (p.ndim + 1) for p in self.plans)
+        # flops one apply performs per precision (merged sweeps + the pair path of the rest)
+        self.flops_by_prec = {}
+        for prec in ("fp64", "fp32"):
+            on = self.sweep[prec]["active"]
+            self.flops_by_prec[prec] = sum(
+                p.sweep()["flops"] if on and p.sweep()["available"] else p.P * p.T * 2 * p.ndim * p.deg ** (p.ndim + 1)
+                for p in self.plans)
 
     def views(self, v):
         return [v[self.offsets[i]:self.offsets[i + 1]] for i in range(len(self.plans))]

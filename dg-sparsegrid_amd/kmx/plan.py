@@ -212,6 +212,16 @@ class Batch:
         check(lib().kmx_batch_create(arr, len(self.plans), C.byref(h)))
         self._h = h
 
+    def set_sweep(self, prec, on=True):
+        """Run the plans that have a dimension-sweep path as merged sweeps, one
+        launch per phase for all of them (kmx_batch_set_sweep); not under capture."""
+        check(lib().kmx_batch_set_sweep(self._h, _prec(prec), int(bool(on))))
+
+    def sweep(self, prec="fp32"):
+        m, a, ph = C.c_int(), C.c_int(), C.c_int()
+        check(lib().kmx_batch_get_sweep(self._h, _prec(prec), C.byref(m), C.byref(a), C.byref(ph)))
+        return {"members": m.value, "active": bool(a.value), "phases": ph.value}
+
     def apply(self, xs, ys, prec="fp32", stream=None):
         n = len(self.plans)
         px = (C.c_void_p * n)(*[_ptr(x).value for x in xs])

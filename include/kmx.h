@@ -162,6 +162,20 @@ int kmx_batch_create(kmx_plan* const* plans, int count, kmx_batch** batch);
 int kmx_batch_apply(kmx_batch* batch, const double* const* xs, double* const* ys, int prec, void* stream);
 int kmx_batch_destroy(kmx_batch* batch);
 
+/* Merged dimension sweeps for a batch.  on = 1: for prec (KMX_PREC_FP64 /
+ * KMX_PREC_FP32), every plan of the batch that has a dimension-sweep path
+ * (kmx_plan_get_sweep; plans created with KMX_PLAN_LEVEL_BLOCKS) runs it, and
+ * the sweeps of all such plans share one launch per phase (a 3D plan has five
+ * phases: two down, one whole operator, two up); the other plans keep the
+ * persistent pair launch.  A plan listed twice runs its sweep once per apply
+ * for its first slot only; the others stay on the pair path.  Term scales are
+ * those set when the sweep is first switched on.  on = 0 goes back to the pair
+ * path.  Synchronous (allocates); not under graph capture.  KMX_E_UNSUPPORTED
+ * when no plan has a sweep path.  get: plans in the merged sweep, whether
+ * prec uses it, and its launches per apply.  No counterpart in the reference. */
+int kmx_batch_set_sweep(kmx_batch* batch, int prec, int on);
+int kmx_batch_get_sweep(const kmx_batch* batch, int prec, int* members, int* active, int* phases);
+
 /* Y = kron(A1,...,Ak) X for k = 1..6 (kron_multd.m:30-42, Kronmult/kronmult{k}.m).
  * A[i]: DEVICE pointer, column-major nrow[i] x ncol[i] (rectangular allowed);
  * X: DEVICE, prod(ncol) x nvec column-major; Y: DEVICE, prod(nrow) x nvec.
