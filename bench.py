@@ -1,21 +1,23 @@
 #!/usr/bin/env python3
 """Operator-apply benchmark on one MI355X.
 
-Workload: 3D sparse grid, lev 7, deg 3 (27-value blocks), 3 terms (one
-deg x deg-block operator per dimension and term), the maximal connect_nD pair
-list (P = 1,903,000 element pairs over N = 1696 elements), x ~ N(0,1) seeded.
-A "step" is one full operator application y = A x.  The FP64 kernel and the
-mixed-precision (FP32) kernels are timed on the same resident operator;
-`value` is the faster pair-by-pair mixed-precision variant's applies/s.  The
-same operator also runs through the dimension sweep (a level-block plan: same
-terms in another summation order) as the runs "fp64_sweep" / "fp32_sweep".
-Extra legs: 64
-adapted instances advanced by a HIP-graph RK3 step (--no-batch skips), the
-6D lev 4 deg 2 precision study with FP32 / BF16 errors and a refined backward
-Euler step (--no-precision skips), and the CPU oracle on the host cores
-(--no-cpu skips).
+Headline workload: 64 independent adapted 2D/3D advection-diffusion
+instances (levels 4-8 in 2D / 4-6 in 3D, deg 2-4, term scales U[0.5, 2],
+one seeded refinement sweep keeping each child with probability 0.6, all
+element pairs: 63.1 M pairs), advanced by an RK3 step replayed from a HIP
+graph.  A "step" is one RK3 step of every instance (three batched applies and
+the FP64 stage updates); `value` is instance-applies/s with the applies in
+mixed precision (FP32 on the matrix cores) on the merged dimension sweeps,
+`speedup_vs_fp64_plain` its ratio to the same steps with the plain FP64 pair
+path, and `rk3_rel_l2_vs_fp64_plain` the final-state error between the two.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-precision]
+Also reported: the single 3D operator (SG lev 7, deg 3, 3 terms, connect_nD,
+P = 1,903,000) in FP64 / FP32 pair by pair and through the dimension sweep
+(--no-single skips), the 6D lev 4 deg 2 precision study with FP32 / BF16
+errors and a refined backward Euler step (--no-precision skips), and the CPU
+oracle on the host cores for both workloads (--no-cpu skips).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-single] [--no-precision]
 
 Prints ONE JSON line.
 """
@@ -50,6 +52,12 @@ MEASURED_TRAFFIC = {
     ("fp32", 256, 0): 15885619, ("fp32", 256, 1): 21643264, ("fp32", 256, 2): 20082586,
     ("fp32", 512, 0): 15846912, ("fp32", 512, 1): 21621043, ("fp32", 512, 2): 20045107,
 }
+
+
+# HBM bytes of one FP32 merged-sweep batched apply of the 64-instance workload
+# (its five phase launches together), from the same FETCH_SIZE / WRITE_SIZE
+# counter runs and scale factors.  None until measured.
+MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = None
 
 
 def build_operator(seed=2):
@@ -116,7 +124,27 @@ def cpu_baseline(op, budget_s=12.0):
                "single_thread_applies_per_s": round(1.0 / dt1, 3)}
 
 
-def batched_leg(steps=20, warmup=3):
+def time_batch_apply(sysm, prec, warmup, steps):
+    """ms per batched apply (every instance once; only the apply's own launches),
+    HIP events on the stream the kernels are launched on."""
+    import torch
+
+    x = sysm.f.clone()
+    y = torch.empty_like(x)
+    for _ in range(warmup):
+        sysm.apply(x, y, prec)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        sysm.apply(x, y, prec)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def batched_leg(steps=50, warmup=5, inst=None):
     """64 adapted 2D/3D advection-diffusion instances (all element pairs), one
     batched apply per RK3 stage, whole step replayed from a HIP graph.  Runs:
     "fp64" / "fp32" = every instance pair by pair in the persistent launch (the
@@ -129,13 +157,14 @@ def batched_leg(steps=20, warmup=3):
     from kmx import batch as BT
 
     t0 = time.perf_counter()
-    inst = BT.make_instances(count=64, seed=0)
-    build_s = time.perf_counter() - t0
-    out = {"instances": len(inst), "build_s": round(build_s, 1)}
+    if inst is None:
+        inst = BT.make_instances(count=64, seed=0)
+    out = {"instances": len(inst), "build_s": round(time.perf_counter() - t0, 1), "runs": {}}
     states = {}
     for name, prec, sweep in (("fp64", "fp64", None), ("fp32", "fp32", None),
                               ("fp32_sweep", "fp32", ("fp32",)), ("fp64_sweep", "fp64", ("fp64",))):
         sysm = BT.BatchedSystem(inst, sweep=sweep)
+        apply_ms = time_batch_apply(sysm, prec, 3, max(10, steps // 2))
         g = BT.BatchRK3Graph(sysm, prec=prec)
         for _ in range(warmup):
             g.step()
@@ -149,21 +178,47 @@ def batched_leg(steps=20, warmup=3):
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / steps
         states[name] = sysm.f.cpu().numpy()
-        out[name] = {"ms_per_rk3_step": round(ms, 4),
-                     "instance_applies_per_s": round(3 * len(inst) / (ms * 1e-3), 1),
-                     "flops_per_apply": sysm.flops_by_prec[prec],
-                     "TFLOPs": round(3 * sysm.flops_by_prec[prec] / (ms * 1e-3) / 1e12, 3)}
+        fl = sysm.flops_by_prec[prec]
+        run = {"ms_per_rk3_step": round(ms, 4),
+               "instance_applies_per_s": round(3 * len(inst) / (ms * 1e-3), 1),
+               "ms_per_batched_apply": round(apply_ms, 4),
+               "flops_per_batched_apply": fl,
+               "TFLOPs_in_apply": round(fl / (apply_ms * 1e-3) / 1e12, 3)}
         if sweep:
-            out[name]["merged_sweep"] = sysm.sweep[prec]
+            run["merged_sweep"] = sysm.sweep[prec]
+        out["runs"][name] = run
         out["pairs"] = int(sysm.pairs)
         out["dof"] = int(sysm.dof)
         del g, sysm
     for name in ("fp32", "fp32_sweep", "fp64_sweep"):
         d = states[name] - states["fp64"]
-        out[name]["rk3_rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(states["fp64"]))
-        out[name]["speedup_vs_fp64"] = round(out["fp64"]["ms_per_rk3_step"] / out[name]["ms_per_rk3_step"], 3)
-    out["steps"] = steps + warmup
-    return out
+        out["runs"][name]["rk3_rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(states["fp64"]))
+        out["runs"][name]["speedup_vs_fp64_plain"] = round(
+            out["runs"]["fp64"]["ms_per_rk3_step"] / out["runs"][name]["ms_per_rk3_step"], 3)
+    out["rk3_steps"] = steps + warmup
+    return out, inst
+
+
+def cpu_baseline_batch(inst, budget_s=12.0):
+    """The oracle (C++ FP64 restatement of apply_A.m, OpenMP over element rows)
+    applying every instance of the batch on the host cores: instance-applies/s."""
+    import oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    inputs = [oracle.ApplyInputs([[it["scales"][t] * it["mats"][t][0]] + list(it["mats"][t][1:])
+                                  for t in range(len(it["mats"]))], it["table"].idx1d, it["deg"]) for it in inst]
+    n, passes, t0 = 0, 0, time.perf_counter()
+    while True:
+        for inp, it in zip(inputs, inst):
+            oracle.apply_A(inp, it["f0"], nthreads=threads)
+            n += 1
+        passes += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt, 3), "unit": "applies/s", "cores": threads, "kind": "port",
+            "sample": f"{passes} pass(es) over all {len(inst)} instances, one FP64 apply each "
+                      "(oracle/kmx_oracle.cpp, OpenMP over element rows)"}
 
 
 def precision_leg(steps=50):
@@ -252,19 +307,69 @@ def sweep_study(steps=50):
     return rows
 
 
+def single_instance_leg(plan, tuned, op, xd, yd, steps, warmup):
+    """cfg2-sized single operator: the FP64 drop-in (pair by pair, the
+    reference's order of operations), the FP32 pair kernels, and the same
+    operator through the dimension sweep of a level-block plan."""
+    import kmx
+
+    runs = {"fp64": "fp64", "fp32": "fp32", "fp32_grouped": "fp32_grouped"}
+    res, err, flops, maxabs = {}, {}, {}, {}
+    for name, prec in runs.items():
+        res[name] = time_apply(plan, xd, yd, prec, warmup, steps)
+    y64 = plan.apply(xd, prec="fp64").cpu().numpy()
+    fl_pairs, _ = work_model(op, plan.P, "fp64")
+    for name, prec in runs.items():
+        yp = plan.apply(xd, prec=prec).cpu().numpy()
+        err[name] = float(np.linalg.norm(yp - y64) / np.linalg.norm(y64))
+        maxabs[name] = float(np.max(np.abs(yp - y64)))
+        flops[name] = fl_pairs
+    sp = kmx.Plan(op["mats"], op["table"].idx1d, op["deg"], op["rp"], op["cols"], index_base=1, fp32=True,
+                  level_blocks=True)
+    names = list(runs)
+    if sp.sweep("fp64")["available"]:
+        for prec in ("fp64", "fp32"):
+            name = prec + "_sweep"
+            sp.set_sweep(prec, True)
+            res[name] = time_apply(sp, xd, yd, prec, warmup, steps)
+            ys = sp.apply(xd, prec=prec).cpu().numpy()
+            err[name] = float(np.linalg.norm(ys - y64) / np.linalg.norm(y64))
+            maxabs[name] = float(np.max(np.abs(ys - y64)))
+            flops[name] = sp.sweep(prec)["flops"]
+            names.append(name)
+    sp.destroy()
+    v = tuned["fp32"]
+    traffic = MEASURED_TRAFFIC.get(("fp32", v["threads"], v["lds_mode"])) if v["threads"] > 0 else None
+    out = {
+        "workload": "3D SG lev7 deg3, 3 terms, connect_nD pairs (P=%d, N=%d)" % (plan.P, plan.N),
+        "data": f"synthetic: x ~ N(0,1) seed 2; 1D operators {op['source']}",
+        "block": plan.B, "tuned": tuned, "steps": steps,
+        "runs": {n: {"applies_per_s": round(1000.0 / res[n], 2), "ms_per_apply": round(res[n], 5),
+                     "rel_err_vs_fp64": err[n], "max_abs_err_vs_fp64": maxabs[n], "flops_per_apply": flops[n],
+                     "TFLOPs": round(flops[n] / (res[n] * 1e-3) / 1e12, 3)} for n in names},
+        "speedup_vs_fp64": {n: round(res["fp64"] / res[n], 3) for n in names if n != "fp64"},
+        "fp32_pair_kernel_hbm_bytes_per_launch": traffic,
+    }
+    return out, y64
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched leg")
+    ap.add_argument("--steps", type=int, default=50, help="timed RK3 steps of the 64-instance batch")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-operator (3D lev 7) legs")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 64-instance legs (profiling runs of the single-operator kernels: "
+                         "the line then has no value)")
     ap.add_argument("--no-precision", action="store_true", help="skip the 6D precision leg")
     ap.add_argument("--sweep-study", action="store_true",
                     help="only time the pair kernels against the opt-in dimension-sweep path (one JSON line)")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="for counter runs: one 256 MiB device copy (calibration), 3 applies per precision, "
-                         "then 3 level-block applies per precision of the 6D plan (unless --no-precision)")
+                         "3 merged-sweep FP32 batched applies, then 3 level-block applies per precision of the "
+                         "6D plan (unless --no-precision)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {a.gpus} requested); the operator apply is not "
@@ -286,6 +391,8 @@ def main():
     plan = kmx.Plan(op["mats"], op["table"].idx1d, op["deg"], op["rp"], op["cols"], index_base=1, fp32=True)
     tuned = plan.autotune()  # workgroup size / LDS staging / lane split per precision, timed on this plan
     if a.pmc_pass:
+        from kmx import batch as BT
+
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -293,6 +400,12 @@ def main():
         for prec in ("fp64", "fp32"):
             for _ in range(3):
                 plan.apply(xd, yd, prec=prec)
+        torch.cuda.synchronize()
+        sysm = BT.BatchedSystem(BT.make_instances(count=64, seed=0), sweep=("fp32",))
+        xb = sysm.f.clone()
+        yb = torch.empty_like(xb)
+        for _ in range(3):
+            sysm.apply(xb, yb, "fp32")
         torch.cuda.synchronize()
         if not a.no_precision:  # matrix-core kernels: the 6D level-block plan, every block pair factorized
             from kmx import coeffs as K
@@ -313,44 +426,19 @@ def main():
         print(json.dumps({"pmc_pass": True, "copy_bytes": src.numel() * 8, "tuned": tuned}), flush=True)
         return
 
-    runs = {"fp64": "fp64", "fp32": "fp32", "fp32_grouped": "fp32_grouped"}
-    res, err, flops, maxabs = {}, {}, {}, {}
-    for name, prec in runs.items():
-        res[name] = time_apply(plan, xd, yd, prec, a.warmup, a.steps)
-    y64 = plan.apply(xd, prec="fp64").cpu().numpy()
-    fl_pairs, _ = work_model(op, plan.P, "fp64")
-    for name, prec in runs.items():
-        yp = plan.apply(xd, prec=prec).cpu().numpy()
-        err[name] = float(np.linalg.norm(yp - y64) / np.linalg.norm(y64))
-        maxabs[name] = float(np.max(np.abs(yp - y64)))
-        flops[name] = fl_pairs
-    # the same operator through the dimension sweep (level-block plan: same terms, summed in
-    # another order; the FP64 baseline above stays the reference's pair-by-pair order)
-    sp = kmx.Plan(op["mats"], op["table"].idx1d, op["deg"], op["rp"], op["cols"], index_base=1, fp32=True,
-                  level_blocks=True)
-    sweep_runs = []
-    if sp.sweep("fp64")["available"]:
-        for prec in ("fp64", "fp32"):
-            name = prec + "_sweep"
-            sp.set_sweep(prec, True)
-            res[name] = time_apply(sp, xd, yd, prec, a.warmup, a.steps)
-            ys = sp.apply(xd, prec=prec).cpu().numpy()
-            err[name] = float(np.linalg.norm(ys - y64) / np.linalg.norm(y64))
-            maxabs[name] = float(np.max(np.abs(ys - y64)))
-            flops[name] = sp.sweep(prec)["flops"]
-            sweep_runs.append(name)
-    sp.destroy()
-    mixed = ("fp32", "fp32_grouped")  # the headline: pair-by-pair FP32 (the sweep is reported beside it)
-    best = min(mixed, key=lambda n: res[n])
-    ms = res[best]
-    value = 1000.0 / ms
-
-    achieved = flops[best] / (ms * 1e-3) / 1e12
-    traffic = None
-    if best == "fp32":
-        v = tuned["fp32"]
-        traffic = MEASURED_TRAFFIC.get(("fp32", v["threads"], v["lds_mode"])) if v["threads"] > 0 else None
-    kern = {"fp32": "k_apply_reg", "fp32_grouped": "k_apply_grouped"}[best]
+    if a.no_batch:
+        single, _ = single_instance_leg(plan, tuned, op, xd, yd, steps=4 * a.steps, warmup=a.warmup)
+        print(json.dumps({"metric": METRIC, "value": None, "unit": "applies/s", "n_gpus": 1,
+                          "note": "64-instance legs skipped (--no-batch)", "single_instance": single}), flush=True)
+        return
+    # the headline: 64 adapted instances, mixed precision (FP32) on the merged
+    # dimension sweeps, against the plain FP64 pair path on the same batch
+    batch, inst = batched_leg(steps=a.steps, warmup=a.warmup)
+    head = batch["runs"]["fp32_sweep"]
+    ms = head["ms_per_rk3_step"]
+    apply_ms = head["ms_per_batched_apply"]
+    achieved = head["flops_per_batched_apply"] / (apply_ms * 1e-3) / 1e12
+    traffic = MEASURED_TRAFFIC_BATCH_SWEEP_FP32
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved, 3),
@@ -358,41 +446,48 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
         "traffic": traffic,
-        "traffic_GBs": round(traffic / (ms * 1e-3) / 1e9, 1) if traffic else None,
-        "kernel": f"{kern} (fp32, d=3, deg=3): 2 d deg^(d+1) flops per pair and term / apply time",
-        "flops_per_apply": flops[best],
+        "traffic_GBs": round(traffic / (apply_ms * 1e-3) / 1e9, 1) if traffic else None,
+        "kernel": "k_ud_batch<float,2> (v_mfma_f32_16x16x4_f32), the phase launches of one batched apply: "
+                  "useful sweep flops per apply / apply time (traffic: HBM bytes per batched apply)",
+        "flops_per_apply": head["flops_per_batched_apply"],
+        "ms_per_apply": apply_ms,
     }
     out = {
         "metric": METRIC,
-        "value": round(value, 2),
+        "value": head["instance_applies_per_s"],
         "unit": "applies/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms, 5),
+        "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": f"synthetic: x ~ N(0,1) seed 2; 1D operators {op['source']}",
-        "config": {"workload": "3D SG lev7 deg3, 3 terms, connect_nD pairs (P=%d, N=%d)" % (plan.P, plan.N),
-                   "variant": best, "block": plan.B, "tuned": tuned},
-        "runs": {n: {"applies_per_s": round(1000.0 / res[n], 2), "ms_per_apply": round(res[n], 5),
-                     "rel_err_vs_fp64": err[n], "max_abs_err_vs_fp64": maxabs[n], "flops_per_apply": flops[n],
-                     "TFLOPs": round(flops[n] / (res[n] * 1e-3) / 1e12, 3)} for n in list(runs) + sweep_runs},
-        "speedup_vs_fp64": round(res["fp64"] / ms, 3),
-        "sweep_speedup_vs_fp64": {n: round(res["fp64"] / res[n], 3) for n in sweep_runs},
+        "data": "synthetic: 64 seeded adapted instances (make_instances seed 0), initial states ~ N(0,1)",
+        "config": {"workload": "64 adapted 2D/3D advection-diffusion instances, lev 4-8 (2D) / 4-6 (3D), "
+                               "deg 2-4, term scales U[0.5,2], refinement keep prob 0.6, all element pairs "
+                               "(%d pairs, %d dof); one RK3 step (3 batched applies) per step, HIP graph"
+                               % (batch["pairs"], batch["dof"]),
+                   "variant": "fp32 merged dimension sweeps (matrix cores), FP64 state and RK3 update",
+                   "instances": batch["instances"]},
+        "speedup_vs_fp64_plain": head["speedup_vs_fp64_plain"],
+        "rk3_rel_l2_vs_fp64_plain": head["rk3_rel_l2_vs_fp64"],
+        "batched": batch,
         "roofline": roofline,
     }
-    if not a.no_batch:
-        out["batched"] = batched_leg()
+    if not a.no_single:
+        single, y64 = single_instance_leg(plan, tuned, op, xd, yd, steps=200, warmup=10)
+        out["single_instance"] = single
     if not a.no_precision:
         out["precision_6d"] = precision_leg()
     if not a.no_cpu:
-        yo, cb = cpu_baseline(op)
-        out["cpu_baseline"] = cb
-        out["fp64_rel_err_vs_oracle"] = float(np.linalg.norm(y64 - yo) / np.linalg.norm(yo))
-        out["fp64_max_abs_err_vs_oracle"] = float(np.max(np.abs(y64 - yo)))
+        out["cpu_baseline"] = cpu_baseline_batch(inst)
+        if not a.no_single:
+            yo, cb = cpu_baseline(op)
+            single["cpu_baseline"] = cb
+            single["fp64_rel_err_vs_oracle"] = float(np.linalg.norm(y64 - yo) / np.linalg.norm(yo))
+            single["fp64_max_abs_err_vs_oracle"] = float(np.max(np.abs(y64 - yo)))
     print(json.dumps(out), flush=True)
 
 

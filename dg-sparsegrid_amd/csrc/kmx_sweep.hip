@@ -384,25 +384,6 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
 
 using namespace kmx;
 
-struct kmx_sweep {
-    int D = 0;
-    long long nb = 0;     // values per intermediate (every box of every level tuple)
-    long long arena = 0;  // elements
-    struct Op {
-        int part;
-        bool final;
-        UdOp a;
-        int wg0, wg1;  // workgroups (UD_WAVES work records each) of this op
-        int nbr;       // branches per tile
-    };
-    std::vector<Op> ops;
-    int* d_xmap = nullptr;
-    int* d_ymap = nullptr;
-    int* d_recs = nullptr;
-    void* d_arena[2] = {nullptr, nullptr};  // FP64, FP32
-    long long flops = 0;                    // useful flops per apply (all terms)
-};
-
 namespace {
 
 using Lev = std::vector<int>;
@@ -435,6 +416,211 @@ struct Group {
 struct HostTile {
     int m0, f0, b, t, group, la, lb, nterm, sper, steps, gsize;
 };
+
+}  // namespace
+
+struct kmx_sweep {
+    int D = 0;
+    long long nb = 0;     // values per intermediate (every box of every level tuple)
+    long long arena = 0;  // elements
+    struct Op {
+        int part;
+        bool final;
+        UdOp a;
+        int wg0, wg1;  // workgroups (UD_WAVES work records each) of this op
+        int nbr;       // branches per tile
+    };
+    std::vector<Op> ops;
+    int* d_xmap = nullptr;
+    int* d_ymap = nullptr;
+    int* d_recs = nullptr;
+    void* d_arena[2] = {nullptr, nullptr};  // FP64, FP32
+    long long flops = 0;                    // useful flops per apply (all terms)
+    // what the work records are emitted from (a batch emits them again with its own split)
+    struct OpDesc {
+        int part;
+        bool fin;
+        UdOp a;
+        long long nbranch;
+    };
+    std::vector<OpDesc> desc;
+    std::vector<Group> groups;
+    std::vector<int> group0;  // groups of op dimension d: [group0[d], group0[d+1])
+    int deg = 0, T = 0, max_nbr = 2;
+};
+
+namespace {
+
+// Load batches per wave before a tile is split over more waves: `dflt`, or
+// KMX_SWEEP_WAVE_BATCHES when set.
+int sweep_wave_batches(int dflt) {
+    if (const char* e = std::getenv("KMX_SWEEP_WAVE_BATCHES")) return std::max(1, std::atoi(e));
+    return dflt;
+}
+
+// Work records of every op of a sweep.  A tile (16 rows x 16 columns x nbr
+// branches) whose K steps exceed wave_batches load batches is split over up to
+// UD_WAVES waves of one workgroup: a lone plan (latency-bound, few tiles) uses
+// one batch per wave, a batch of plans (many tiles) four.  Widest tiles first
+// (power-of-two wave counts then never straddle a workgroup), longest first.
+void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector<int>>& recs,
+                  std::vector<kmx_sweep::Op>& ops) {
+    const int deg = sw.deg, T = sw.T;
+    recs.clear();
+    ops.clear();
+    for (const kmx_sweep::OpDesc& od : sw.desc) {
+        const int part = od.part;
+        const bool fin = od.fin;
+        const long long nbranch = od.nbranch;
+        const int nbr = int(std::min<long long>(nbranch, sw.max_nbr));  // branches per tile (a power of two)
+        kmx_sweep::Op op{part, fin, od.a, int(recs.size() / UD_WAVES), 0, nbr};
+        const int d = od.a.d;
+        const int nterm = fin ? T : 1;
+        const int batch = ud_split_batch(nbr) * std::max(1, wave_batches);
+        std::vector<HostTile> ot;
+        for (long long b = 0; b < nbranch; b += nbr)
+            for (int t = 0; t < (fin ? 1 : T); ++t)
+                for (int gi = sw.group0[d]; gi < sw.group0[d + 1]; ++gi) {
+                    const Group& g = sw.groups[gi];
+                    const int n = deg << g.m;
+                    for (int m0 = 0; m0 < n; m0 += 16) {
+                        // source levels: all (whole operator), >= the rows' lowest (down), < their highest (up)
+                        int la = 0, lb = g.m;
+                        if (part == UD_DOWN) la = host_level(m0, deg);
+                        if (part == UD_UP) lb = host_level(std::min(n, m0 + 16) - 1, deg) - 1;
+                        const int sper = steps_of(la, lb, deg);
+                        const int steps = nterm * sper;
+                        int gsize = 1;  // waves sharing the tile: at most `batch` steps each
+                        while (gsize < UD_WAVES && (steps + gsize - 1) / gsize > batch) gsize *= 2;
+                        for (int f0 = 0; f0 < g.O * g.I; f0 += 16)
+                            ot.push_back(HostTile{m0, f0, int(b), t, gi, la, lb, nterm, sper, steps, gsize});
+                    }
+                }
+        std::stable_sort(ot.begin(), ot.end(), [](const HostTile& u, const HostTile& v) {
+            return u.gsize != v.gsize ? u.gsize > v.gsize : u.steps > v.steps;
+        });
+        for (const HostTile& h : ot) {
+            const int slot = int(recs.size() % UD_WAVES);
+            if (slot % h.gsize) throw std::logic_error("sweep tile straddles a workgroup");
+            const Group& g = sw.groups[h.group];
+            const int n = deg << g.m;
+            const int qn = (h.steps + h.gsize - 1) / h.gsize;
+            for (int w = 0; w < h.gsize; ++w) {
+                std::vector<int> r(UD_REC, 0);
+                r[R_M0] = h.m0;
+                r[R_F0] = h.f0;
+                r[R_B] = h.b;
+                r[R_TOUT] = h.t;
+                r[R_O] = g.O;
+                r[R_I] = g.I;
+                r[R_LA] = h.la;
+                r[R_LB] = h.lb;
+                r[R_SPER] = h.sper;
+                r[R_GSIZE] = h.gsize;
+                r[R_GBASE] = slot;
+                r[R_MAGIC] = int(g.magic);
+                const int q0 = std::min(h.steps, w * qn), q1 = std::min(h.steps, (w + 1) * qn);
+                if (q1 > q0) {
+                    r[R_TLO] = h.t + q0 / h.sper;
+                    r[R_SLO] = q0 % h.sper;
+                    r[R_TN] = (q1 - 1) / h.sper - q0 / h.sper + 1;
+                    r[R_SHI] = (q1 - 1) % h.sper + 1;
+                } else {
+                    r[R_TLO] = h.t;
+                }
+                for (int i = 0; i < 16; ++i) {
+                    const int row = h.m0 + i;
+                    if (row < n) {
+                        const int l = host_level(row, deg);
+                        r[R_ROWBASE + i] = g.box[l] + int(row - deg * start_of(l)) * g.I;
+                        r[R_ROWNL + i] = int(deg * cells_of(l)) * g.I;
+                        r[R_ROWLEV + i] = l;
+                    } else {
+                        r[R_ROWLEV + i] = -1;
+                    }
+                }
+                for (int l = 0; l <= g.m; ++l) r[UD_BOX + l] = g.box[l];
+                recs.push_back(std::move(r));
+            }
+        }
+        while (recs.size() % UD_WAVES) recs.push_back(std::vector<int>(UD_REC, 0));  // empty slots
+        op.wg1 = int(recs.size() / UD_WAVES);
+        ops.push_back(op);
+    }
+}
+
+
+// Every work record: its tile inside a group of its op, its slots inside the
+// arena, its rows and boxes as the group defines them (host checks, before upload).
+void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& recs,
+                   const std::vector<kmx_sweep::Op>& ops) {
+    const int deg = sw.deg, T = sw.T;
+    const long long nb = sw.nb, half = 1LL << (sw.D - 1);
+    const std::vector<Group>& groups = sw.groups;
+    const std::vector<int>& group0 = sw.group0;
+    auto slot_ok = [&](long long off) { return off >= 0 && off + nb <= sw.arena; };
+    for (const auto& op : ops) {
+        const long long nbranch = op.final ? 1 : (op.part == UD_FULL ? half : (1LL << op.a.d));
+        if (op.nbr < 1 || op.nbr > 4 || nbranch % op.nbr || (op.final && op.nbr != 1))
+            throw std::logic_error("sweep branches per tile");
+        for (int wg = op.wg0; wg < op.wg1; ++wg)
+            for (int w = 0; w < UD_WAVES; ++w) {
+                const std::vector<int>& r = recs[size_t(wg) * UD_WAVES + w];
+                if (r[R_GSIZE] == 0) continue;
+                const int gbase = r[R_GBASE], gsize = r[R_GSIZE];
+                if (gsize < 1 || gsize > UD_WAVES || gbase < 0 || gbase > w || w >= gbase + gsize ||
+                    gbase + gsize > UD_WAVES)
+                    throw std::logic_error("sweep record wave group");
+                for (int v = gbase; v < gbase + gsize; ++v) {  // the waves of one tile agree on it
+                    const std::vector<int>& o = recs[size_t(wg) * UD_WAVES + v];
+                    for (int k : {int(R_M0), int(R_F0), int(R_B), int(R_TOUT), int(R_O), int(R_I), int(R_LA),
+                                  int(R_LB), int(R_SPER), int(R_GSIZE), int(R_GBASE)})
+                        if (o[k] != r[k]) throw std::logic_error("sweep record wave group");
+                }
+                // the group: same O, I and level-0 box
+                const Group* gp = nullptr;
+                for (int gi = group0[op.a.d]; gi < group0[op.a.d + 1] && !gp; ++gi)
+                    if (groups[gi].O == r[R_O] && groups[gi].I == r[R_I] && groups[gi].box[0] == r[UD_BOX])
+                        gp = &groups[gi];
+                if (!gp) throw std::logic_error("sweep record group");
+                const Group& g = *gp;
+                const int n = deg << g.m;
+                const int sper = steps_of(r[R_LA], r[R_LB], deg);
+                if (r[R_M0] < 0 || r[R_M0] >= n || r[R_M0] % 16 || r[R_F0] < 0 || r[R_F0] >= g.O * g.I ||
+                    r[R_LA] < 0 || r[R_LB] > g.m || r[R_SPER] != sper || r[R_B] < 0 || r[R_B] % op.nbr ||
+                    r[R_B] + op.nbr > nbranch ||
+                    r[R_TOUT] < 0 || r[R_TOUT] >= T || r[R_MAGIC] != int(g.magic))
+                    throw std::logic_error("sweep record outside its group");
+                if (r[R_TN] < 0 || (r[R_TN] > 0 && (r[R_TLO] < 0 || r[R_TLO] + r[R_TN] > T || r[R_SLO] < 0 ||
+                                                    r[R_SLO] >= sper || r[R_SHI] < 1 || r[R_SHI] > sper ||
+                                                    (!op.final && (r[R_TN] != 1 || r[R_TLO] != r[R_TOUT])))))
+                    throw std::logic_error("sweep record steps");
+                for (int i = 0; i < 16; ++i) {
+                    const int row = r[R_M0] + i;
+                    const int l = row < n ? host_level(row, deg) : -1;
+                    if (r[R_ROWLEV + i] != l ||
+                        (l >= 0 && (r[R_ROWBASE + i] != g.box[l] + int(row - deg * start_of(l)) * g.I ||
+                                    r[R_ROWNL + i] != int(deg * cells_of(l)) * g.I)))
+                        throw std::logic_error("sweep record rows");
+                }
+                for (int l = 0; l <= g.m; ++l)
+                    if (r[UD_BOX + l] != g.box[l]) throw std::logic_error("sweep record boxes");
+                for (int bj = r[R_B]; bj < r[R_B] + op.nbr; ++bj) {  // every branch of the tile
+                    for (int t = r[R_TLO]; t < r[R_TLO] + r[R_TN]; ++t)
+                        if (!(op.a.x0 && bj == 0) && !slot_ok(op.a.in_base + ((long long)bj * T + t) * nb))
+                            throw std::logic_error("sweep input slot outside the arena");
+                    for (int t = 0; t < T; ++t) {
+                        if (!op.final && t != r[R_TOUT]) continue;
+                        if (op.part == UD_UP &&
+                            !slot_ok(op.a.add_base + ((long long)(bj | op.a.add_bit) * T + t) * nb))
+                            throw std::logic_error("sweep add slot outside the arena");
+                        if (!op.final && !slot_ok(op.a.out_base + ((long long)(bj | op.a.out_bit) * T + t) * nb))
+                            throw std::logic_error("sweep output slot outside the arena");
+                    }
+                }
+            }
+    }
+}
 
 }  // namespace
 
@@ -558,83 +744,8 @@ void kmx_sweep_build(kmx_plan* p) {
         auto add_op = [&](int part, bool fin, UdOp a, long long nbranch) {
             a.part = part;
             a.final = fin ? 1 : 0;
-            const int nbr = int(std::min<long long>(nbranch, max_nbr));  // branches per tile (a power of two)
-            kmx_sweep::Op op{part, fin, a, int(recs.size() / UD_WAVES), 0, nbr};
-            const int d = a.d;
-            const int nterm = fin ? T : 1;
-            const int batch = ud_split_batch(nbr);
-            std::vector<HostTile> ot;
-            for (long long b = 0; b < nbranch; b += nbr)
-                for (int t = 0; t < (fin ? 1 : T); ++t)
-                    for (int gi = group0[d]; gi < group0[d + 1]; ++gi) {
-                        const Group& g = groups[gi];
-                        const int n = deg << g.m;
-                        for (int m0 = 0; m0 < n; m0 += 16) {
-                            // source levels: all (whole operator), >= the rows' lowest (down), < their highest (up)
-                            int la = 0, lb = g.m;
-                            if (part == UD_DOWN) la = host_level(m0, deg);
-                            if (part == UD_UP) lb = host_level(std::min(n, m0 + 16) - 1, deg) - 1;
-                            const int sper = steps_of(la, lb, deg);
-                            const int steps = nterm * sper;
-                            int gsize = 1;  // waves sharing the tile: at most one batch of steps each
-                            while (gsize < UD_WAVES && (steps + gsize - 1) / gsize > batch) gsize *= 2;
-                            for (int f0 = 0; f0 < g.O * g.I; f0 += 16)
-                                ot.push_back(HostTile{m0, f0, int(b), t, gi, la, lb, nterm, sper, steps, gsize});
-                        }
-                    }
-            // widest tiles first (power-of-two wave counts then never straddle a workgroup), longest first
-            std::stable_sort(ot.begin(), ot.end(), [](const HostTile& u, const HostTile& v) {
-                return u.gsize != v.gsize ? u.gsize > v.gsize : u.steps > v.steps;
-            });
-            for (const HostTile& h : ot) {
-                const int slot = int(recs.size() % UD_WAVES);
-                if (slot % h.gsize) throw std::logic_error("sweep tile straddles a workgroup");
-                const Group& g = groups[h.group];
-                const int n = deg << g.m;
-                const int qn = (h.steps + h.gsize - 1) / h.gsize;
-                for (int w = 0; w < h.gsize; ++w) {
-                    std::vector<int> r(UD_REC, 0);
-                    r[R_M0] = h.m0;
-                    r[R_F0] = h.f0;
-                    r[R_B] = h.b;
-                    r[R_TOUT] = h.t;
-                    r[R_O] = g.O;
-                    r[R_I] = g.I;
-                    r[R_LA] = h.la;
-                    r[R_LB] = h.lb;
-                    r[R_SPER] = h.sper;
-                    r[R_GSIZE] = h.gsize;
-                    r[R_GBASE] = slot;
-                    r[R_MAGIC] = int(g.magic);
-                    const int q0 = std::min(h.steps, w * qn), q1 = std::min(h.steps, (w + 1) * qn);
-                    if (q1 > q0) {
-                        r[R_TLO] = h.t + q0 / h.sper;
-                        r[R_SLO] = q0 % h.sper;
-                        r[R_TN] = (q1 - 1) / h.sper - q0 / h.sper + 1;
-                        r[R_SHI] = (q1 - 1) % h.sper + 1;
-                    } else {
-                        r[R_TLO] = h.t;
-                    }
-                    for (int i = 0; i < 16; ++i) {
-                        const int row = h.m0 + i;
-                        if (row < n) {
-                            const int l = host_level(row, deg);
-                            r[R_ROWBASE + i] = g.box[l] + int(row - deg * start_of(l)) * g.I;
-                            r[R_ROWNL + i] = int(deg * cells_of(l)) * g.I;
-                            r[R_ROWLEV + i] = l;
-                        } else {
-                            r[R_ROWLEV + i] = -1;
-                        }
-                    }
-                    for (int l = 0; l <= g.m; ++l) r[UD_BOX + l] = g.box[l];
-                    recs.push_back(std::move(r));
-                }
-            }
-            while (recs.size() % UD_WAVES) recs.push_back(std::vector<int>(UD_REC, 0));  // empty slots
-            op.wg1 = int(recs.size() / UD_WAVES);
-            sw->ops.push_back(op);
+            sw->desc.push_back(kmx_sweep::OpDesc{part, fin, a, nbranch});
         };
-        // host checks need each record's tile: rebuilt from the records themselves below
         long long flops = 0;
         if (D == 1) {
             UdOp a = base_op(0);
@@ -673,6 +784,12 @@ void kmx_sweep_build(kmx_plan* p) {
             }
         }
         sw->flops = flops;
+        sw->deg = deg;
+        sw->T = T;
+        sw->max_nbr = max_nbr;
+        sw->groups = groups;
+        sw->group0 = group0;
+        emit_records(*sw, sweep_wave_batches(1), recs, sw->ops);
         // ---- box maps: x -> boxes (read side) and boxes -> y
         std::vector<int> xmap(size_t(nb), -1);
         std::vector<int> ymap(size_t(nb), -1);
@@ -706,70 +823,7 @@ void kmx_sweep_build(kmx_plan* p) {
                 ymap[size_t(pos)] = e * p->B + a;
             }
         }
-        // ---- every record: its tile inside a group of its op, its slots inside the arena,
-        // its rows and boxes as the group defines them (checked here, on the host)
-        auto slot_ok = [&](long long off) { return off >= 0 && off + nb <= sw->arena; };
-        for (const auto& op : sw->ops) {
-            const long long nbranch = op.final ? 1 : (op.part == UD_FULL ? half : (1LL << op.a.d));
-            if (op.nbr < 1 || op.nbr > 4 || nbranch % op.nbr || (op.final && op.nbr != 1))
-                throw std::logic_error("sweep branches per tile");
-            for (int wg = op.wg0; wg < op.wg1; ++wg)
-                for (int w = 0; w < UD_WAVES; ++w) {
-                    const std::vector<int>& r = recs[size_t(wg) * UD_WAVES + w];
-                    if (r[R_GSIZE] == 0) continue;
-                    const int gbase = r[R_GBASE], gsize = r[R_GSIZE];
-                    if (gsize < 1 || gsize > UD_WAVES || gbase < 0 || gbase > w || w >= gbase + gsize ||
-                        gbase + gsize > UD_WAVES)
-                        throw std::logic_error("sweep record wave group");
-                    for (int v = gbase; v < gbase + gsize; ++v) {  // the waves of one tile agree on it
-                        const std::vector<int>& o = recs[size_t(wg) * UD_WAVES + v];
-                        for (int k : {int(R_M0), int(R_F0), int(R_B), int(R_TOUT), int(R_O), int(R_I), int(R_LA),
-                                      int(R_LB), int(R_SPER), int(R_GSIZE), int(R_GBASE)})
-                            if (o[k] != r[k]) throw std::logic_error("sweep record wave group");
-                    }
-                    // the group: same O, I and level-0 box
-                    const Group* gp = nullptr;
-                    for (int gi = group0[op.a.d]; gi < group0[op.a.d + 1] && !gp; ++gi)
-                        if (groups[gi].O == r[R_O] && groups[gi].I == r[R_I] && groups[gi].box[0] == r[UD_BOX])
-                            gp = &groups[gi];
-                    if (!gp) throw std::logic_error("sweep record group");
-                    const Group& g = *gp;
-                    const int n = deg << g.m;
-                    const int sper = steps_of(r[R_LA], r[R_LB], deg);
-                    if (r[R_M0] < 0 || r[R_M0] >= n || r[R_M0] % 16 || r[R_F0] < 0 || r[R_F0] >= g.O * g.I ||
-                        r[R_LA] < 0 || r[R_LB] > g.m || r[R_SPER] != sper || r[R_B] < 0 || r[R_B] % op.nbr ||
-                        r[R_B] + op.nbr > nbranch ||
-                        r[R_TOUT] < 0 || r[R_TOUT] >= T || r[R_MAGIC] != int(g.magic))
-                        throw std::logic_error("sweep record outside its group");
-                    if (r[R_TN] < 0 || (r[R_TN] > 0 && (r[R_TLO] < 0 || r[R_TLO] + r[R_TN] > T || r[R_SLO] < 0 ||
-                                                        r[R_SLO] >= sper || r[R_SHI] < 1 || r[R_SHI] > sper ||
-                                                        (!op.final && (r[R_TN] != 1 || r[R_TLO] != r[R_TOUT])))))
-                        throw std::logic_error("sweep record steps");
-                    for (int i = 0; i < 16; ++i) {
-                        const int row = r[R_M0] + i;
-                        const int l = row < n ? host_level(row, deg) : -1;
-                        if (r[R_ROWLEV + i] != l ||
-                            (l >= 0 && (r[R_ROWBASE + i] != g.box[l] + int(row - deg * start_of(l)) * g.I ||
-                                        r[R_ROWNL + i] != int(deg * cells_of(l)) * g.I)))
-                            throw std::logic_error("sweep record rows");
-                    }
-                    for (int l = 0; l <= g.m; ++l)
-                        if (r[UD_BOX + l] != g.box[l]) throw std::logic_error("sweep record boxes");
-                    for (int bj = r[R_B]; bj < r[R_B] + op.nbr; ++bj) {  // every branch of the tile
-                        for (int t = r[R_TLO]; t < r[R_TLO] + r[R_TN]; ++t)
-                            if (!(op.a.x0 && bj == 0) && !slot_ok(op.a.in_base + ((long long)bj * T + t) * nb))
-                                throw std::logic_error("sweep input slot outside the arena");
-                        for (int t = 0; t < T; ++t) {
-                            if (!op.final && t != r[R_TOUT]) continue;
-                            if (op.part == UD_UP &&
-                                !slot_ok(op.a.add_base + ((long long)(bj | op.a.add_bit) * T + t) * nb))
-                                throw std::logic_error("sweep add slot outside the arena");
-                            if (!op.final && !slot_ok(op.a.out_base + ((long long)(bj | op.a.out_bit) * T + t) * nb))
-                                throw std::logic_error("sweep output slot outside the arena");
-                        }
-                    }
-                }
-        }
+        check_records(*sw, recs, sw->ops);
         // records interleaved so that lane j reads ints j and 64 + j as one int2
         std::vector<int> flat(recs.size() * UD_REC);
         for (size_t i = 0; i < recs.size(); ++i)
@@ -867,24 +921,31 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members) {
     int nph = 0;
     for (size_t q = 0; q < members.size(); ++q) {
         const kmx_plan* p = members[q];
-        if (!p || !p->sw || p->sw->ops.empty()) throw std::invalid_argument("sweep batch member without a sweep");
+        if (!p || !p->sw || p->sw->desc.empty()) throw std::invalid_argument("sweep batch member without a sweep");
         for (size_t r = 0; r < q; ++r)
             if (members[r] == p) throw std::invalid_argument("sweep batch member twice (one arena per plan)");
-        nph = std::max(nph, int(p->sw->ops.size()));
+        nph = std::max(nph, int(p->sw->desc.size()));
+    }
+    // every member's records again, for throughput: fewer waves per tile
+    std::vector<std::vector<std::vector<int>>> recs(members.size());
+    std::vector<std::vector<kmx_sweep::Op>> mops(members.size());
+    for (size_t q = 0; q < members.size(); ++q) {
+        emit_records(*members[q]->sw, sweep_wave_batches(4), recs[q], mops[q]);
+        check_records(*members[q]->sw, recs[q], mops[q]);
     }
     auto* sb = new kmx_sweep_batch();
     try {
         std::vector<UdBatchOp> ops;
         std::vector<int> wgop;
-        std::vector<std::pair<const int*, long long>> copies;  // (source records, workgroups)
+        std::vector<int> flat;  // records interleaved so that lane j reads ints j and 64 + j as one int2
         for (int ph = 0; ph < nph; ++ph) {
             kmx_sweep_batch::Phase P{int(wgop.size()), 0, 1};
             for (size_t q = 0; q < members.size(); ++q) {
                 kmx_plan* p = members[q];
                 const kmx_sweep* sw = p->sw;
-                const int k = ph - (nph - int(sw->ops.size()));
+                const int k = ph - (nph - int(mops[q].size()));
                 if (k < 0) continue;
-                const kmx_sweep::Op& op = sw->ops[size_t(k)];
+                const kmx_sweep::Op& op = mops[q][size_t(k)];
                 UdBatchOp o{};
                 o.a = op.a;
                 if (op.final)
@@ -902,30 +963,30 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members) {
                 P.maxnbr = std::max(P.maxnbr, op.nbr);
                 const int oi = int(ops.size());
                 ops.push_back(o);
-                for (int wg = op.wg0; wg < op.wg1; ++wg) wgop.push_back(oi);
-                copies.emplace_back(sw->d_recs + size_t(op.wg0) * UD_WAVES * UD_REC, op.wg1 - op.wg0);
+                for (int wg = op.wg0; wg < op.wg1; ++wg) {
+                    wgop.push_back(oi);
+                    for (int w = 0; w < UD_WAVES; ++w) {
+                        const std::vector<int>& r = recs[q][size_t(wg) * UD_WAVES + w];
+                        for (int j = 0; j < UD_REC / 2; ++j) {
+                            flat.push_back(r[j]);
+                            flat.push_back(r[UD_REC / 2 + j]);
+                        }
+                    }
+                }
                 P.nwg += op.wg1 - op.wg0;
                 if (wgop.size() >= size_t(1) << 30) throw std::runtime_error("sweep batch too large");
             }
             sb->phases.push_back(P);
         }
-        auto alloc = [](auto*& d, size_t n) {
-            if (hipMalloc(&d, std::max<size_t>(1, n) * sizeof(*d)) != hipSuccess) throw std::bad_alloc();
-        };
-        alloc(sb->d_ops, ops.size());
-        alloc(sb->d_wgop, wgop.size());
-        alloc(sb->d_recs, wgop.size() * UD_WAVES * UD_REC);
-        if (hipMemcpy(sb->d_ops, ops.data(), ops.size() * sizeof(UdBatchOp), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(sb->d_wgop, wgop.data(), wgop.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
-            throw std::runtime_error("hipMemcpy failed");
-        size_t at = 0;  // records land in the order their workgroups were listed
-        for (const auto& c : copies) {
-            const size_t n = size_t(c.second) * UD_WAVES * UD_REC;
-            if (n && hipMemcpy(sb->d_recs + at, c.first, n * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess)
+        if (flat.size() != wgop.size() * UD_WAVES * UD_REC) throw std::logic_error("sweep batch records");
+        auto up = [](const auto* h, size_t n, auto*& d) {
+            if (hipMalloc(&d, std::max<size_t>(1, n) * sizeof(*h)) != hipSuccess) throw std::bad_alloc();
+            if (n && hipMemcpy(d, h, n * sizeof(*h), hipMemcpyHostToDevice) != hipSuccess)
                 throw std::runtime_error("hipMemcpy failed");
-            at += n;
-        }
-        if (at != wgop.size() * UD_WAVES * UD_REC) throw std::logic_error("sweep batch records");
+        };
+        up(ops.data(), ops.size(), sb->d_ops);
+        up(wgop.data(), wgop.size(), sb->d_wgop);
+        up(flat.data(), flat.size(), sb->d_recs);
     } catch (...) {
         kmx_sweep_batch_free(sb);
         throw;
