@@ -349,6 +349,7 @@ int kmx_plan_update_term(kmx_plan* p, int term, int dim, const double* mat, void
         // the float staging vectors die here: wait for the copies that read them
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         kmx_blocks_update_term(p, term, dim, mat, s);
+        kmx_sweep_term_updated(p, term, dim);
         return KMX_OK;
     } catch (const std::exception& e) {
         set_error(std::string("kmx_plan_update_term: ") + e.what());

@@ -193,6 +193,8 @@ void kmx_sweep_build(kmx_plan* p);
 int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s);
 void kmx_sweep_free(kmx_plan* p);
 double kmx_sweep_flops(const kmx_plan* p);
+// after kmx_plan_update_term(term, dim) (the FP64 block copy already updated)
+void kmx_sweep_term_updated(kmx_plan* p, int term, int dim);
 // The sweeps of several plans merged into one launch per phase (kmx_sweep.hip):
 // phase k runs op k - (phases - ops of the plan) of every plan, so each plan's ops
 // keep their order and every plan's last op falls in the last phase.  members:

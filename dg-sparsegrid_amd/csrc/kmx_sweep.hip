@@ -38,6 +38,7 @@
 #include "kmx_internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <map>
 #include <set>
@@ -99,6 +100,7 @@ constexpr int UD_WAVES = 8;                     // waves per workgroup
 struct UdOp {
     int d, deg, lda, T;
     int part, final, x0, out_bit, add_bit;
+    int add_y;  // final op: add into y (the ops of a separable sweep after the first)
     long long nb;
     long long in_base, out_base, add_base;
     long long a_off[KMX_MAX_TERMS];  // column-major masked matrix of (term, d)
@@ -286,7 +288,10 @@ __device__ __forceinline__ void ud_tile(const T* __restrict__ mat, T* __restrict
     if (FINAL) {
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (ook[q] && yi[q] >= 0) y[yi[q]] = double(acc[0][q] + addv[0][q]);
+            if (ook[q] && yi[q] >= 0) {
+                const double v = double(acc[0][q] + addv[0][q]);
+                y[yi[q]] = op.add_y ? y[yi[q]] + v : v;
+            }
     } else {
 #pragma unroll
         for (int j = 0; j < NBR; ++j) {
@@ -429,6 +434,7 @@ struct kmx_sweep {
         UdOp a;
         int wg0, wg1;  // workgroups (UD_WAVES work records each) of this op
         int nbr;       // branches per tile
+        std::vector<int> terms;  // plan term of each of the op's terms (scales of a final op)
     };
     std::vector<Op> ops;
     int* d_xmap = nullptr;
@@ -442,8 +448,17 @@ struct kmx_sweep {
         bool fin;
         UdOp a;
         long long nbranch;
+        std::vector<int> terms;
     };
     std::vector<OpDesc> desc;
+    // separable form: every term acts in at most one dimension (its other 1D
+    // factors are the identity), so the apply is one whole-operator op per
+    // dimension, x to y; identity[t * D + k] records the factors taken as the
+    // identity, and stale is set when kmx_plan_update_term replaces one of them
+    // by another matrix
+    bool separable = false;
+    std::vector<char> identity;
+    bool stale = false;
     std::vector<Group> groups;
     std::vector<int> group0;  // groups of op dimension d: [group0[d], group0[d+1])
     int deg = 0, T = 0, max_nbr = 2;
@@ -465,15 +480,16 @@ int sweep_wave_batches(int dflt) {
 // (power-of-two wave counts then never straddle a workgroup), longest first.
 void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector<int>>& recs,
                   std::vector<kmx_sweep::Op>& ops) {
-    const int deg = sw.deg, T = sw.T;
+    const int deg = sw.deg;
     recs.clear();
     ops.clear();
     for (const kmx_sweep::OpDesc& od : sw.desc) {
+        const int T = od.a.T;  // the op's terms
         const int part = od.part;
         const bool fin = od.fin;
         const long long nbranch = od.nbranch;
         const int nbr = int(std::min<long long>(nbranch, sw.max_nbr));  // branches per tile (a power of two)
-        kmx_sweep::Op op{part, fin, od.a, int(recs.size() / UD_WAVES), 0, nbr};
+        kmx_sweep::Op op{part, fin, od.a, int(recs.size() / UD_WAVES), 0, nbr, od.terms};
         const int d = od.a.d;
         const int nterm = fin ? T : 1;
         const int batch = ud_split_batch(nbr) * std::max(1, wave_batches);
@@ -554,12 +570,16 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
 // arena, its rows and boxes as the group defines them (host checks, before upload).
 void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& recs,
                    const std::vector<kmx_sweep::Op>& ops) {
-    const int deg = sw.deg, T = sw.T;
+    const int deg = sw.deg;
     const long long nb = sw.nb, half = 1LL << (sw.D - 1);
     const std::vector<Group>& groups = sw.groups;
     const std::vector<int>& group0 = sw.group0;
     auto slot_ok = [&](long long off) { return off >= 0 && off + nb <= sw.arena; };
     for (const auto& op : ops) {
+        const int T = op.a.T;
+        if (T < 1 || T > KMX_MAX_TERMS || int(op.terms.size()) != T) throw std::logic_error("sweep op terms");
+        for (int t : op.terms)
+            if (t < 0 || t >= sw.T) throw std::logic_error("sweep op terms");
         const long long nbranch = op.final ? 1 : (op.part == UD_FULL ? half : (1LL << op.a.d));
         if (op.nbr < 1 || op.nbr > 4 || nbranch % op.nbr || (op.final && op.nbr != 1))
             throw std::logic_error("sweep branches per tile");
@@ -623,6 +643,23 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
 }
 
 }  // namespace
+
+// Whether 1D factor (t, k) of the plan is the identity: every diagonal block
+// within 4 ulps of I and every other block within 4 ulps of 0 (the FP64 copy the
+// plan keeps, block-row-major).
+bool identity_factor(const kmx_plan* p, int t, int k) {
+    const double tol = 4.0 * 2.220446049250313e-16;
+    const int deg = p->deg, nbk = p->nb[k];
+    const double* m = p->h_blk64.data() + p->off[t * p->D + k];
+    for (long long I = 0; I < nbk; ++I)
+        for (long long J = 0; J < nbk; ++J) {
+            const double* b = m + (I * nbk + J) * deg * deg;
+            for (int a = 0; a < deg; ++a)
+                for (int c = 0; c < deg; ++c)
+                    if (std::fabs(b[a * deg + c] - (I == J && a == c ? 1.0 : 0.0)) > tol) return false;
+        }
+    return true;
+}
 
 void kmx_sweep_build(kmx_plan* p) {
     kmx_blockset* bs = p->bs;
@@ -741,16 +778,56 @@ void kmx_sweep_build(kmx_plan* p) {
             }
             return a;
         };
-        auto add_op = [&](int part, bool fin, UdOp a, long long nbranch) {
+        std::vector<int> all_terms(T);
+        for (int t = 0; t < T; ++t) all_terms[t] = t;
+        auto add_op = [&](int part, bool fin, UdOp a, long long nbranch, const std::vector<int>& terms) {
             a.part = part;
             a.final = fin ? 1 : 0;
-            sw->desc.push_back(kmx_sweep::OpDesc{part, fin, a, nbranch});
+            sw->desc.push_back(kmx_sweep::OpDesc{part, fin, a, nbranch, terms});
         };
         long long flops = 0;
-        if (D == 1) {
+        // separable terms: each acts in one dimension (a term that acts in none is
+        // taken as acting in the first); the terms of dimension d share one whole-
+        // operator op on d that reads x and writes (first op) or adds to y
+        std::vector<char> ident(size_t(T) * D, 0);
+        std::vector<int> active(T, 0);
+        bool separable = !p->h_blk64.empty();
+        for (int t = 0; t < T && separable; ++t) {
+            int na = 0;
+            for (int k = 0; k < D; ++k) {
+                ident[size_t(t) * D + k] = identity_factor(p, t, k) ? 1 : 0;
+                if (!ident[size_t(t) * D + k]) {
+                    active[t] = k;
+                    ++na;
+                }
+            }
+            separable = na <= 1;
+        }
+        if (separable) {
+            sw->separable = true;
+            sw->identity = ident;
+            bool first = true;
+            for (int d = 0; d < D; ++d) {
+                std::vector<int> ts;
+                for (int t = 0; t < T; ++t)
+                    if (active[t] == d) ts.push_back(t);
+                if (ts.empty()) continue;
+                UdOp a = base_op(d);
+                a.T = int(ts.size());
+                for (int i = 0; i < KMX_MAX_TERMS; ++i) {
+                    a.a_off[i] = i < a.T ? p->off[ts[size_t(i)] * D + d] : 0;
+                    a.scale[i] = 1.0;
+                }
+                a.x0 = 1;
+                a.add_y = first ? 0 : 1;
+                first = false;
+                add_op(UD_FULL, true, a, 1, ts);
+                flops += fl_full[d] * a.T;
+            }
+        } else if (D == 1) {
             UdOp a = base_op(0);
             a.x0 = 1;
-            add_op(UD_FULL, true, a, 1);
+            add_op(UD_FULL, true, a, 1, all_terms);
             flops += fl_full[0] * T;
         } else {
             for (int d = 0; d + 1 < D; ++d) {  // down: slot b -> slot b | 2^d
@@ -759,7 +836,7 @@ void kmx_sweep_build(kmx_plan* p) {
                 a.in_base = zb - T * nb;
                 a.out_base = zb - T * nb;
                 a.out_bit = 1 << d;
-                add_op(UD_DOWN, false, a, 1LL << d);
+                add_op(UD_DOWN, false, a, 1LL << d, all_terms);
                 flops += fl_down[d] * T * (1LL << d);
             }
             {  // whole operator on dimension D-1: slot b -> whole-operator slot b
@@ -767,7 +844,7 @@ void kmx_sweep_build(kmx_plan* p) {
                 a.x0 = 1;
                 a.in_base = zb - T * nb;
                 a.out_base = gb;
-                add_op(UD_FULL, false, a, half);
+                add_op(UD_FULL, false, a, half, all_terms);
                 flops += fl_full[D - 1] * T * half;
             }
             long long src = gb;  // up: slot b + slot b | 2^d -> slot b of the other array
@@ -778,11 +855,12 @@ void kmx_sweep_build(kmx_plan* p) {
                 a.add_base = src;
                 a.add_bit = 1 << d;
                 a.out_base = dst;
-                add_op(UD_UP, d == 0, a, 1LL << d);
+                add_op(UD_UP, d == 0, a, 1LL << d, all_terms);
                 flops += fl_up[d] * T * (1LL << d);
                 src = dst;
             }
         }
+        if (sw->separable) sw->arena = nb;  // x in, y out: no intermediates
         sw->flops = flops;
         sw->deg = deg;
         sw->T = T;
@@ -851,13 +929,18 @@ void kmx_sweep_build(kmx_plan* p) {
 
 int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s) {
     kmx_sweep* sw = p->sw;
+    if (sw->stale) {
+        set_error("kmx_apply: a 1D factor the dimension sweep takes as the identity was replaced "
+                  "(kmx_plan_update_term); switch the sweep off (kmx_plan_set_sweep) or recreate the plan");
+        return KMX_E_UNSUPPORTED;
+    }
     kmx_blockset* bs = p->bs;
     auto run = [&](auto* arena, const auto* mat) {
         using T = std::remove_pointer_t<decltype(arena)>;
         for (const auto& op : sw->ops) {
             UdOp a = op.a;
             if (op.final)
-                for (int t = 0; t < p->T; ++t) a.scale[t] = p->scale[t];
+                for (int i = 0; i < a.T; ++i) a.scale[i] = p->scale[op.terms[size_t(i)]];
             const dim3 grid(unsigned(op.wg1 - op.wg0)), block(UD_WAVES * 64);
             const int2* rc = reinterpret_cast<const int2*>(sw->d_recs) + size_t(op.wg0) * UD_WAVES * (UD_REC / 2);
             auto pick = [&](auto nbr_c) {
@@ -910,6 +993,7 @@ struct kmx_sweep_batch {
         int wg0, nwg, maxnbr;  // first workgroup, workgroups, most branches per tile
     };
     std::vector<Phase> phases;
+    std::vector<const kmx_plan*> members;
     UdBatchOp* d_ops = nullptr;
     int* d_wgop = nullptr;  // op of every workgroup, phases in order
     int* d_recs = nullptr;  // UD_WAVES work records per workgroup, copied from the plans
@@ -934,6 +1018,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members) {
         check_records(*members[q]->sw, recs[q], mops[q]);
     }
     auto* sb = new kmx_sweep_batch();
+    sb->members.assign(members.begin(), members.end());
     try {
         std::vector<UdBatchOp> ops;
         std::vector<int> wgop;
@@ -949,7 +1034,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members) {
                 UdBatchOp o{};
                 o.a = op.a;
                 if (op.final)
-                    for (int t = 0; t < p->T; ++t) o.a.scale[t] = p->scale[t];
+                    for (int i = 0; i < o.a.T; ++i) o.a.scale[i] = p->scale[op.terms[size_t(i)]];
                 o.mat[0] = p->bs->d_mat64;
                 o.mat[1] = p->bs->d_mat32;
                 o.arena[0] = sw->d_arena[0];
@@ -995,6 +1080,12 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members) {
 }
 
 int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io, hipStream_t s) {
+    for (const kmx_plan* p : sb->members)
+        if (p->sw && p->sw->stale) {
+            set_error("kmx_batch_apply: a 1D factor a member's dimension sweep takes as the identity was "
+                      "replaced (kmx_plan_update_term); switch the batch sweep off and on again");
+            return KMX_E_UNSUPPORTED;
+        }
     if (slot == 1 && !sb->has32) {
         set_error("kmx_batch_apply: a sweep member has no FP32 blocks");
         return KMX_E_ARG;
@@ -1029,3 +1120,9 @@ void kmx_sweep_batch_free(kmx_sweep_batch* sb) {
 }
 
 int kmx_sweep_batch_phases(const kmx_sweep_batch* sb) { return sb ? int(sb->phases.size()) : 0; }
+
+void kmx_sweep_term_updated(kmx_plan* p, int term, int dim) {
+    kmx_sweep* sw = p->sw;
+    if (!sw || !sw->separable) return;
+    if (sw->identity[size_t(term) * p->D + dim] && !identity_factor(p, term, dim)) sw->stale = true;
+}

@@ -89,7 +89,8 @@ def test_batched_sweep_mixed_dims_matches_oracle(kmx_gpu):
     """Merged dimension sweeps (one launch per phase for every plan that has a
     sweep path) next to plans on the pair path: 2D to 4D (and a 1D plan, which
     has no sweep), connect_nD and all pairs, an adapted grid with partially filled level blocks, a plan listed
-    twice and one without level blocks; FP64 1e-13 and FP32 5e-6 against the
+    twice, one without level blocks and one whose terms each act in one
+    dimension (the separable form: one op per dimension); FP64 1e-13 and FP32 5e-6 against the
     oracle, bit-identical on a second apply, and the pair path back after
     switching the sweep off."""
     import torch
@@ -103,10 +104,16 @@ def test_batched_sweep_mixed_dims_matches_oracle(kmx_gpu):
     n = 2 * 2 ** int(table.lev.max())
     cases.append(dict(mats=[[rng.standard_normal((n, n)) for _ in range(3)] for _ in range(2)], idx=table.idx1d,
                       deg=2, rp=None, cols=None, x=rng.standard_normal(table.num_elem * 8)))
+    sep = _lb_case(3, 4, 3, 3, True, 57)  # separable: term t acts in dimension t only
+    for t in range(3):
+        for k in range(3):
+            if k != t:
+                sep["mats"][t][k] = np.eye(sep["mats"][t][k].shape[0])
+    cases.append(sep)
     plans = [kmx.Plan(c["mats"], c["idx"], c["deg"], c["rp"], c["cols"], index_base=1, fp32=True,
                       level_blocks=True) for c in cases]
     # level blocks need d >= 2: the 1D plan stays on the pair path
-    assert [p.sweep("fp64")["available"] for p in plans] == [False] + [True] * 5
+    assert [p.sweep("fp64")["available"] for p in plans] == [False] + [True] * 6
     scales = [1.0, 0.5, 1.0]
     for t, s in enumerate(scales):
         plans[2].set_term_scale(t, s)
@@ -119,7 +126,7 @@ def test_batched_sweep_mixed_dims_matches_oracle(kmx_gpu):
     for prec, tol in (("fp64", 1e-13), ("fp32", 5e-6)):
         batch.set_sweep(prec, True)
         info = batch.sweep(prec)
-        assert info == {"members": 5, "active": True, "phases": 7}, info
+        assert info == {"members": 6, "active": True, "phases": 7}, info
         ys = [torch.full_like(x, np.nan) for x in xs]
         batch.apply(xs, ys, prec=prec)
         got = [y.cpu().numpy() for y in ys]
