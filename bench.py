@@ -124,24 +124,35 @@ def cpu_baseline(op, budget_s=12.0):
                "single_thread_applies_per_s": round(1.0 / dt1, 3)}
 
 
-def time_batch_apply(sysm, prec, warmup, steps):
-    """ms per batched apply (every instance once; only the apply's own launches),
-    HIP events on the stream the kernels are launched on."""
+def time_batch_apply(sysm, prec, warmup, steps, per_graph=10):
+    """ms per batched apply (every instance once; only the apply's own launches):
+    per_graph applies captured in a HIP graph (no host work between them),
+    replayed, HIP events on the replay stream."""
     import torch
 
     x = sysm.f.clone()
     y = torch.empty_like(x)
-    for _ in range(warmup):
-        sysm.apply(x, y, prec)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(warmup):
+            sysm.apply(x, y, prec)
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(per_graph):
+            sysm.apply(x, y, prec)
+    g.replay()
     torch.cuda.synchronize()
+    reps = max(1, steps // per_graph)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
-    for _ in range(steps):
-        sysm.apply(x, y, prec)
+    for _ in range(reps):
+        g.replay()
     e1.record()
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / steps
+    return e0.elapsed_time(e1) / (reps * per_graph)
 
 
 def batched_leg(steps=50, warmup=5, inst=None):

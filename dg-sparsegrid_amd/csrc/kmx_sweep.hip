@@ -71,7 +71,10 @@ struct SwMfma<float> {
     static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-enum { UD_DOWN = 0, UD_FULL = 1, UD_UP = 2 };
+// UD_SEP: one dimension's op of a separable sweep: the whole operators of its terms,
+// summed, from the box copy of x (slot in_base), plus the previous dimensions' sum
+// (slot add_base, when add_y), into slot out_base or, the last op, into y.
+enum { UD_DOWN = 0, UD_FULL = 1, UD_UP = 2, UD_SEP = 3 };
 
 // Work record of one wave (UD_REC ints, stored so that lane j reads ints j and
 // 64 + j as one int2).  The tile: rows [m0, m0+16) of the group's positions
@@ -100,7 +103,7 @@ constexpr int UD_WAVES = 8;                     // waves per workgroup
 struct UdOp {
     int d, deg, lda, T;
     int part, final, x0, out_bit, add_bit;
-    int add_y;  // final op: add into y (the ops of a separable sweep after the first)
+    int add_y;  // UD_SEP: add slot add_base (every op of a separable sweep after the first)
     long long nb;
     long long in_base, out_base, add_base;
     long long a_off[KMX_MAX_TERMS];  // column-major masked matrix of (term, d)
@@ -182,6 +185,9 @@ __device__ __forceinline__ void ud_tile(const T* __restrict__ mat, T* __restrict
                         for (int q = 0; q < 4; ++q) addv[j][q] += sc * add[oaddr[q]];
                     }
             }
+            if (PART == UD_SEP && op.add_y)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) addv[0][q] = arena[op.add_base + oaddr[q]];
             if (FINAL)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) yi[q] = ymap[oaddr[q]];
@@ -199,8 +205,10 @@ __device__ __forceinline__ void ud_tile(const T* __restrict__ mat, T* __restrict
             const T* bbase[NBR];
 #pragma unroll
             for (int j = 0; j < NBR; ++j)
-                bbase[j] = arena + ((j == 0 && xin0) ? 0 : op.in_base + ((long long)(b + j) * op.T + t) * op.nb);
-            const T sc = FINAL ? T(op.scale[t]) : T(1);
+                bbase[j] = arena + (PART == UD_SEP                ? op.in_base
+                                    : (j == 0 && xin0) ? 0
+                                                       : op.in_base + ((long long)(b + j) * op.T + t) * op.nb);
+            const T sc = (FINAL || PART == UD_SEP) ? T(op.scale[t]) : T(1);
             int l = la, rem = s0;  // level and step of step s0 (wave-uniform)
             for (;;) {
                 const int ns = (op.deg * (l ? 1 << (l - 1) : 1) + 3) >> 2;
@@ -221,7 +229,7 @@ __device__ __forceinline__ void ud_tile(const T* __restrict__ mat, T* __restrict
                         const int st = l ? 1 << (l - 1) : 0;
                         nl = op.deg * (l ? st : 1);
                         const int box = __builtin_amdgcn_readlane(rv.y, min(l, UD_MAXLEV));
-                        aon = rok && (PART == UD_FULL || (PART == UD_DOWN ? lr <= l : lr > l));
+                        aon = rok && (PART == UD_FULL || PART == UD_SEP || (PART == UD_DOWN ? lr <= l : lr > l));
                         aidx = r0 + (op.deg * st + p + lk) * op.lda;
                         bidx = box + (fo * nl + p + lk) * I + fi;
                         newlev = false;
@@ -288,14 +296,12 @@ __device__ __forceinline__ void ud_tile(const T* __restrict__ mat, T* __restrict
     if (FINAL) {
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (ook[q] && yi[q] >= 0) {
-                const double v = double(acc[0][q] + addv[0][q]);
-                y[yi[q]] = op.add_y ? y[yi[q]] + v : v;
-            }
+            if (ook[q] && yi[q] >= 0) y[yi[q]] = double(acc[0][q] + addv[0][q]);
     } else {
 #pragma unroll
         for (int j = 0; j < NBR; ++j) {
-            T* out = arena + op.out_base + ((long long)((b + j) | op.out_bit) * op.T + tout) * op.nb;
+            T* out = arena + op.out_base +
+                     (PART == UD_SEP ? 0 : ((long long)((b + j) | op.out_bit) * op.T + tout) * op.nb);
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 if (ook[q]) out[oaddr[q]] = acc[j][q] + addv[j][q];
@@ -316,18 +322,55 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ m
     ud_tile<T, PART, FINAL, NBR>(mat, arena, rv, w, lane, op, x, xmap, y, ymap, part);
 }
 
+// Box copy of x for a separable sweep: xb[i] = x[xmap[i]], 0 where the cell is absent.
+template <typename T>
+__global__ void __launch_bounds__(256) k_box_gather(const double* __restrict__ x, const int* __restrict__ xmap,
+                                                    T* __restrict__ xb, long long nb) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nb; i += (long long)gridDim.x * 256) {
+        const int m = xmap[i];
+        xb[i] = m >= 0 ? T(x[m]) : T(0);
+    }
+}
+
+// The box copies of x of every separable instance of a batch in one launch:
+// job j covers workgroups [blk0_j, blk0_{j+1}), 256 positions each.
+struct GatherJob {
+    const int* xmap;
+    void* xb[2];  // FP64, FP32 arena of the plan (slot 0)
+    long long nb;
+    int inst, blk0;
+};
+template <typename T>
+__global__ void __launch_bounds__(256) k_box_gather_batch(const GatherJob* __restrict__ jobs, int njobs, BatchIO io) {
+    int lo = 0, hi = njobs - 1;  // the last job starting at or before this workgroup
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].blk0 <= int(blockIdx.x)) lo = mid;
+        else hi = mid - 1;
+    }
+    const GatherJob& j = jobs[lo];
+    const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
+    if (i < j.nb) {
+        const int m = j.xmap[i];
+        T* xb = static_cast<T*>(j.xb[sizeof(T) == 8 ? 0 : 1]);
+        xb[i] = m >= 0 ? T(io.x[j.inst][m]) : T(0);
+    }
+}
+
 // Op kinds of the batched launch: (part, final, branches per tile).
 enum {
-    UK_DOWN1, UK_DOWN2, UK_DOWN4, UK_FULL1, UK_FULL2, UK_FULL4, UK_UP1, UK_UP2, UK_UP4, UK_FULL_FINAL, UK_UP_FINAL
+    UK_DOWN1, UK_DOWN2, UK_DOWN4, UK_FULL1, UK_FULL2, UK_FULL4, UK_UP1, UK_UP2, UK_UP4, UK_FULL_FINAL, UK_UP_FINAL,
+    UK_SEP, UK_SEP_LAST
 };
 int ud_kind(int part, bool fin, int nbr) {
     const int nb = nbr == 1 ? 0 : nbr == 2 ? 1 : 2;
+    if (part == UD_SEP) return fin ? UK_SEP_LAST : UK_SEP;
     return part == UD_DOWN ? UK_DOWN1 + nb
            : part == UD_FULL ? (fin ? int(UK_FULL_FINAL) : UK_FULL1 + nb)
                              : (fin ? int(UK_UP_FINAL) : UK_UP1 + nb);
 }
 int ud_kind_nbr(int kind) {
-    return kind == UK_FULL_FINAL || kind == UK_UP_FINAL ? 1 : 1 << ((kind - UK_DOWN1) % 3);
+    return kind >= UK_FULL_FINAL ? 1 : 1 << ((kind - UK_DOWN1) % 3);
 }
 
 // One op of one instance of a batch: its parameters (term scales as at batch
@@ -371,6 +414,10 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
             ud_tile<T, UD_FULL, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part);
             break;
         case UK_UP_FINAL: ud_tile<T, UD_UP, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
+        case UK_SEP: ud_tile<T, UD_SEP, false, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
+        case UK_SEP_LAST:
+            ud_tile<T, UD_SEP, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part);
+            break;
         default:
             if constexpr (MAXNBR == 4) {
                 if (o.kind == UK_DOWN4)
@@ -491,11 +538,12 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
         const int nbr = int(std::min<long long>(nbranch, sw.max_nbr));  // branches per tile (a power of two)
         kmx_sweep::Op op{part, fin, od.a, int(recs.size() / UD_WAVES), 0, nbr, od.terms};
         const int d = od.a.d;
-        const int nterm = fin ? T : 1;
+        const bool multi = fin || part == UD_SEP;  // one tile sums every term of the op
+        const int nterm = multi ? T : 1;
         const int batch = ud_split_batch(nbr) * std::max(1, wave_batches);
         std::vector<HostTile> ot;
         for (long long b = 0; b < nbranch; b += nbr)
-            for (int t = 0; t < (fin ? 1 : T); ++t)
+            for (int t = 0; t < (multi ? 1 : T); ++t)
                 for (int gi = sw.group0[d]; gi < sw.group0[d + 1]; ++gi) {
                     const Group& g = sw.groups[gi];
                     const int n = deg << g.m;
@@ -580,7 +628,8 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
         if (T < 1 || T > KMX_MAX_TERMS || int(op.terms.size()) != T) throw std::logic_error("sweep op terms");
         for (int t : op.terms)
             if (t < 0 || t >= sw.T) throw std::logic_error("sweep op terms");
-        const long long nbranch = op.final ? 1 : (op.part == UD_FULL ? half : (1LL << op.a.d));
+        const bool sep = op.part == UD_SEP;
+        const long long nbranch = (op.final || sep) ? 1 : (op.part == UD_FULL ? half : (1LL << op.a.d));
         if (op.nbr < 1 || op.nbr > 4 || nbranch % op.nbr || (op.final && op.nbr != 1))
             throw std::logic_error("sweep branches per tile");
         for (int wg = op.wg0; wg < op.wg1; ++wg)
@@ -613,7 +662,7 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
                     throw std::logic_error("sweep record outside its group");
                 if (r[R_TN] < 0 || (r[R_TN] > 0 && (r[R_TLO] < 0 || r[R_TLO] + r[R_TN] > T || r[R_SLO] < 0 ||
                                                     r[R_SLO] >= sper || r[R_SHI] < 1 || r[R_SHI] > sper ||
-                                                    (!op.final && (r[R_TN] != 1 || r[R_TLO] != r[R_TOUT])))))
+                                                    (!op.final && !sep && (r[R_TN] != 1 || r[R_TLO] != r[R_TOUT])))))
                     throw std::logic_error("sweep record steps");
                 for (int i = 0; i < 16; ++i) {
                     const int row = r[R_M0] + i;
@@ -625,6 +674,12 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
                 }
                 for (int l = 0; l <= g.m; ++l)
                     if (r[UD_BOX + l] != g.box[l]) throw std::logic_error("sweep record boxes");
+                if (sep) {  // one box copy of x in, the running sum in (after the first op) and out
+                    if (op.a.x0 || !slot_ok(op.a.in_base) || (op.a.add_y && !slot_ok(op.a.add_base)) ||
+                        (!op.final && !slot_ok(op.a.out_base)))
+                        throw std::logic_error("sweep separable slots outside the arena");
+                    continue;
+                }
                 for (int bj = r[R_B]; bj < r[R_B] + op.nbr; ++bj) {  // every branch of the tile
                     for (int t = r[R_TLO]; t < r[R_TLO] + r[R_TN]; ++t)
                         if (!(op.a.x0 && bj == 0) && !slot_ok(op.a.in_base + ((long long)bj * T + t) * nb))
@@ -807,6 +862,8 @@ void kmx_sweep_build(kmx_plan* p) {
             sw->separable = true;
             sw->identity = ident;
             bool first = true;
+            int last_d = 0;
+            for (int t = 0; t < T; ++t) last_d = std::max(last_d, active[t]);
             for (int d = 0; d < D; ++d) {
                 std::vector<int> ts;
                 for (int t = 0; t < T; ++t)
@@ -818,10 +875,13 @@ void kmx_sweep_build(kmx_plan* p) {
                     a.a_off[i] = i < a.T ? p->off[ts[size_t(i)] * D + d] : 0;
                     a.scale[i] = 1.0;
                 }
-                a.x0 = 1;
+                a.x0 = 0;
+                a.in_base = 0;   // slot 0: the box copy of x
+                a.add_base = nb;  // slot 1: the sum over the dimensions so far
+                a.out_base = nb;
                 a.add_y = first ? 0 : 1;
                 first = false;
-                add_op(UD_FULL, true, a, 1, ts);
+                add_op(UD_SEP, d == last_d, a, 1, ts);
                 flops += fl_full[d] * a.T;
             }
         } else if (D == 1) {
@@ -860,7 +920,7 @@ void kmx_sweep_build(kmx_plan* p) {
                 src = dst;
             }
         }
-        if (sw->separable) sw->arena = nb;  // x in, y out: no intermediates
+        if (sw->separable) sw->arena = 2 * nb;  // the box copy of x and the running sum
         sw->flops = flops;
         sw->deg = deg;
         sw->T = T;
@@ -937,14 +997,19 @@ int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream
     kmx_blockset* bs = p->bs;
     auto run = [&](auto* arena, const auto* mat) {
         using T = std::remove_pointer_t<decltype(arena)>;
+        if (sw->separable) {
+            const unsigned g = unsigned(std::min<long long>((sw->nb + 255) / 256, 8192));
+            hipLaunchKernelGGL(k_box_gather<T>, dim3(g), dim3(256), 0, s, x, sw->d_xmap, arena, sw->nb);
+        }
         for (const auto& op : sw->ops) {
             UdOp a = op.a;
-            if (op.final)
+            if (op.final || op.part == UD_SEP)  // the ops that scale their terms
                 for (int i = 0; i < a.T; ++i) a.scale[i] = p->scale[op.terms[size_t(i)]];
             const dim3 grid(unsigned(op.wg1 - op.wg0)), block(UD_WAVES * 64);
             const int2* rc = reinterpret_cast<const int2*>(sw->d_recs) + size_t(op.wg0) * UD_WAVES * (UD_REC / 2);
             auto pick = [&](auto nbr_c) {
                 constexpr int NBR = decltype(nbr_c)::value;
+                if (op.part == UD_SEP) return op.final ? k_ud_op<T, UD_SEP, true, 1> : k_ud_op<T, UD_SEP, false, 1>;
                 return op.part == UD_DOWN ? k_ud_op<T, UD_DOWN, false, NBR>
                        : op.part == UD_FULL ? (op.final ? k_ud_op<T, UD_FULL, true, 1> : k_ud_op<T, UD_FULL, false, NBR>)
                                             : (op.final ? k_ud_op<T, UD_UP, true, 1> : k_ud_op<T, UD_UP, false, NBR>);
@@ -994,6 +1059,8 @@ struct kmx_sweep_batch {
     };
     std::vector<Phase> phases;
     std::vector<const kmx_plan*> members;
+    GatherJob* d_jobs = nullptr;  // box copies of x of the separable members (one launch first)
+    int njobs = 0, gather_wgs = 0;
     UdBatchOp* d_ops = nullptr;
     int* d_wgop = nullptr;  // op of every workgroup, phases in order
     int* d_recs = nullptr;  // UD_WAVES work records per workgroup, copied from the plans
@@ -1033,7 +1100,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members) {
                 const kmx_sweep::Op& op = mops[q][size_t(k)];
                 UdBatchOp o{};
                 o.a = op.a;
-                if (op.final)
+                if (op.final || op.part == UD_SEP)
                     for (int i = 0; i < o.a.T; ++i) o.a.scale[i] = p->scale[op.terms[size_t(i)]];
                 o.mat[0] = p->bs->d_mat64;
                 o.mat[1] = p->bs->d_mat32;
@@ -1064,6 +1131,17 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members) {
             sb->phases.push_back(P);
         }
         if (flat.size() != wgop.size() * UD_WAVES * UD_REC) throw std::logic_error("sweep batch records");
+        std::vector<GatherJob> jobs;
+        long long blk = 0;
+        for (size_t q = 0; q < members.size(); ++q) {
+            const kmx_sweep* sw = members[q]->sw;
+            if (!sw->separable) continue;
+            jobs.push_back(GatherJob{sw->d_xmap, {sw->d_arena[0], sw->d_arena[1]}, sw->nb, int(q), int(blk)});
+            blk += (sw->nb + 255) / 256;
+            if (blk >= (1LL << 31)) throw std::runtime_error("sweep batch too large");
+        }
+        sb->njobs = int(jobs.size());
+        sb->gather_wgs = int(blk);
         auto up = [](const auto* h, size_t n, auto*& d) {
             if (hipMalloc(&d, std::max<size_t>(1, n) * sizeof(*h)) != hipSuccess) throw std::bad_alloc();
             if (n && hipMemcpy(d, h, n * sizeof(*h), hipMemcpyHostToDevice) != hipSuccess)
@@ -1072,6 +1150,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members) {
         up(ops.data(), ops.size(), sb->d_ops);
         up(wgop.data(), wgop.size(), sb->d_wgop);
         up(flat.data(), flat.size(), sb->d_recs);
+        up(jobs.data(), jobs.size(), sb->d_jobs);
     } catch (...) {
         kmx_sweep_batch_free(sb);
         throw;
@@ -1089,6 +1168,14 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
     if (slot == 1 && !sb->has32) {
         set_error("kmx_batch_apply: a sweep member has no FP32 blocks");
         return KMX_E_ARG;
+    }
+    if (sb->njobs > 0) {
+        if (slot == 0)
+            hipLaunchKernelGGL(k_box_gather_batch<double>, dim3(sb->gather_wgs), dim3(256), 0, s, sb->d_jobs, sb->njobs,
+                               io);
+        else
+            hipLaunchKernelGGL(k_box_gather_batch<float>, dim3(sb->gather_wgs), dim3(256), 0, s, sb->d_jobs, sb->njobs,
+                               io);
     }
     for (const auto& P : sb->phases) {
         if (P.nwg == 0) continue;
@@ -1116,6 +1203,7 @@ void kmx_sweep_batch_free(kmx_sweep_batch* sb) {
     if (sb->d_ops) hipFree(sb->d_ops);
     if (sb->d_wgop) hipFree(sb->d_wgop);
     if (sb->d_recs) hipFree(sb->d_recs);
+    if (sb->d_jobs) hipFree(sb->d_jobs);
     delete sb;
 }
 
