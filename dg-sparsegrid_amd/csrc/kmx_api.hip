@@ -363,7 +363,7 @@ int kmx_plan_set_term_scale(kmx_plan* p, int term, double alpha) {
         return KMX_E_ARG;
     }
     p->scale[term] = alpha;
-    return KMX_OK;
+    return kmx_sweep_scales_updated(p);
 }
 
 int kmx_plan_destroy(kmx_plan* p) {

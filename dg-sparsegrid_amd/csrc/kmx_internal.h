@@ -195,13 +195,19 @@ void kmx_sweep_free(kmx_plan* p);
 double kmx_sweep_flops(const kmx_plan* p);
 // after kmx_plan_update_term(term, dim) (the FP64 block copy already updated)
 void kmx_sweep_term_updated(kmx_plan* p, int term, int dim);
+// after kmx_plan_set_term_scale: the plan's own launch tables take the new scales
+int kmx_sweep_scales_updated(kmx_plan* p);
 // The sweeps of several plans merged into one launch per phase (kmx_sweep.hip):
-// phase k runs op k - (phases - ops of the plan) of every plan, so each plan's ops
-// keep their order and every plan's last op falls in the last phase.  members:
+// a plan's ops form launch groups (one op each for a general sweep, in order; all
+// ops together for a separable one), phase k runs group k - (phases - groups of
+// the plan) of every plan, so each plan's groups keep their order and every
+// plan's last group falls in the last phase; separable plans add one launch
+// before (box copies of x) and one after (their slots summed into y).  members:
 // distinct plans, each with a sweep path; io slot q is members[q].  Term scales
-// are taken at build time.  Build throws std::exception on failure.
+// are taken at build time; wave_batches <= 0: the batch default (four load
+// batches per wave).  Build throws std::exception on failure.
 struct kmx_sweep_batch;
-kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members);
+kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, int wave_batches = 0);
 int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const kmx::BatchIO& io, hipStream_t s);
 void kmx_sweep_batch_free(kmx_sweep_batch* sb);
 int kmx_sweep_batch_phases(const kmx_sweep_batch* sb);

@@ -40,6 +40,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <set>
 #include <stdexcept>
@@ -72,8 +73,8 @@ struct SwMfma<float> {
 };
 
 // UD_SEP: one dimension's op of a separable sweep: the whole operators of its terms,
-// summed, from the box copy of x (slot in_base), plus the previous dimensions' sum
-// (slot add_base, when add_y), into slot out_base or, the last op, into y.
+// summed, from the box copy of x (slot in_base) into the dimension's own slot
+// (out_base); the slots are summed into y afterwards.
 enum { UD_DOWN = 0, UD_FULL = 1, UD_UP = 2, UD_SEP = 3 };
 
 // Work record of one wave (UD_REC ints, stored so that lane j reads ints j and
@@ -103,7 +104,6 @@ constexpr int UD_WAVES = 8;                     // waves per workgroup
 struct UdOp {
     int d, deg, lda, T;
     int part, final, x0, out_bit, add_bit;
-    int add_y;  // UD_SEP: add slot add_base (every op of a separable sweep after the first)
     long long nb;
     long long in_base, out_base, add_base;
     long long a_off[KMX_MAX_TERMS];  // column-major masked matrix of (term, d)
@@ -185,9 +185,6 @@ __device__ __forceinline__ void ud_tile(const T* __restrict__ mat, T* __restrict
                         for (int q = 0; q < 4; ++q) addv[j][q] += sc * add[oaddr[q]];
                     }
             }
-            if (PART == UD_SEP && op.add_y)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) addv[0][q] = arena[op.add_base + oaddr[q]];
             if (FINAL)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) yi[q] = ymap[oaddr[q]];
@@ -322,49 +319,58 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ m
     ud_tile<T, PART, FINAL, NBR>(mat, arena, rv, w, lane, op, x, xmap, y, ymap, part);
 }
 
-// Box copy of x for a separable sweep: xb[i] = x[xmap[i]], 0 where the cell is absent.
-template <typename T>
-__global__ void __launch_bounds__(256) k_box_gather(const double* __restrict__ x, const int* __restrict__ xmap,
-                                                    T* __restrict__ xb, long long nb) {
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nb; i += (long long)gridDim.x * 256) {
-        const int m = xmap[i];
-        xb[i] = m >= 0 ? T(x[m]) : T(0);
-    }
-}
-
-// The box copies of x of every separable instance of a batch in one launch:
-// job j covers workgroups [blk0_j, blk0_{j+1}), 256 positions each.
-struct GatherJob {
+// Box copies of x and slot sums of every separable instance of a batch, one
+// launch each: job j covers workgroups [blk0_j, blk0_{j+1}), 256 positions each.
+struct BoxJob {
     const int* xmap;
-    void* xb[2];  // FP64, FP32 arena of the plan (slot 0)
+    const int* ymap;
+    void* arena[2];  // FP64, FP32 arena of the plan: slot 0 x's box copy, slots 1..nslot the ops' sums
     long long nb;
-    int inst, blk0;
+    int nslot, inst, blk0;
 };
-template <typename T>
-__global__ void __launch_bounds__(256) k_box_gather_batch(const GatherJob* __restrict__ jobs, int njobs, BatchIO io) {
+__device__ __forceinline__ const BoxJob& box_job(const BoxJob* __restrict__ jobs, int njobs) {
     int lo = 0, hi = njobs - 1;  // the last job starting at or before this workgroup
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (jobs[mid].blk0 <= int(blockIdx.x)) lo = mid;
         else hi = mid - 1;
     }
-    const GatherJob& j = jobs[lo];
+    return jobs[lo];
+}
+template <typename T>
+__global__ void __launch_bounds__(256) k_box_gather_batch(const BoxJob* __restrict__ jobs, int njobs, BatchIO io) {
+    const BoxJob& j = box_job(jobs, njobs);
     const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
     if (i < j.nb) {
         const int m = j.xmap[i];
-        T* xb = static_cast<T*>(j.xb[sizeof(T) == 8 ? 0 : 1]);
+        T* xb = static_cast<T*>(j.arena[sizeof(T) == 8 ? 0 : 1]);
         xb[i] = m >= 0 ? T(io.x[j.inst][m]) : T(0);
+    }
+}
+// y = the ops' slots summed (in op order), at the present cells
+template <typename T>
+__global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict__ jobs, int njobs, BatchIO io) {
+    const BoxJob& j = box_job(jobs, njobs);
+    const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
+    if (i < j.nb) {
+        const int m = j.ymap[i];
+        if (m >= 0) {
+            const T* sl = static_cast<const T*>(j.arena[sizeof(T) == 8 ? 0 : 1]) + j.nb + i;
+            double v = 0.0;
+            for (int k = 0; k < j.nslot; ++k) v += double(sl[(long long)k * j.nb]);
+            io.y[j.inst][m] = v;
+        }
     }
 }
 
 // Op kinds of the batched launch: (part, final, branches per tile).
 enum {
     UK_DOWN1, UK_DOWN2, UK_DOWN4, UK_FULL1, UK_FULL2, UK_FULL4, UK_UP1, UK_UP2, UK_UP4, UK_FULL_FINAL, UK_UP_FINAL,
-    UK_SEP, UK_SEP_LAST
+    UK_SEP
 };
 int ud_kind(int part, bool fin, int nbr) {
     const int nb = nbr == 1 ? 0 : nbr == 2 ? 1 : 2;
-    if (part == UD_SEP) return fin ? UK_SEP_LAST : UK_SEP;
+    if (part == UD_SEP) return UK_SEP;
     return part == UD_DOWN ? UK_DOWN1 + nb
            : part == UD_FULL ? (fin ? int(UK_FULL_FINAL) : UK_FULL1 + nb)
                              : (fin ? int(UK_UP_FINAL) : UK_UP1 + nb);
@@ -415,9 +421,6 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
             break;
         case UK_UP_FINAL: ud_tile<T, UD_UP, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
         case UK_SEP: ud_tile<T, UD_SEP, false, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
-        case UK_SEP_LAST:
-            ud_tile<T, UD_SEP, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part);
-            break;
         default:
             if constexpr (MAXNBR == 4) {
                 if (o.kind == UK_DOWN4)
@@ -506,6 +509,7 @@ struct kmx_sweep {
     bool separable = false;
     std::vector<char> identity;
     bool stale = false;
+    kmx_sweep_batch* self = nullptr;  // separable: the plan's own launches (a batch of one)
     std::vector<Group> groups;
     std::vector<int> group0;  // groups of op dimension d: [group0[d], group0[d+1])
     int deg = 0, T = 0, max_nbr = 2;
@@ -675,8 +679,7 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
                 for (int l = 0; l <= g.m; ++l)
                     if (r[UD_BOX + l] != g.box[l]) throw std::logic_error("sweep record boxes");
                 if (sep) {  // one box copy of x in, the running sum in (after the first op) and out
-                    if (op.a.x0 || !slot_ok(op.a.in_base) || (op.a.add_y && !slot_ok(op.a.add_base)) ||
-                        (!op.final && !slot_ok(op.a.out_base)))
+                    if (op.a.x0 || op.final || !slot_ok(op.a.in_base) || !slot_ok(op.a.out_base))
                         throw std::logic_error("sweep separable slots outside the arena");
                     continue;
                 }
@@ -861,9 +864,7 @@ void kmx_sweep_build(kmx_plan* p) {
         if (separable) {
             sw->separable = true;
             sw->identity = ident;
-            bool first = true;
-            int last_d = 0;
-            for (int t = 0; t < T; ++t) last_d = std::max(last_d, active[t]);
+            int nsep = 0;
             for (int d = 0; d < D; ++d) {
                 std::vector<int> ts;
                 for (int t = 0; t < T; ++t)
@@ -876,12 +877,10 @@ void kmx_sweep_build(kmx_plan* p) {
                     a.scale[i] = 1.0;
                 }
                 a.x0 = 0;
-                a.in_base = 0;   // slot 0: the box copy of x
-                a.add_base = nb;  // slot 1: the sum over the dimensions so far
-                a.out_base = nb;
-                a.add_y = first ? 0 : 1;
-                first = false;
-                add_op(UD_SEP, d == last_d, a, 1, ts);
+                a.in_base = 0;                         // slot 0: the box copy of x
+                a.out_base = (1 + nsep) * (long long)nb;  // slot 1 + k: op k's sum
+                ++nsep;
+                add_op(UD_SEP, false, a, 1, ts);
                 flops += fl_full[d] * a.T;
             }
         } else if (D == 1) {
@@ -920,7 +919,7 @@ void kmx_sweep_build(kmx_plan* p) {
                 src = dst;
             }
         }
-        if (sw->separable) sw->arena = 2 * nb;  // the box copy of x and the running sum
+        if (sw->separable) sw->arena = (1 + (long long)sw->desc.size()) * nb;  // x's box copy, one slot per op
         sw->flops = flops;
         sw->deg = deg;
         sw->T = T;
@@ -981,6 +980,7 @@ void kmx_sweep_build(kmx_plan* p) {
         if (bs->d_mat32 && hipMalloc(&sw->d_arena[1], size_t(sw->arena) * sizeof(float)) != hipSuccess)
             throw std::bad_alloc();
         p->sw = sw;
+        if (sw->separable) sw->self = kmx_sweep_batch_build({p}, sweep_wave_batches(1));
     } catch (const std::exception&) {
         p->sw = sw;
         kmx_sweep_free(p);
@@ -989,6 +989,13 @@ void kmx_sweep_build(kmx_plan* p) {
 
 int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s) {
     kmx_sweep* sw = p->sw;
+    if (sw->self) {  // separable: x's box copy, every op in one launch, the slots summed into y
+        BatchIO io;
+        std::memset(&io, 0, sizeof(io));
+        io.x[0] = x;
+        io.y[0] = y;
+        return kmx_sweep_batch_apply(sw->self, slot, io, s);
+    }
     if (sw->stale) {
         set_error("kmx_apply: a 1D factor the dimension sweep takes as the identity was replaced "
                   "(kmx_plan_update_term); switch the sweep off (kmx_plan_set_sweep) or recreate the plan");
@@ -997,10 +1004,6 @@ int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream
     kmx_blockset* bs = p->bs;
     auto run = [&](auto* arena, const auto* mat) {
         using T = std::remove_pointer_t<decltype(arena)>;
-        if (sw->separable) {
-            const unsigned g = unsigned(std::min<long long>((sw->nb + 255) / 256, 8192));
-            hipLaunchKernelGGL(k_box_gather<T>, dim3(g), dim3(256), 0, s, x, sw->d_xmap, arena, sw->nb);
-        }
         for (const auto& op : sw->ops) {
             UdOp a = op.a;
             if (op.final || op.part == UD_SEP)  // the ops that scale their terms
@@ -1040,6 +1043,7 @@ int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream
 void kmx_sweep_free(kmx_plan* p) {
     kmx_sweep* sw = p->sw;
     if (!sw) return;
+    kmx_sweep_batch_free(sw->self);
     if (sw->d_xmap) hipFree(sw->d_xmap);
     if (sw->d_ymap) hipFree(sw->d_ymap);
     if (sw->d_recs) hipFree(sw->d_recs);
@@ -1059,30 +1063,36 @@ struct kmx_sweep_batch {
     };
     std::vector<Phase> phases;
     std::vector<const kmx_plan*> members;
-    GatherJob* d_jobs = nullptr;  // box copies of x of the separable members (one launch first)
-    int njobs = 0, gather_wgs = 0;
+    BoxJob* d_jobs = nullptr;  // separable members: box copy of x first, slot sums into y last
+    int njobs = 0, box_wgs = 0;
     UdBatchOp* d_ops = nullptr;
+    std::vector<UdBatchOp> h_ops;            // host copy (term scales refreshed from it)
+    std::vector<std::vector<int>> op_terms;  // plan term of each term of op i
     int* d_wgop = nullptr;  // op of every workgroup, phases in order
-    int* d_recs = nullptr;  // UD_WAVES work records per workgroup, copied from the plans
+    int* d_recs = nullptr;  // UD_WAVES work records per workgroup
     bool has32 = true;
 };
 
-kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members) {
+kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, int wave_batches) {
     if (members.empty() || members.size() > KMX_BATCH_MAX) throw std::invalid_argument("sweep batch size");
-    int nph = 0;
+    if (wave_batches <= 0) wave_batches = sweep_wave_batches(4);
     for (size_t q = 0; q < members.size(); ++q) {
         const kmx_plan* p = members[q];
         if (!p || !p->sw || p->sw->desc.empty()) throw std::invalid_argument("sweep batch member without a sweep");
         for (size_t r = 0; r < q; ++r)
             if (members[r] == p) throw std::invalid_argument("sweep batch member twice (one arena per plan)");
-        nph = std::max(nph, int(p->sw->desc.size()));
     }
-    // every member's records again, for throughput: fewer waves per tile
+    // every member's records for this batch's split; a member's ops form launch
+    // groups: one op each (general sweeps, in order) or all ops together (separable)
     std::vector<std::vector<std::vector<int>>> recs(members.size());
     std::vector<std::vector<kmx_sweep::Op>> mops(members.size());
+    std::vector<int> ngroups(members.size());
+    int nph = 0;
     for (size_t q = 0; q < members.size(); ++q) {
-        emit_records(*members[q]->sw, sweep_wave_batches(4), recs[q], mops[q]);
+        emit_records(*members[q]->sw, wave_batches, recs[q], mops[q]);
         check_records(*members[q]->sw, recs[q], mops[q]);
+        ngroups[q] = members[q]->sw->separable ? 1 : int(mops[q].size());
+        nph = std::max(nph, ngroups[q]);
     }
     auto* sb = new kmx_sweep_batch();
     sb->members.assign(members.begin(), members.end());
@@ -1090,64 +1100,70 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members) {
         std::vector<UdBatchOp> ops;
         std::vector<int> wgop;
         std::vector<int> flat;  // records interleaved so that lane j reads ints j and 64 + j as one int2
-        for (int ph = 0; ph < nph; ++ph) {
+        for (int ph = 0; ph < nph; ++ph) {  // groups end-aligned: every member's last group in the last phase
             kmx_sweep_batch::Phase P{int(wgop.size()), 0, 1};
             for (size_t q = 0; q < members.size(); ++q) {
                 kmx_plan* p = members[q];
                 const kmx_sweep* sw = p->sw;
-                const int k = ph - (nph - int(mops[q].size()));
-                if (k < 0) continue;
-                const kmx_sweep::Op& op = mops[q][size_t(k)];
-                UdBatchOp o{};
-                o.a = op.a;
-                if (op.final || op.part == UD_SEP)
-                    for (int i = 0; i < o.a.T; ++i) o.a.scale[i] = p->scale[op.terms[size_t(i)]];
-                o.mat[0] = p->bs->d_mat64;
-                o.mat[1] = p->bs->d_mat32;
-                o.arena[0] = sw->d_arena[0];
-                o.arena[1] = sw->d_arena[1];
-                if (!sw->d_arena[1] || !p->bs->d_mat32) sb->has32 = false;
-                o.xmap = sw->d_xmap;
-                o.ymap = sw->d_ymap;
-                o.inst = int(q);
-                o.kind = ud_kind(op.part, op.final, op.nbr);
-                if (ud_kind_nbr(o.kind) != op.nbr || op.wg1 < op.wg0) throw std::logic_error("sweep batch op");
-                P.maxnbr = std::max(P.maxnbr, op.nbr);
-                const int oi = int(ops.size());
-                ops.push_back(o);
-                for (int wg = op.wg0; wg < op.wg1; ++wg) {
-                    wgop.push_back(oi);
-                    for (int w = 0; w < UD_WAVES; ++w) {
-                        const std::vector<int>& r = recs[q][size_t(wg) * UD_WAVES + w];
-                        for (int j = 0; j < UD_REC / 2; ++j) {
-                            flat.push_back(r[j]);
-                            flat.push_back(r[UD_REC / 2 + j]);
+                const int gi = ph - (nph - ngroups[q]);
+                if (gi < 0) continue;
+                const size_t k0 = sw->separable ? 0 : size_t(gi), k1 = sw->separable ? mops[q].size() : k0 + 1;
+                for (size_t k = k0; k < k1; ++k) {
+                    const kmx_sweep::Op& op = mops[q][k];
+                    UdBatchOp o{};
+                    o.a = op.a;
+                    if (op.final || op.part == UD_SEP)  // the ops that scale their terms
+                        for (int i = 0; i < o.a.T; ++i) o.a.scale[i] = p->scale[op.terms[size_t(i)]];
+                    o.mat[0] = p->bs->d_mat64;
+                    o.mat[1] = p->bs->d_mat32;
+                    o.arena[0] = sw->d_arena[0];
+                    o.arena[1] = sw->d_arena[1];
+                    if (!sw->d_arena[1] || !p->bs->d_mat32) sb->has32 = false;
+                    o.xmap = sw->d_xmap;
+                    o.ymap = sw->d_ymap;
+                    o.inst = int(q);
+                    o.kind = ud_kind(op.part, op.final, op.nbr);
+                    if (ud_kind_nbr(o.kind) != op.nbr || op.wg1 < op.wg0) throw std::logic_error("sweep batch op");
+                    P.maxnbr = std::max(P.maxnbr, op.nbr);
+                    const int oi = int(ops.size());
+                    ops.push_back(o);
+                    sb->op_terms.push_back(op.terms);
+                    for (int wg = op.wg0; wg < op.wg1; ++wg) {
+                        wgop.push_back(oi);
+                        for (int w = 0; w < UD_WAVES; ++w) {
+                            const std::vector<int>& r = recs[q][size_t(wg) * UD_WAVES + w];
+                            for (int j = 0; j < UD_REC / 2; ++j) {
+                                flat.push_back(r[j]);
+                                flat.push_back(r[UD_REC / 2 + j]);
+                            }
                         }
                     }
+                    P.nwg += op.wg1 - op.wg0;
+                    if (wgop.size() >= size_t(1) << 30) throw std::runtime_error("sweep batch too large");
                 }
-                P.nwg += op.wg1 - op.wg0;
-                if (wgop.size() >= size_t(1) << 30) throw std::runtime_error("sweep batch too large");
             }
             sb->phases.push_back(P);
         }
         if (flat.size() != wgop.size() * UD_WAVES * UD_REC) throw std::logic_error("sweep batch records");
-        std::vector<GatherJob> jobs;
+        std::vector<BoxJob> jobs;
         long long blk = 0;
         for (size_t q = 0; q < members.size(); ++q) {
             const kmx_sweep* sw = members[q]->sw;
             if (!sw->separable) continue;
-            jobs.push_back(GatherJob{sw->d_xmap, {sw->d_arena[0], sw->d_arena[1]}, sw->nb, int(q), int(blk)});
+            jobs.push_back(BoxJob{sw->d_xmap, sw->d_ymap, {sw->d_arena[0], sw->d_arena[1]}, sw->nb,
+                                  int(sw->desc.size()), int(q), int(blk)});
             blk += (sw->nb + 255) / 256;
             if (blk >= (1LL << 31)) throw std::runtime_error("sweep batch too large");
         }
         sb->njobs = int(jobs.size());
-        sb->gather_wgs = int(blk);
+        sb->box_wgs = int(blk);
         auto up = [](const auto* h, size_t n, auto*& d) {
             if (hipMalloc(&d, std::max<size_t>(1, n) * sizeof(*h)) != hipSuccess) throw std::bad_alloc();
             if (n && hipMemcpy(d, h, n * sizeof(*h), hipMemcpyHostToDevice) != hipSuccess)
                 throw std::runtime_error("hipMemcpy failed");
         };
         up(ops.data(), ops.size(), sb->d_ops);
+        sb->h_ops = ops;
         up(wgop.data(), wgop.size(), sb->d_wgop);
         up(flat.data(), flat.size(), sb->d_recs);
         up(jobs.data(), jobs.size(), sb->d_jobs);
@@ -1161,21 +1177,18 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members) {
 int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io, hipStream_t s) {
     for (const kmx_plan* p : sb->members)
         if (p->sw && p->sw->stale) {
-            set_error("kmx_batch_apply: a 1D factor a member's dimension sweep takes as the identity was "
-                      "replaced (kmx_plan_update_term); switch the batch sweep off and on again");
+            set_error("kmx_apply: a 1D factor the dimension sweep takes as the identity was replaced "
+                      "(kmx_plan_update_term); switch the sweep off and on again, or recreate the plan");
             return KMX_E_UNSUPPORTED;
         }
     if (slot == 1 && !sb->has32) {
         set_error("kmx_batch_apply: a sweep member has no FP32 blocks");
         return KMX_E_ARG;
     }
+    const dim3 bg(unsigned(sb->box_wgs)), bb(256);
     if (sb->njobs > 0) {
-        if (slot == 0)
-            hipLaunchKernelGGL(k_box_gather_batch<double>, dim3(sb->gather_wgs), dim3(256), 0, s, sb->d_jobs, sb->njobs,
-                               io);
-        else
-            hipLaunchKernelGGL(k_box_gather_batch<float>, dim3(sb->gather_wgs), dim3(256), 0, s, sb->d_jobs, sb->njobs,
-                               io);
+        if (slot == 0) hipLaunchKernelGGL(k_box_gather_batch<double>, bg, bb, 0, s, sb->d_jobs, sb->njobs, io);
+        else hipLaunchKernelGGL(k_box_gather_batch<float>, bg, bb, 0, s, sb->d_jobs, sb->njobs, io);
     }
     for (const auto& P : sb->phases) {
         if (P.nwg == 0) continue;
@@ -1189,6 +1202,10 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
             if (P.maxnbr <= 2) hipLaunchKernelGGL((k_ud_batch<float, 2>), grid, block, 0, s, sb->d_ops, wgop, rc, io);
             else hipLaunchKernelGGL((k_ud_batch<float, 4>), grid, block, 0, s, sb->d_ops, wgop, rc, io);
         }
+    }
+    if (sb->njobs > 0) {
+        if (slot == 0) hipLaunchKernelGGL(k_box_sum_batch<double>, bg, bb, 0, s, sb->d_jobs, sb->njobs, io);
+        else hipLaunchKernelGGL(k_box_sum_batch<float>, bg, bb, 0, s, sb->d_jobs, sb->njobs, io);
     }
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
@@ -1213,4 +1230,23 @@ void kmx_sweep_term_updated(kmx_plan* p, int term, int dim) {
     kmx_sweep* sw = p->sw;
     if (!sw || !sw->separable) return;
     if (sw->identity[size_t(term) * p->D + dim] && !identity_factor(p, term, dim)) sw->stale = true;
+}
+
+int kmx_sweep_scales_updated(kmx_plan* p) {
+    kmx_sweep* sw = p->sw;
+    if (!sw || !sw->self) return KMX_OK;
+    kmx_sweep_batch* sb = sw->self;
+    for (size_t i = 0; i < sb->h_ops.size(); ++i) {
+        UdBatchOp& o = sb->h_ops[i];
+        if (o.kind == UK_SEP)
+            for (int k = 0; k < o.a.T; ++k) o.a.scale[k] = p->scale[sb->op_terms[i][size_t(k)]];
+    }
+    // launches already queued may still read the old table
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(sb->d_ops, sb->h_ops.data(), sb->h_ops.size() * sizeof(UdBatchOp), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+        set_error("kmx_plan_set_term_scale: refreshing the dimension sweep's scales failed");
+        return KMX_E_HIP;
+    }
+    return KMX_OK;
 }

@@ -85,7 +85,9 @@ int kmx_plan_create(const kmx_plan_desc* desc, kmx_plan** plan);
 int kmx_plan_update_term(kmx_plan* plan, int term, int dim, const double* mat, void* stream);
 
 /* Scale term t's contribution by alpha (default 1).  Used for per-term
- * coefficient scalings (batched instances, sensitivities). */
+ * coefficient scalings (batched instances, sensitivities).  On a plan whose
+ * dimension sweep has the separable form it waits for the device and refreshes
+ * the sweep's launch table; not under graph capture. */
 int kmx_plan_set_term_scale(kmx_plan* plan, int term, double alpha);
 
 int kmx_plan_destroy(kmx_plan* plan);
