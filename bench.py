@@ -55,9 +55,10 @@ MEASURED_TRAFFIC = {
 
 
 # HBM bytes of one FP32 merged-sweep batched apply of the 64-instance workload
-# (its five phase launches together), from the same FETCH_SIZE / WRITE_SIZE
-# counter runs and scale factors.  None until measured.
-MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = None
+# (its three launches together: box copies of x, every dimension's op, slot
+# sums into y), from the same FETCH_SIZE / WRITE_SIZE counter runs and scale
+# factors.  Source: profiles/batch_sweep_hbm_traffic.md.
+MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = 692120352
 
 
 def build_operator(seed=2):
