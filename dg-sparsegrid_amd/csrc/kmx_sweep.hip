@@ -319,6 +319,25 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ m
     ud_tile<T, PART, FINAL, NBR>(mat, arena, rv, w, lane, op, x, xmap, y, ymap, part);
 }
 
+// The combined 1D operator of one dimension of a separable sweep: out = sum over
+// the dimension's terms of scale * masked matrix (FP64, column-major n x n, terms
+// in order), and its FP32 copy.
+struct CombineArgs {
+    long long off[KMX_MAX_TERMS];
+    double scale[KMX_MAX_TERMS];
+    int nt;
+    long long n2;
+};
+__global__ void __launch_bounds__(256) k_combine(const double* __restrict__ mat, CombineArgs a,
+                                                 double* __restrict__ out64, float* __restrict__ out32) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < a.n2; i += (long long)gridDim.x * 256) {
+        double v = 0.0;
+        for (int k = 0; k < a.nt; ++k) v += a.scale[k] * mat[a.off[k] + i];
+        out64[i] = v;
+        if (out32) out32[i] = float(v);
+    }
+}
+
 // Box copies of x and slot sums of every separable instance of a batch, one
 // launch each: job j covers workgroups [blk0_j, blk0_{j+1}), 256 positions each.
 struct BoxJob {
@@ -510,6 +529,10 @@ struct kmx_sweep {
     std::vector<char> identity;
     bool stale = false;
     kmx_sweep_batch* self = nullptr;  // separable: the plan's own launches (a batch of one)
+    // separable: op k's terms folded into one matrix (scales included) at comb_off[k]
+    double* d_comb64 = nullptr;
+    float* d_comb32 = nullptr;
+    std::vector<long long> comb_off;
     std::vector<Group> groups;
     std::vector<int> group0;  // groups of op dimension d: [group0[d], group0[d+1])
     int deg = 0, T = 0, max_nbr = 2;
@@ -629,7 +652,10 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
     auto slot_ok = [&](long long off) { return off >= 0 && off + nb <= sw.arena; };
     for (const auto& op : ops) {
         const int T = op.a.T;
-        if (T < 1 || T > KMX_MAX_TERMS || int(op.terms.size()) != T) throw std::logic_error("sweep op terms");
+        // (a separable op's terms are folded into its one combined matrix)
+        if (T < 1 || T > KMX_MAX_TERMS || op.terms.empty() || (op.part != UD_SEP && int(op.terms.size()) != T) ||
+            (op.part == UD_SEP && T != 1))
+            throw std::logic_error("sweep op terms");
         for (int t : op.terms)
             if (t < 0 || t >= sw.T) throw std::logic_error("sweep op terms");
         const bool sep = op.part == UD_SEP;
@@ -717,6 +743,31 @@ bool identity_factor(const kmx_plan* p, int t, int k) {
                     if (std::fabs(b[a * deg + c] - (I == J && a == c ? 1.0 : 0.0)) > tol) return false;
         }
     return true;
+}
+
+// (Re)computes the combined matrices of a separable sweep (every op, or op `only`)
+// from the plan's masked matrices and term scales.  Waits for the device first
+// (queued applies may read them) and after.
+int combine_matrices(kmx_plan* p, int only = -1) {
+    kmx_sweep* sw = p->sw;
+    if (hipDeviceSynchronize() != hipSuccess) return KMX_E_HIP;
+    for (size_t k = 0; k < sw->desc.size(); ++k) {
+        if (only >= 0 && int(k) != only) continue;
+        const kmx_sweep::OpDesc& od = sw->desc[k];
+        CombineArgs a{};
+        a.nt = int(od.terms.size());
+        const int d = od.a.d;
+        for (int i = 0; i < a.nt; ++i) {
+            a.off[i] = p->off[od.terms[size_t(i)] * p->D + d];
+            a.scale[i] = p->scale[od.terms[size_t(i)]];
+        }
+        a.n2 = (long long)p->n[d] * p->n[d];
+        const unsigned g = unsigned(std::min<long long>((a.n2 + 255) / 256, 4096));
+        hipLaunchKernelGGL(k_combine, dim3(g), dim3(256), 0, 0, p->bs->d_mat64, a, sw->d_comb64 + sw->comb_off[k],
+                           sw->d_comb32 ? sw->d_comb32 + sw->comb_off[k] : nullptr);
+    }
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return KMX_E_HIP;
+    return KMX_OK;
 }
 
 void kmx_sweep_build(kmx_plan* p) {
@@ -865,23 +916,26 @@ void kmx_sweep_build(kmx_plan* p) {
             sw->separable = true;
             sw->identity = ident;
             int nsep = 0;
+            long long comb_elems = 0;
             for (int d = 0; d < D; ++d) {
                 std::vector<int> ts;
                 for (int t = 0; t < T; ++t)
                     if (active[t] == d) ts.push_back(t);
                 if (ts.empty()) continue;
                 UdOp a = base_op(d);
-                a.T = int(ts.size());
+                a.T = 1;  // the dimension's terms, folded into one combined matrix
                 for (int i = 0; i < KMX_MAX_TERMS; ++i) {
-                    a.a_off[i] = i < a.T ? p->off[ts[size_t(i)] * D + d] : 0;
+                    a.a_off[i] = i == 0 ? comb_elems : 0;
                     a.scale[i] = 1.0;
                 }
+                sw->comb_off.push_back(comb_elems);
+                comb_elems += (long long)p->n[d] * p->n[d];
                 a.x0 = 0;
                 a.in_base = 0;                         // slot 0: the box copy of x
                 a.out_base = (1 + nsep) * (long long)nb;  // slot 1 + k: op k's sum
                 ++nsep;
                 add_op(UD_SEP, false, a, 1, ts);
-                flops += fl_full[d] * a.T;
+                flops += fl_full[d];
             }
         } else if (D == 1) {
             UdOp a = base_op(0);
@@ -980,7 +1034,20 @@ void kmx_sweep_build(kmx_plan* p) {
         if (bs->d_mat32 && hipMalloc(&sw->d_arena[1], size_t(sw->arena) * sizeof(float)) != hipSuccess)
             throw std::bad_alloc();
         p->sw = sw;
-        if (sw->separable) sw->self = kmx_sweep_batch_build({p}, sweep_wave_batches(1));
+        if (sw->separable) {
+            const long long ce = sw->comb_off.empty() ? 0 : sw->comb_off.back() + (long long)p->n[sw->desc.back().a.d] *
+                                                                                   p->n[sw->desc.back().a.d];
+            for (size_t k = 0; k < sw->desc.size(); ++k)
+                if (sw->desc[k].a.a_off[0] != sw->comb_off[k] ||
+                    sw->comb_off[k] + (long long)p->n[sw->desc[k].a.d] * p->n[sw->desc[k].a.d] > ce)
+                    throw std::logic_error("sweep combined matrix offsets");
+            if (hipMalloc(&sw->d_comb64, size_t(std::max(1LL, ce)) * sizeof(double)) != hipSuccess)
+                throw std::bad_alloc();
+            if (bs->d_mat32 && hipMalloc(&sw->d_comb32, size_t(std::max(1LL, ce)) * sizeof(float)) != hipSuccess)
+                throw std::bad_alloc();
+            if (combine_matrices(p) != KMX_OK) throw std::runtime_error("combining the separable terms failed");
+            sw->self = kmx_sweep_batch_build({p}, sweep_wave_batches(1));
+        }
     } catch (const std::exception&) {
         p->sw = sw;
         kmx_sweep_free(p);
@@ -1006,7 +1073,7 @@ int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream
         using T = std::remove_pointer_t<decltype(arena)>;
         for (const auto& op : sw->ops) {
             UdOp a = op.a;
-            if (op.final || op.part == UD_SEP)  // the ops that scale their terms
+            if (op.final)  // the ops that scale their terms (a separable op's scales are in its matrix)
                 for (int i = 0; i < a.T; ++i) a.scale[i] = p->scale[op.terms[size_t(i)]];
             const dim3 grid(unsigned(op.wg1 - op.wg0)), block(UD_WAVES * 64);
             const int2* rc = reinterpret_cast<const int2*>(sw->d_recs) + size_t(op.wg0) * UD_WAVES * (UD_REC / 2);
@@ -1044,6 +1111,8 @@ void kmx_sweep_free(kmx_plan* p) {
     kmx_sweep* sw = p->sw;
     if (!sw) return;
     kmx_sweep_batch_free(sw->self);
+    if (sw->d_comb64) hipFree(sw->d_comb64);
+    if (sw->d_comb32) hipFree(sw->d_comb32);
     if (sw->d_xmap) hipFree(sw->d_xmap);
     if (sw->d_ymap) hipFree(sw->d_ymap);
     if (sw->d_recs) hipFree(sw->d_recs);
@@ -1066,8 +1135,6 @@ struct kmx_sweep_batch {
     BoxJob* d_jobs = nullptr;  // separable members: box copy of x first, slot sums into y last
     int njobs = 0, box_wgs = 0;
     UdBatchOp* d_ops = nullptr;
-    std::vector<UdBatchOp> h_ops;            // host copy (term scales refreshed from it)
-    std::vector<std::vector<int>> op_terms;  // plan term of each term of op i
     int* d_wgop = nullptr;  // op of every workgroup, phases in order
     int* d_recs = nullptr;  // UD_WAVES work records per workgroup
     bool has32 = true;
@@ -1112,10 +1179,12 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                     const kmx_sweep::Op& op = mops[q][k];
                     UdBatchOp o{};
                     o.a = op.a;
-                    if (op.final || op.part == UD_SEP)  // the ops that scale their terms
+                    if (op.final)  // the ops that scale their terms (a separable op's scales are in its matrix)
                         for (int i = 0; i < o.a.T; ++i) o.a.scale[i] = p->scale[op.terms[size_t(i)]];
-                    o.mat[0] = p->bs->d_mat64;
-                    o.mat[1] = p->bs->d_mat32;
+                    const bool sep = op.part == UD_SEP;
+                    o.mat[0] = sep ? static_cast<const void*>(sw->d_comb64) : p->bs->d_mat64;
+                    o.mat[1] = sep ? static_cast<const void*>(sw->d_comb32) : p->bs->d_mat32;
+                    if (sep && !sw->d_comb32) sb->has32 = false;
                     o.arena[0] = sw->d_arena[0];
                     o.arena[1] = sw->d_arena[1];
                     if (!sw->d_arena[1] || !p->bs->d_mat32) sb->has32 = false;
@@ -1127,7 +1196,6 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                     P.maxnbr = std::max(P.maxnbr, op.nbr);
                     const int oi = int(ops.size());
                     ops.push_back(o);
-                    sb->op_terms.push_back(op.terms);
                     for (int wg = op.wg0; wg < op.wg1; ++wg) {
                         wgop.push_back(oi);
                         for (int w = 0; w < UD_WAVES; ++w) {
@@ -1163,7 +1231,6 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                 throw std::runtime_error("hipMemcpy failed");
         };
         up(ops.data(), ops.size(), sb->d_ops);
-        sb->h_ops = ops;
         up(wgop.data(), wgop.size(), sb->d_wgop);
         up(flat.data(), flat.size(), sb->d_recs);
         up(jobs.data(), jobs.size(), sb->d_jobs);
@@ -1230,23 +1297,15 @@ void kmx_sweep_term_updated(kmx_plan* p, int term, int dim) {
     kmx_sweep* sw = p->sw;
     if (!sw || !sw->separable) return;
     if (sw->identity[size_t(term) * p->D + dim] && !identity_factor(p, term, dim)) sw->stale = true;
+    for (size_t k = 0; k < sw->desc.size(); ++k)
+        if (sw->desc[k].a.d == dim &&
+            std::find(sw->desc[k].terms.begin(), sw->desc[k].terms.end(), term) != sw->desc[k].terms.end())
+            if (combine_matrices(p, int(k)) != KMX_OK) throw std::runtime_error("combining the separable terms failed");
 }
 
 int kmx_sweep_scales_updated(kmx_plan* p) {
-    kmx_sweep* sw = p->sw;
-    if (!sw || !sw->self) return KMX_OK;
-    kmx_sweep_batch* sb = sw->self;
-    for (size_t i = 0; i < sb->h_ops.size(); ++i) {
-        UdBatchOp& o = sb->h_ops[i];
-        if (o.kind == UK_SEP)
-            for (int k = 0; k < o.a.T; ++k) o.a.scale[k] = p->scale[sb->op_terms[i][size_t(k)]];
-    }
-    // launches already queued may still read the old table
-    if (hipDeviceSynchronize() != hipSuccess ||
-        hipMemcpy(sb->d_ops, sb->h_ops.data(), sb->h_ops.size() * sizeof(UdBatchOp), hipMemcpyHostToDevice) !=
-            hipSuccess) {
-        set_error("kmx_plan_set_term_scale: refreshing the dimension sweep's scales failed");
-        return KMX_E_HIP;
-    }
-    return KMX_OK;
+    if (!p->sw || !p->sw->separable) return KMX_OK;
+    const int rc = combine_matrices(p);
+    if (rc != KMX_OK) set_error("kmx_plan_set_term_scale: recombining the separable terms failed");
+    return rc;
 }
