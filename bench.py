@@ -58,7 +58,7 @@ MEASURED_TRAFFIC = {
 # (its three launches together: box copies of x, every dimension's op, slot
 # sums into y), from the same FETCH_SIZE / WRITE_SIZE counter runs and scale
 # factors.  Source: profiles/batch_sweep_hbm_traffic.md.
-MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = 692120352
+MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = 360299136
 
 
 def build_operator(seed=2):
@@ -123,6 +123,38 @@ def cpu_baseline(op, budget_s=12.0):
     return y, {"value": 1.0 / dt, "unit": "applies/s", "cores": threads, "kind": "port",
                "sample": f"{n} full applies of the same workload (oracle/kmx_oracle.cpp, OpenMP over element rows)",
                "single_thread_applies_per_s": round(1.0 / dt1, 3)}
+
+
+def _cells(l):
+    return 1 if l == 0 else 1 << (l - 1)
+
+
+def separable_sweep_model(inst, elem):
+    """Work model of one batched apply on separable sweeps (every term acting in
+    one dimension): per instance and dimension one GEMM per group of level
+    tuples agreeing off that dimension (2 n^2 c flops: n positions of the group's
+    levels along the dimension, c columns), reading the dimension's combined 1D
+    matrix once (its largest leading block), the box copy of x once and writing
+    its slot once; plus the box copy of x (map, x, copy) and the slot sum (map,
+    slots, y).  Returns (flops, algorithmic bytes)."""
+    flops = nbytes = 0
+    for it in inst:
+        D, deg = it["nd"], it["deg"]
+        S = set(map(tuple, np.asarray(it["table"].lev).reshape(-1, D).tolist()))
+        nb = sum(int(np.prod([deg * _cells(l) for l in L])) for L in S)
+        for d in range(D):
+            rests = {}
+            for L in S:
+                R = L[:d] + L[d + 1:]
+                rests[R] = max(rests.get(R, -1), L[d])
+            for R, m in rests.items():
+                n = deg << m
+                flops += 2 * n * n * int(np.prod([deg * _cells(l) for l in R]))
+            nmax = deg << max(L[d] for L in S)
+            nbytes += nmax * nmax * elem + 2 * nb * elem
+        dof = it["table"].num_elem * deg ** D
+        nbytes += nb * (4 + elem) + dof * 8 + nb * (4 + D * elem) + dof * 8
+    return flops, nbytes
 
 
 def time_batch_apply(sysm, prec, warmup, steps, per_graph=10):
@@ -449,21 +481,27 @@ def main():
     head = batch["runs"]["fp32_sweep"]
     ms = head["ms_per_rk3_step"]
     apply_ms = head["ms_per_batched_apply"]
-    achieved = head["flops_per_batched_apply"] / (apply_ms * 1e-3) / 1e12
+    tflops = head["flops_per_batched_apply"] / (apply_ms * 1e-3) / 1e12
     traffic = MEASURED_TRAFFIC_BATCH_SWEEP_FP32
-    roofline = {
-        "bound": "mfma",
-        "achieved": round(achieved, 3),
-        "peak": FP32_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-        "traffic": traffic,
+    mflops, alg_bytes = separable_sweep_model(inst, 4)
+    if mflops == head["flops_per_batched_apply"]:  # every instance on the separable sweep: HBM-bound
+        achieved = alg_bytes / (apply_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "algorithmic_bytes_per_apply": alg_bytes,
+                    "intensity_flops_per_byte": round(mflops / alg_bytes, 3)}
+    else:
+        roofline = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": traffic}
+    roofline.update({
         "traffic_GBs": round(traffic / (apply_ms * 1e-3) / 1e9, 1) if traffic else None,
-        "kernel": "k_ud_batch<float,2> (v_mfma_f32_16x16x4_f32), the phase launches of one batched apply: "
-                  "useful sweep flops per apply / apply time (traffic: HBM bytes per batched apply)",
+        "kernel": "one FP32 batched apply = k_box_gather_batch<float> + k_ud_batch<float,2> "
+                  "(v_mfma_f32_16x16x4_f32) + k_box_sum_batch<float>; achieved = algorithmic bytes (or "
+                  "flops) per apply / apply time; traffic = measured HBM bytes per apply",
         "flops_per_apply": head["flops_per_batched_apply"],
+        "mfma_TFLOPs": round(tflops, 3),
         "ms_per_apply": apply_ms,
-    }
+    })
     out = {
         "metric": METRIC,
         "value": head["instance_applies_per_s"],

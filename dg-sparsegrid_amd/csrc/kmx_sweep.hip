@@ -901,6 +901,7 @@ void kmx_sweep_build(kmx_plan* p) {
         std::vector<char> ident(size_t(T) * D, 0);
         std::vector<int> active(T, 0);
         bool separable = !p->h_blk64.empty();
+        if (const char* e = std::getenv("KMX_SWEEP_SEPARABLE")) separable = separable && std::atoi(e) != 0;
         for (int t = 0; t < T && separable; ++t) {
             int na = 0;
             for (int k = 0; k < D; ++k) {
