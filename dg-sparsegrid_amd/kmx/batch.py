@@ -118,6 +118,11 @@ class BatchRK3Graph:
         dev = system.f.device
         self.k = [torch.empty(n, dtype=torch.float64, device=dev) for _ in range(3)]
         self.y = torch.empty(n, dtype=torch.float64, device=dev)
+        # per-dof stage weights dt * a_ij and dt * b_i: every stage update is then
+        # one or two fused multiply-adds over the state
+        dt = system.dt
+        self.c21, self.c31, self.c32 = dt * A21, dt * A31, dt * A32
+        self.b1, self.b2, self.b3 = dt * B1, dt * B2, dt * B3
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
@@ -128,19 +133,22 @@ class BatchRK3Graph:
         with torch.cuda.graph(self.graph):
             self._body(dry=False)
 
+    def _stages(self, f, k1, k2, k3, y):
+        import torch
+
+        s = self.s
+        s.apply(f, k1, self.prec)
+        torch.addcmul(f, self.c21, k1, out=y)
+        s.apply(y, k2, self.prec)
+        torch.addcmul(f, self.c31, k1, out=y).addcmul_(self.c32, k2)
+        s.apply(y, k3, self.prec)
+
     def _body(self, dry):
         s = self.s
         f = s.f.clone() if dry else s.f
         k1, k2, k3 = self.k
-        dt = s.dt
-        s.apply(f, k1, self.prec)
-        self.y.copy_(f).addcmul_(dt * A21, k1)
-        s.apply(self.y, k2, self.prec)
-        self.y.copy_(f).addcmul_(dt, A31 * k1 + A32 * k2)
-        s.apply(self.y, k3, self.prec)
-        upd = (B1 * k1 + B2 * k2 + B3 * k3) * dt
-        if not dry:
-            s.f.add_(upd)
+        self._stages(f, k1, k2, k3, self.y)
+        f.addcmul_(self.b1, k1).addcmul_(self.b2, k2).addcmul_(self.b3, k3)
 
     def step(self):
         self.graph.replay()
@@ -150,12 +158,7 @@ class BatchRK3Graph:
         """The same update without the graph (for checking replays)."""
         import torch
 
-        s = self.s
         k1, k2, k3 = (torch.empty_like(f) for _ in range(3))
         y = torch.empty_like(f)
-        s.apply(f, k1, self.prec)
-        y.copy_(f).addcmul_(s.dt * A21, k1)
-        s.apply(y, k2, self.prec)
-        y.copy_(f).addcmul_(s.dt, A31 * k1 + A32 * k2)
-        s.apply(y, k3, self.prec)
-        return f + (B1 * k1 + B2 * k2 + B3 * k3) * s.dt
+        self._stages(f, k1, k2, k3, y)
+        return f.clone().addcmul_(self.b1, k1).addcmul_(self.b2, k2).addcmul_(self.b3, k3)
