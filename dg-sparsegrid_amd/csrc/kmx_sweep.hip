@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
             ud_tile<T, UD_FULL, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part);
             break;
         case UK_UP_FINAL: ud_tile<T, UD_UP, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
-        case UK_SEP: ud_tile<T, UD_SEP, false, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
+        case UK_SEP: sep_tile<T>(mat, arena, rv, w, lane, o.a, part); break;
         default:
             if constexpr (MAXNBR == 4) {
                 if (o.kind == UK_DOWN4)
@@ -451,6 +451,126 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
             }
             break;
     }
+}
+
+// One wave of a separable op (UD_SEP, one combined term, scale folded in): the
+// same tile and record as ud_tile, with the K loop split by level.  Within a level
+// the A and B elements of step s sit at a wave-uniform offset (4 s columns of A, 4 s
+// positions of the source box) from this lane's own fixed offset, so a load is a
+// scalar base plus one vector offset and a step costs no vector address or mask
+// work.  Rows and columns past the group read row / column 0 (finite values that
+// reach only outputs the tile does not store); positions past a level, in its
+// last step when deg * cells is not a multiple of 4, read zero.
+template <typename T>
+__device__ __forceinline__ void sep_tile(const T* __restrict__ mat, T* __restrict__ arena, const int2 rv, const int w,
+                                         const int lane, const UdOp& op, T* __restrict__ part) {
+    using V = typename SwMfma<T>::V;
+    constexpr int NB = 8;  // steps per batch of loads
+    const int gsize = __builtin_amdgcn_readlane(rv.x, R_GSIZE);
+    const int gbase = __builtin_amdgcn_readlane(rv.x, R_GBASE);
+    const bool leader = gsize > 0 && w == gbase;
+    V acc = V{T(0), T(0), T(0), T(0)};
+    int oaddr[4] = {0, 0, 0, 0};
+    bool ook[4] = {false, false, false, false};
+    if (gsize > 0) {
+        const int m0 = __builtin_amdgcn_readlane(rv.x, R_M0), f0 = __builtin_amdgcn_readlane(rv.x, R_F0);
+        const int O = __builtin_amdgcn_readlane(rv.x, R_O), I = __builtin_amdgcn_readlane(rv.x, R_I);
+        const int li = lane & 15, lk = lane >> 4;
+        const int lr = __shfl(rv.x, R_ROWLEV + li);  // level of A row m0 + li (-1: past the group)
+        const int r0 = lr >= 0 ? m0 + li : 0;
+        const int f = f0 + li;
+        const bool fok = f < O * I;
+        const int fc = fok ? f : 0;
+        const unsigned magic = unsigned(__builtin_amdgcn_readlane(rv.x, R_MAGIC));
+        const int fo = I == 1 ? fc : magic ? int(__umulhi(unsigned(fc), magic)) : fc / I;
+        const int fi = fc - fo * I;
+        if (leader)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = SwMfma<T>::row(lane, q);
+                const int rb = __shfl(rv.x, R_ROWBASE + row), rn = __shfl(rv.x, R_ROWNL + row);
+                const int rl = __shfl(rv.x, R_ROWLEV + row);
+                ook[q] = fok && rl >= 0;
+                oaddr[q] = ook[q] ? rb + fo * rn + fi : 0;
+            }
+        const int tn = __builtin_amdgcn_readlane(rv.x, R_TN);
+        if (tn > 0) {
+            const int deg = op.deg, lda = op.lda;
+            const int s1 = __builtin_amdgcn_readlane(rv.x, R_SHI);
+            int l = __builtin_amdgcn_readlane(rv.x, R_LA), rem = __builtin_amdgcn_readlane(rv.x, R_SLO);
+            for (;;) {  // level and step of this wave's first step
+                const int ns = (deg * (l ? 1 << (l - 1) : 1) + 3) >> 2;
+                if (rem < ns) break;
+                rem -= ns;
+                ++l;
+            }
+            int q = __builtin_amdgcn_readlane(rv.x, R_SLO);  // steps done, over the tile's K range
+            int sin = rem;                                   // step inside level l
+            const T* amat = mat + op.a_off[0];
+            const T* bsrc = arena + op.in_base;
+            const int alane = r0 + lk * lda;  // this lane's A offset at step 0 of any level (plus the level's columns)
+            while (q < s1) {
+                const int st = l ? 1 << (l - 1) : 0;
+                const int nl = deg * (l ? st : 1);  // positions of level l
+                const int nsl = (nl + 3) >> 2;
+                const int nst = min(nsl - sin, s1 - q);  // steps of level l in this wave's share
+                const bool tail = (nl & 3) && sin + nst == nsl;  // the level's partial last step is among them
+                const int nfull = tail ? nst - 1 : nst;
+                const int box = __builtin_amdgcn_readlane(rv.y, min(l, UD_MAXLEV));
+                const int blane = (fo * nl + lk) * I + fi;  // this lane's B offset at step 0 of level l
+                const T* a0 = amat + (long long)(deg * st + 4 * sin) * lda;  // uniform: first step of the share
+                const T* b0 = bsrc + box + 4 * sin * I;
+                for (int s = 0; s < nfull; s += NB) {
+                    T av[NB], bv[NB];
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) {
+                        const int sc = min(s + c, nfull - 1);  // steps past the share repeat a valid one
+                        av[c] = (a0 + (long long)(4 * sc) * lda)[alane];
+                        bv[c] = (b0 + 4 * sc * I)[blane];
+                    }
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) acc = SwMfma<T>::run(s + c < nfull ? av[c] : T(0), bv[c], acc);
+                }
+                if (tail) {  // positions nl - (nl & 3) .. nl - 1 of the level, the rest zero
+                    const int pl = 4 * (nsl - 1) + lk;  // this lane's position in the level
+                    const bool pin = pl < nl;
+                    const T av = pin ? amat[(long long)(deg * st + pl) * lda + r0] : T(0);
+                    const T bv = pin ? bsrc[box + (fo * nl + pl) * I + fi] : T(0);
+                    acc = SwMfma<T>::run(av, bv, acc);
+                }
+                q += nst;
+                sin = 0;
+                ++l;
+            }
+        }
+        if (gsize > 1 && !leader)
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) part[((w - 1) * 4 + qq) * 64 + lane] = acc[qq];
+    }
+    __syncthreads();  // every wave of the workgroup
+    if (!leader) return;
+    for (int v = 1; v < gsize; ++v)  // the tile's partial sums in wave order
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) acc[qq] += part[((w + v - 1) * 4 + qq) * 64 + lane];
+    T* out = arena + op.out_base;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq)
+        if (ook[qq]) out[oaddr[qq]] = acc[qq];
+}
+
+// A phase of separable ops only (every op UD_SEP, one column tile): the small tile
+// body alone, so the launch keeps the occupancy its registers allow.
+template <typename T>
+__global__ void __launch_bounds__(UD_WAVES * 64) k_sep_batch(const UdBatchOp* __restrict__ ops,
+                                                              const int* __restrict__ wg_op,
+                                                              const int2* __restrict__ recs) {
+    __shared__ T part[(UD_WAVES - 1) * 4 * 64];
+    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int2 rv = recs[((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane];
+    const UdBatchOp& o = ops[wg_op[blockIdx.x]];
+    constexpr int sl = sizeof(T) == 8 ? 0 : 1;
+    sep_tile<T>(static_cast<const T*>(o.mat[sl]), static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
 }
 
 }  // namespace
@@ -1130,6 +1250,7 @@ double kmx_sweep_flops(const kmx_plan* p) { return p->sw ? double(p->sw->flops) 
 struct kmx_sweep_batch {
     struct Phase {
         int wg0, nwg, maxnbr;  // first workgroup, workgroups, most branches per tile
+        bool sep_only;         // every op separable: the separable kernel alone
     };
     std::vector<Phase> phases;
     std::vector<const kmx_plan*> members;
@@ -1169,7 +1290,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
         std::vector<int> wgop;
         std::vector<int> flat;  // records interleaved so that lane j reads ints j and 64 + j as one int2
         for (int ph = 0; ph < nph; ++ph) {  // groups end-aligned: every member's last group in the last phase
-            kmx_sweep_batch::Phase P{int(wgop.size()), 0, 1};
+            kmx_sweep_batch::Phase P{int(wgop.size()), 0, 1, true};
             for (size_t q = 0; q < members.size(); ++q) {
                 kmx_plan* p = members[q];
                 const kmx_sweep* sw = p->sw;
@@ -1195,6 +1316,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                     o.kind = ud_kind(op.part, op.final, op.nbr);
                     if (ud_kind_nbr(o.kind) != op.nbr || op.wg1 < op.wg0) throw std::logic_error("sweep batch op");
                     P.maxnbr = std::max(P.maxnbr, op.nbr);
+                    if (op.part != UD_SEP || op.nbr != 1) P.sep_only = false;
                     const int oi = int(ops.size());
                     ops.push_back(o);
                     for (int wg = op.wg0; wg < op.wg1; ++wg) {
@@ -1263,7 +1385,10 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
         const int* wgop = sb->d_wgop + P.wg0;
         const int2* rc = reinterpret_cast<const int2*>(sb->d_recs) + size_t(P.wg0) * UD_WAVES * (UD_REC / 2);
         const dim3 grid(unsigned(P.nwg)), block(UD_WAVES * 64);
-        if (slot == 0) {
+        if (P.sep_only) {
+            if (slot == 0) hipLaunchKernelGGL(k_sep_batch<double>, grid, block, 0, s, sb->d_ops, wgop, rc);
+            else hipLaunchKernelGGL(k_sep_batch<float>, grid, block, 0, s, sb->d_ops, wgop, rc);
+        } else if (slot == 0) {
             if (P.maxnbr <= 2) hipLaunchKernelGGL((k_ud_batch<double, 2>), grid, block, 0, s, sb->d_ops, wgop, rc, io);
             else hipLaunchKernelGGL((k_ud_batch<double, 4>), grid, block, 0, s, sb->d_ops, wgop, rc, io);
         } else {
