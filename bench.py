@@ -58,7 +58,7 @@ MEASURED_TRAFFIC = {
 # (its three launches together: box copies of x, every dimension's op, slot
 # sums into y), from the same FETCH_SIZE / WRITE_SIZE counter runs and scale
 # factors.  Source: profiles/batch_sweep_hbm_traffic.md.
-MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = 360299136
+MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = 365936896
 
 
 def build_operator(seed=2):
@@ -495,9 +495,10 @@ def main():
                     "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": traffic}
     roofline.update({
         "traffic_GBs": round(traffic / (apply_ms * 1e-3) / 1e9, 1) if traffic else None,
-        "kernel": "one FP32 batched apply = k_box_gather_batch<float> + k_ud_batch<float,2> "
-                  "(v_mfma_f32_16x16x4_f32) + k_box_sum_batch<float>; achieved = algorithmic bytes (or "
-                  "flops) per apply / apply time; traffic = measured HBM bytes per apply",
+        "kernel": "one FP32 batched apply = k_box_gather_batch<float> + k_sep_batch<float> "
+                  "(v_mfma_f32_16x16x4_f32; k_ud_batch<float,*> for general sweeps) + k_box_sum_batch<float>; "
+                  "achieved = algorithmic bytes (or flops) per apply / apply time; traffic = measured HBM "
+                  "bytes per apply",
         "flops_per_apply": head["flops_per_batched_apply"],
         "mfma_TFLOPs": round(tflops, 3),
         "ms_per_apply": apply_ms,
