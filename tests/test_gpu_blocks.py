@@ -308,3 +308,49 @@ def test_separable_sweep_update_term(kmx_gpu):
     plan.set_sweep("fp64", False)
     assert rel(plan.apply(xd, prec="fp64").cpu().numpy(), oracle_y(mats, idx, deg, rp, cols, x)) <= TOL64
     plan.destroy()
+
+
+def test_separable_terms_on_the_general_sweep(kmx_gpu, monkeypatch):
+    """KMX_SWEEP_SEPARABLE=0 at plan creation: separable terms run the general
+    down/up sweep (identity factors as ordinary matrices); same oracle bar."""
+    import torch
+
+    mats, idx, rp, cols, x = separable_case(3, 4, 2, (0, 1, 2, -1), True, seed=9)
+    yo = oracle_y(mats, idx, 2, rp, cols, x)
+    monkeypatch.setenv("KMX_SWEEP_SEPARABLE", "0")
+    plan = kmx_gpu.Plan(mats, idx, 2, rp, cols, index_base=1, fp32=True, level_blocks=True)
+    monkeypatch.delenv("KMX_SWEEP_SEPARABLE")
+    sep = kmx_gpu.Plan(mats, idx, 2, rp, cols, index_base=1, fp32=True, level_blocks=True)
+    assert plan.sweep("fp64")["flops"] > sep.sweep("fp64")["flops"]  # the general form does more work
+    xd = torch.from_numpy(x).cuda()
+    for prec, tol in (("fp64", TOL64), ("fp32", TOL32)):
+        plan.set_sweep(prec, True)
+        assert rel(plan.apply(xd, prec=prec).cpu().numpy(), yo) <= tol, prec
+    plan.destroy()
+    sep.destroy()
+
+
+def test_separable_batch_refuses_stale_member(kmx_gpu):
+    """A batch on merged sweeps refuses to apply once a member's identity factor
+    was replaced, and applies again on the pair path after the sweep is off."""
+    import torch
+
+    mats, idx, rp, cols, x = separable_case(2, 4, 3, (0, 1), True, seed=10)
+    plan = kmx_gpu.Plan(mats, idx, 3, rp, cols, index_base=1, fp32=True, level_blocks=True)
+    batch = kmx_gpu.Batch([plan])
+    batch.set_sweep("fp64", True)
+    xd = torch.from_numpy(x).cuda()
+    y = torch.empty_like(xd)
+    batch.apply([xd], [y], prec="fp64")
+    assert rel(y.cpu().numpy(), oracle_y(mats, idx, 3, rp, cols, x)) <= TOL64
+    n = mats[0][1].shape[0]
+    other = np.random.default_rng(11).standard_normal((n, n))
+    plan.update_term(0, 1, other)  # term 0 acted in dimension 0 only
+    mats[0][1] = other
+    with pytest.raises(RuntimeError):
+        batch.apply([xd], [y], prec="fp64")
+    batch.set_sweep("fp64", False)
+    batch.apply([xd], [y], prec="fp64")
+    assert rel(y.cpu().numpy(), oracle_y(mats, idx, 3, rp, cols, x)) <= TOL64
+    batch.destroy()
+    plan.destroy()
