@@ -363,7 +363,7 @@ int kmx_plan_set_term_scale(kmx_plan* p, int term, double alpha) {
         return KMX_E_ARG;
     }
     p->scale[term] = alpha;
-    return kmx_sweep_scales_updated(p);
+    return kmx_sweep_scales_updated(p, term);
 }
 
 int kmx_plan_destroy(kmx_plan* p) {
@@ -401,8 +401,10 @@ int kmx_apply_transpose(kmx_plan* p, const double* x, double* y, int prec, void*
         set_error("kmx_apply_transpose: null argument");
         return KMX_E_ARG;
     }
-    if (!p->d_blkT64) {
-        set_error("kmx_apply_transpose: plan was created without KMX_PLAN_ADJOINT");
+    const int slot = prec == KMX_PREC_FP64 ? 0 : 1;
+    const bool on_sweep = prec != KMX_PREC_BF16 && kmx_sweep_has_transpose(p) && p->var[slot].sweep;
+    if (!p->d_blkT64 && !on_sweep) {
+        set_error("kmx_apply_transpose: plan was created without KMX_PLAN_ADJOINT (and runs no separable sweep)");
         return KMX_E_ARG;
     }
     if (x == y && p->N > 0) {

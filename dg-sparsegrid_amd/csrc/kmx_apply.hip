@@ -164,8 +164,10 @@ int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, in
         set_error("kmx_apply: unknown precision");
         return KMX_E_ARG;
     }
-    if (!transpose && prec != KMX_PREC_FP32_GROUPED && p->sw && p->var[prec_slot(prec)].sweep)
-        return kmx_sweep_apply(p, prec_slot(prec), x, y, s);
+    if (prec != KMX_PREC_FP32_GROUPED && p->sw && p->var[prec_slot(prec)].sweep) {
+        if (!transpose) return kmx_sweep_apply(p, prec_slot(prec), x, y, s);
+        if (kmx_sweep_has_transpose(p)) return kmx_sweep_apply_transpose(p, prec_slot(prec), x, y, s);
+    }
     if (!transpose && prec != KMX_PREC_FP32_GROUPED && p->bs && p->bs->ok && p->bs->split[prec_slot(prec)].active)
         return kmx_blocks_apply(p, prec_slot(prec), x, y, s);
     kmx_variant v = prec == KMX_PREC_FP32_GROUPED ? p->var[1] : p->var[prec_slot(prec)];

@@ -196,7 +196,7 @@ double kmx_sweep_flops(const kmx_plan* p);
 // after kmx_plan_update_term(term, dim) (the FP64 block copy already updated)
 void kmx_sweep_term_updated(kmx_plan* p, int term, int dim);
 // after kmx_plan_set_term_scale: the plan's own launch tables take the new scales
-int kmx_sweep_scales_updated(kmx_plan* p);
+int kmx_sweep_scales_updated(kmx_plan* p, int term);
 // The sweeps of several plans merged into one launch per phase (kmx_sweep.hip):
 // a plan's ops form launch groups (one op each for a general sweep, in order; all
 // ops together for a separable one), phase k runs group k - (phases - groups of
@@ -205,9 +205,14 @@ int kmx_sweep_scales_updated(kmx_plan* p);
 // before (box copies of x) and one after (their slots summed into y).  members:
 // distinct plans, each with a sweep path; io slot q is members[q].  Term scales
 // are taken at build time; wave_batches <= 0: the batch default (four load
-// batches per wave).  Build throws std::exception on failure.
+// batches per wave); transpose: the separable members' transposed combined
+// matrices (the adjoint apply).  Build throws std::exception on failure.
 struct kmx_sweep_batch;
-kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, int wave_batches = 0);
+kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, int wave_batches = 0,
+                                       bool transpose = false);
+// the adjoint apply on a separable sweep (transposed combined matrices, same launches)
+bool kmx_sweep_has_transpose(const kmx_plan* p);
+int kmx_sweep_apply_transpose(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s);
 int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const kmx::BatchIO& io, hipStream_t s);
 void kmx_sweep_batch_free(kmx_sweep_batch* sb);
 int kmx_sweep_batch_phases(const kmx_sweep_batch* sb);

@@ -325,16 +325,22 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ m
 struct CombineArgs {
     long long off[KMX_MAX_TERMS];
     double scale[KMX_MAX_TERMS];
-    int nt;
+    int nt, n;
     long long n2;
 };
+// (and its transpose, for the adjoint apply: element (r, c) of the combined
+// matrix lands at (c, r))
 __global__ void __launch_bounds__(256) k_combine(const double* __restrict__ mat, CombineArgs a,
-                                                 double* __restrict__ out64, float* __restrict__ out32) {
+                                                 double* __restrict__ out64, float* __restrict__ out32,
+                                                 double* __restrict__ outT64, float* __restrict__ outT32) {
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < a.n2; i += (long long)gridDim.x * 256) {
         double v = 0.0;
         for (int k = 0; k < a.nt; ++k) v += a.scale[k] * mat[a.off[k] + i];
         out64[i] = v;
         if (out32) out32[i] = float(v);
+        const long long r = i % a.n, c = i / a.n, it = c + r * a.n;
+        outT64[it] = v;
+        if (outT32) outT32[it] = float(v);
     }
 }
 
@@ -652,7 +658,10 @@ struct kmx_sweep {
     // separable: op k's terms folded into one matrix (scales included) at comb_off[k]
     double* d_comb64 = nullptr;
     float* d_comb32 = nullptr;
+    double* d_combT64 = nullptr;  // their transposes (the adjoint apply), same offsets
+    float* d_combT32 = nullptr;
     std::vector<long long> comb_off;
+    kmx_sweep_batch* selfT = nullptr;  // separable: the adjoint apply's launches
     std::vector<Group> groups;
     std::vector<int> group0;  // groups of op dimension d: [group0[d], group0[d+1])
     int deg = 0, T = 0, max_nbr = 2;
@@ -881,10 +890,13 @@ int combine_matrices(kmx_plan* p, int only = -1) {
             a.off[i] = p->off[od.terms[size_t(i)] * p->D + d];
             a.scale[i] = p->scale[od.terms[size_t(i)]];
         }
+        a.n = p->n[d];
         a.n2 = (long long)p->n[d] * p->n[d];
         const unsigned g = unsigned(std::min<long long>((a.n2 + 255) / 256, 4096));
-        hipLaunchKernelGGL(k_combine, dim3(g), dim3(256), 0, 0, p->bs->d_mat64, a, sw->d_comb64 + sw->comb_off[k],
-                           sw->d_comb32 ? sw->d_comb32 + sw->comb_off[k] : nullptr);
+        const long long o = sw->comb_off[k];
+        hipLaunchKernelGGL(k_combine, dim3(g), dim3(256), 0, 0, p->bs->d_mat64, a, sw->d_comb64 + o,
+                           sw->d_comb32 ? sw->d_comb32 + o : nullptr, sw->d_combT64 + o,
+                           sw->d_combT32 ? sw->d_combT32 + o : nullptr);
     }
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return KMX_E_HIP;
     return KMX_OK;
@@ -1162,12 +1174,16 @@ void kmx_sweep_build(kmx_plan* p) {
                 if (sw->desc[k].a.a_off[0] != sw->comb_off[k] ||
                     sw->comb_off[k] + (long long)p->n[sw->desc[k].a.d] * p->n[sw->desc[k].a.d] > ce)
                     throw std::logic_error("sweep combined matrix offsets");
-            if (hipMalloc(&sw->d_comb64, size_t(std::max(1LL, ce)) * sizeof(double)) != hipSuccess)
+            const size_t cn = size_t(std::max(1LL, ce));
+            if (hipMalloc(&sw->d_comb64, cn * sizeof(double)) != hipSuccess ||
+                hipMalloc(&sw->d_combT64, cn * sizeof(double)) != hipSuccess)
                 throw std::bad_alloc();
-            if (bs->d_mat32 && hipMalloc(&sw->d_comb32, size_t(std::max(1LL, ce)) * sizeof(float)) != hipSuccess)
+            if (bs->d_mat32 && (hipMalloc(&sw->d_comb32, cn * sizeof(float)) != hipSuccess ||
+                                hipMalloc(&sw->d_combT32, cn * sizeof(float)) != hipSuccess))
                 throw std::bad_alloc();
             if (combine_matrices(p) != KMX_OK) throw std::runtime_error("combining the separable terms failed");
             sw->self = kmx_sweep_batch_build({p}, sweep_wave_batches(1));
+            sw->selfT = kmx_sweep_batch_build({p}, sweep_wave_batches(1), true);
         }
     } catch (const std::exception&) {
         p->sw = sw;
@@ -1232,8 +1248,11 @@ void kmx_sweep_free(kmx_plan* p) {
     kmx_sweep* sw = p->sw;
     if (!sw) return;
     kmx_sweep_batch_free(sw->self);
+    kmx_sweep_batch_free(sw->selfT);
     if (sw->d_comb64) hipFree(sw->d_comb64);
     if (sw->d_comb32) hipFree(sw->d_comb32);
+    if (sw->d_combT64) hipFree(sw->d_combT64);
+    if (sw->d_combT32) hipFree(sw->d_combT32);
     if (sw->d_xmap) hipFree(sw->d_xmap);
     if (sw->d_ymap) hipFree(sw->d_ymap);
     if (sw->d_recs) hipFree(sw->d_recs);
@@ -1262,8 +1281,11 @@ struct kmx_sweep_batch {
     bool has32 = true;
 };
 
-kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, int wave_batches) {
+kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, int wave_batches, bool transpose) {
     if (members.empty() || members.size() > KMX_BATCH_MAX) throw std::invalid_argument("sweep batch size");
+    for (const kmx_plan* p : members)
+        if (transpose && (!p || !p->sw || !p->sw->separable))
+            throw std::invalid_argument("the adjoint sweep needs the separable form");
     if (wave_batches <= 0) wave_batches = sweep_wave_batches(4);
     for (size_t q = 0; q < members.size(); ++q) {
         const kmx_plan* p = members[q];
@@ -1304,8 +1326,10 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                     if (op.final)  // the ops that scale their terms (a separable op's scales are in its matrix)
                         for (int i = 0; i < o.a.T; ++i) o.a.scale[i] = p->scale[op.terms[size_t(i)]];
                     const bool sep = op.part == UD_SEP;
-                    o.mat[0] = sep ? static_cast<const void*>(sw->d_comb64) : p->bs->d_mat64;
-                    o.mat[1] = sep ? static_cast<const void*>(sw->d_comb32) : p->bs->d_mat32;
+                    o.mat[0] = sep ? static_cast<const void*>(transpose ? sw->d_combT64 : sw->d_comb64)
+                                   : p->bs->d_mat64;
+                    o.mat[1] = sep ? static_cast<const void*>(transpose ? sw->d_combT32 : sw->d_comb32)
+                                   : p->bs->d_mat32;
                     if (sep && !sw->d_comb32) sb->has32 = false;
                     o.arena[0] = sw->d_arena[0];
                     o.arena[1] = sw->d_arena[1];
@@ -1429,9 +1453,24 @@ void kmx_sweep_term_updated(kmx_plan* p, int term, int dim) {
             if (combine_matrices(p, int(k)) != KMX_OK) throw std::runtime_error("combining the separable terms failed");
 }
 
-int kmx_sweep_scales_updated(kmx_plan* p) {
-    if (!p->sw || !p->sw->separable) return KMX_OK;
-    const int rc = combine_matrices(p);
-    if (rc != KMX_OK) set_error("kmx_plan_set_term_scale: recombining the separable terms failed");
-    return rc;
+int kmx_sweep_scales_updated(kmx_plan* p, int term) {
+    kmx_sweep* sw = p->sw;
+    if (!sw || !sw->separable) return KMX_OK;
+    for (size_t k = 0; k < sw->desc.size(); ++k)  // the one op whose combined matrix holds the term
+        if (std::find(sw->desc[k].terms.begin(), sw->desc[k].terms.end(), term) != sw->desc[k].terms.end()) {
+            const int rc = combine_matrices(p, int(k));
+            if (rc != KMX_OK) set_error("kmx_plan_set_term_scale: recombining the separable terms failed");
+            return rc;
+        }
+    return KMX_OK;
+}
+
+bool kmx_sweep_has_transpose(const kmx_plan* p) { return p->sw && p->sw->selfT; }
+
+int kmx_sweep_apply_transpose(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s) {
+    BatchIO io;
+    std::memset(&io, 0, sizeof(io));
+    io.x[0] = x;
+    io.y[0] = y;
+    return kmx_sweep_batch_apply(p->sw->selfT, slot, io, s);
 }

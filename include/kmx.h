@@ -147,7 +147,9 @@ int kmx_plan_info(const kmx_plan* plan, int64_t* num_pairs, int* block, int64_t*
 int kmx_apply(kmx_plan* plan, const double* x, double* y, int prec, void* stream);
 
 /* y = A^T x over the same pair list and terms (adjoint; no counterpart in the
- * reference, which has no gradient code).  Needs KMX_PLAN_ADJOINT. */
+ * reference, which has no gradient code).  Needs KMX_PLAN_ADJOINT, or a plan
+ * whose dimension sweep has the separable form and is on for prec (the sweep
+ * then runs with the transposed combined 1D matrices). */
 int kmx_apply_transpose(kmx_plan* plan, const double* x, double* y, int prec, void* stream);
 
 /* Host-pointer convenience for callers without device memory (a MEX gateway):
