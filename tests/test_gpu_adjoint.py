@@ -19,14 +19,18 @@ def build(lev, deg):
     return pde, table, rp, cols, mats
 
 
-@pytest.mark.parametrize("lev,deg,nsteps,every", [(3, 2, 6, 2), (5, 3, 20, 5)])
-def test_gradient_vs_central_fd(kmx_gpu, lev, deg, nsteps, every):
+@pytest.mark.parametrize("lev,deg,nsteps,every,sweep", [(3, 2, 6, 2, False), (5, 3, 20, 5, False), (3, 2, 6, 2, True)])
+def test_gradient_vs_central_fd(kmx_gpu, lev, deg, nsteps, every, sweep):
+    """sweep: forward and adjoint applies on the separable dimension sweep (the
+    terms of each dimension folded into one matrix, refolded at every scale change)."""
     import torch
 
     from kmx import autograd as AG
 
     pde, table, rp, cols, mats = build(lev, deg)
-    plan = kmx_gpu.Plan(mats, table.idx1d, deg, rp, cols, adjoint=True)
+    plan = kmx_gpu.Plan(mats, table.idx1d, deg, rp, cols, adjoint=True, level_blocks=sweep)
+    if sweep:
+        plan.set_sweep("fp64", True)
     groups = [[d, 3 + d] for d in range(3)]  # advection + diffusion terms of dim d
     op = AG.ScaledOperator(plan, groups)
     dt = pde.set_dt(pde, 0.01)
