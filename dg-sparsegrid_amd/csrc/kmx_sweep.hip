@@ -127,7 +127,8 @@ int ud_split_batch(int nbr) {
 // position 4 step + lk of the current level; the cursor is wave-uniform.  rv: this
 // lane's two ints of the wave's work record; part: LDS for the partial tiles of waves
 // 1.. of a split tile, (UD_WAVES - 1) x NBR x 4 x 64 values.  Every wave of the
-// workgroup calls it (it holds one workgroup barrier).
+// workgroup calls it (it holds one workgroup barrier).  Down, whole and up parts of
+// the general sweep; a separable op has its own body (sep_tile).
 template <typename T, int PART, bool FINAL, int NBR>
 __device__ __forceinline__ void ud_tile(const T* __restrict__ mat, T* __restrict__ arena, const int2 rv, const int w,
                                         const int lane, const UdOp& op, const double* __restrict__ x,
@@ -202,10 +203,8 @@ __device__ __forceinline__ void ud_tile(const T* __restrict__ mat, T* __restrict
             const T* bbase[NBR];
 #pragma unroll
             for (int j = 0; j < NBR; ++j)
-                bbase[j] = arena + (PART == UD_SEP                ? op.in_base
-                                    : (j == 0 && xin0) ? 0
-                                                       : op.in_base + ((long long)(b + j) * op.T + t) * op.nb);
-            const T sc = (FINAL || PART == UD_SEP) ? T(op.scale[t]) : T(1);
+                bbase[j] = arena + ((j == 0 && xin0) ? 0 : op.in_base + ((long long)(b + j) * op.T + t) * op.nb);
+            const T sc = FINAL ? T(op.scale[t]) : T(1);
             int l = la, rem = s0;  // level and step of step s0 (wave-uniform)
             for (;;) {
                 const int ns = (op.deg * (l ? 1 << (l - 1) : 1) + 3) >> 2;
@@ -226,7 +225,7 @@ __device__ __forceinline__ void ud_tile(const T* __restrict__ mat, T* __restrict
                         const int st = l ? 1 << (l - 1) : 0;
                         nl = op.deg * (l ? st : 1);
                         const int box = __builtin_amdgcn_readlane(rv.y, min(l, UD_MAXLEV));
-                        aon = rok && (PART == UD_FULL || PART == UD_SEP || (PART == UD_DOWN ? lr <= l : lr > l));
+                        aon = rok && (PART == UD_FULL || (PART == UD_DOWN ? lr <= l : lr > l));
                         aidx = r0 + (op.deg * st + p + lk) * op.lda;
                         bidx = box + (fo * nl + p + lk) * I + fi;
                         newlev = false;
@@ -297,8 +296,7 @@ __device__ __forceinline__ void ud_tile(const T* __restrict__ mat, T* __restrict
     } else {
 #pragma unroll
         for (int j = 0; j < NBR; ++j) {
-            T* out = arena + op.out_base +
-                     (PART == UD_SEP ? 0 : ((long long)((b + j) | op.out_bit) * op.T + tout) * op.nb);
+            T* out = arena + op.out_base + ((long long)((b + j) | op.out_bit) * op.T + tout) * op.nb;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 if (ook[q]) out[oaddr[q]] = acc[j][q] + addv[j][q];
@@ -1193,7 +1191,11 @@ void kmx_sweep_build(kmx_plan* p) {
 
 int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s) {
     kmx_sweep* sw = p->sw;
-    if (sw->self) {  // separable: x's box copy, every op in one launch, the slots summed into y
+    if (sw->separable) {  // x's box copy, every op in one launch, the slots summed into y
+        if (!sw->self) {
+            set_error("kmx_apply: separable sweep without its launch tables");
+            return KMX_E_ARG;
+        }
         BatchIO io;
         std::memset(&io, 0, sizeof(io));
         io.x[0] = x;
@@ -1216,7 +1218,6 @@ int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream
             const int2* rc = reinterpret_cast<const int2*>(sw->d_recs) + size_t(op.wg0) * UD_WAVES * (UD_REC / 2);
             auto pick = [&](auto nbr_c) {
                 constexpr int NBR = decltype(nbr_c)::value;
-                if (op.part == UD_SEP) return op.final ? k_ud_op<T, UD_SEP, true, 1> : k_ud_op<T, UD_SEP, false, 1>;
                 return op.part == UD_DOWN ? k_ud_op<T, UD_DOWN, false, NBR>
                        : op.part == UD_FULL ? (op.final ? k_ud_op<T, UD_FULL, true, 1> : k_ud_op<T, UD_FULL, false, NBR>)
                                             : (op.final ? k_ud_op<T, UD_UP, true, 1> : k_ud_op<T, UD_UP, false, NBR>);
