@@ -11,13 +11,14 @@ mixed precision (FP32 on the matrix cores) on the merged dimension sweeps,
 `speedup_vs_fp64_plain` its ratio to the same steps with the plain FP64 pair
 path, and `rk3_rel_l2_vs_fp64_plain` the final-state error between the two.
 
-Also reported: the single 3D operator (SG lev 7, deg 3, 3 terms, connect_nD,
-P = 1,903,000) in FP64 / FP32 pair by pair and through the dimension sweep
-(--no-single skips), the 6D lev 4 deg 2 precision study with FP32 / BF16
+Also reported: the same batching config over its full level range (3D levels
+up to 8, merged sweeps only; --no-full-range skips), the single 3D operator (SG
+lev 7, deg 3, 3 terms, connect_nD, P = 1,903,000) in FP64 / FP32 pair by pair
+and through the dimension sweep (--no-single skips), the 6D lev 4 deg 2 precision study with FP32 / BF16
 errors and a refined backward Euler step (--no-precision skips), and the CPU
 oracle on the host cores for both workloads (--no-cpu skips).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-single] [--no-precision]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-single] [--no-precision] [--no-full-range]
 
 Prints ONE JSON line.
 """
@@ -243,6 +244,51 @@ def batched_leg(steps=50, warmup=5, inst=None):
     return out, inst
 
 
+def batched_full_range_leg(steps=20, warmup=3):
+    """The batching config over its full level range: 64 adapted instances with
+    levels 4-8 in 2D and in 3D (make_instances seed 0, lev3d=(4, 8); 3D level-8
+    instances reach N = 8,869 elements and 78.7 M element pairs each).  Only the
+    merged dimension sweeps run: FP32 (timed) and FP64 (its reference
+    trajectory); the plain FP64 pair path would cost ~0.6 TFLOP per batched apply."""
+    import torch
+
+    from kmx import batch as BT
+
+    t0 = time.perf_counter()
+    inst = BT.make_instances(count=64, seed=0, lev3d=(4, 8))
+    out = {"instances": len(inst), "build_s": round(time.perf_counter() - t0, 1), "runs": {}}
+    states = {}
+    for name, prec in (("fp32_sweep", "fp32"), ("fp64_sweep", "fp64")):
+        sysm = BT.BatchedSystem(inst, sweep=(prec,))
+        apply_ms = time_batch_apply(sysm, prec, 3, max(10, steps // 2))
+        g = BT.BatchRK3Graph(sysm, prec=prec)
+        for _ in range(warmup):
+            g.step()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            g.step()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        states[name] = sysm.f.cpu().numpy()
+        out["runs"][name] = {"ms_per_rk3_step": round(ms, 4),
+                             "instance_applies_per_s": round(3 * len(inst) / (ms * 1e-3), 1),
+                             "ms_per_batched_apply": round(apply_ms, 4),
+                             "flops_per_batched_apply": sysm.flops_by_prec[prec],
+                             "merged_sweep": sysm.sweep[prec]}
+        out["pairs"] = int(sysm.pairs)
+        out["dof"] = int(sysm.dof)
+        out["pair_path_flops_per_apply"] = int(sysm.flops)
+        del g, sysm
+    d = states["fp32_sweep"] - states["fp64_sweep"]
+    out["runs"]["fp32_sweep"]["rk3_rel_l2_vs_fp64_sweep"] = float(np.linalg.norm(d) / np.linalg.norm(states["fp64_sweep"]))
+    out["rk3_steps"] = steps + warmup
+    return out
+
+
 def cpu_baseline_batch(inst, budget_s=12.0):
     """The oracle (C++ FP64 restatement of apply_A.m, OpenMP over element rows)
     applying every instance of the batch on the host cores: instance-applies/s."""
@@ -408,6 +454,8 @@ def main():
                     help="skip the 64-instance legs (profiling runs of the single-operator kernels: "
                          "the line then has no value)")
     ap.add_argument("--no-precision", action="store_true", help="skip the 6D precision leg")
+    ap.add_argument("--no-full-range", action="store_true",
+                    help="skip the 64-instance leg over the full level range (3D levels up to 8, sweeps only)")
     ap.add_argument("--sweep-study", action="store_true",
                     help="only time the pair kernels against the opt-in dimension-sweep path (one JSON line)")
     ap.add_argument("--pmc-pass", action="store_true",
@@ -527,6 +575,8 @@ def main():
         "batched": batch,
         "roofline": roofline,
     }
+    if not a.no_full_range:
+        out["batched_full_range"] = batched_full_range_leg()
     if not a.no_single:
         single, y64 = single_instance_leg(plan, tuned, op, xd, yd, steps=200, warmup=10)
         out["single_instance"] = single

@@ -209,8 +209,10 @@ int kmx_plan_create(const kmx_plan_desc* desc, kmx_plan** out) {
         }
 
         // prefix groups for the grouped FP32 kernel: per row, pairs sorted by
-        // (J_1..J_{d-1}) then J_d, groups ordered by size (largest first)
-        if (want32 && D >= 2) {
+        // (J_1..J_{d-1}) then J_d, groups ordered by size (largest first).  Not built
+        // for lists above 2^25 pairs (host sort time and a copy of the list); such
+        // plans run KMX_PREC_FP32_GROUPED as KMX_PREC_FP32.
+        if (want32 && D >= 2 && p->P <= (1LL << 25)) {
             int chunk = 4;
             if (const char* e = getenv("KMX_GROUP_CHUNK")) chunk = std::max(1, atoi(e));
             std::vector<int> gcols;
