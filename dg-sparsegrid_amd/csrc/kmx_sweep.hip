@@ -469,7 +469,7 @@ template <typename T>
 __device__ __forceinline__ void sep_tile(const T* __restrict__ mat, T* __restrict__ arena, const int2 rv, const int w,
                                          const int lane, const UdOp& op, T* __restrict__ part) {
     using V = typename SwMfma<T>::V;
-    constexpr int NB = 8;  // steps per batch of loads
+    constexpr int NB = 4;  // steps per whole batch of loads (4 timed fastest of 1, 2, 4, 8, 16)
     const int gsize = __builtin_amdgcn_readlane(rv.x, R_GSIZE);
     const int gbase = __builtin_amdgcn_readlane(rv.x, R_GBASE);
     const bool leader = gsize > 0 && w == gbase;
@@ -524,16 +524,27 @@ __device__ __forceinline__ void sep_tile(const T* __restrict__ mat, T* __restric
                 const int blane = (fo * nl + lk) * I + fi;  // this lane's B offset at step 0 of level l
                 const T* a0 = amat + (long long)(deg * st + 4 * sin) * lda;  // uniform: first step of the share
                 const T* b0 = bsrc + box + 4 * sin * I;
-                for (int s = 0; s < nfull; s += NB) {
+                int s = 0;
+                for (; s + NB <= nfull; s += NB) {  // whole batches: every load in flight at once
                     T av[NB], bv[NB];
 #pragma unroll
                     for (int c = 0; c < NB; ++c) {
-                        const int sc = min(s + c, nfull - 1);  // steps past the share repeat a valid one
+                        av[c] = (a0 + (long long)(4 * (s + c)) * lda)[alane];
+                        bv[c] = (b0 + 4 * (s + c) * I)[blane];
+                    }
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) acc = SwMfma<T>::run(av[c], bv[c], acc);
+                }
+                for (; s < nfull; s += 2) {  // the rest (short low levels) two steps at a time
+                    T av[2], bv[2];
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        const int sc = min(s + c, nfull - 1);  // a step past the share repeats a valid one
                         av[c] = (a0 + (long long)(4 * sc) * lda)[alane];
                         bv[c] = (b0 + 4 * sc * I)[blane];
                     }
 #pragma unroll
-                    for (int c = 0; c < NB; ++c) acc = SwMfma<T>::run(s + c < nfull ? av[c] : T(0), bv[c], acc);
+                    for (int c = 0; c < 2; ++c) acc = SwMfma<T>::run(s + c < nfull ? av[c] : T(0), bv[c], acc);
                 }
                 if (tail) {  // positions nl - (nl & 3) .. nl - 1 of the level, the rest zero
                     const int pl = 4 * (nsl - 1) + lk;  // this lane's position in the level
