@@ -44,7 +44,7 @@ FP32_PEAK_TFLOPS = 157.3
 # LDS, 2 strips of dims 2..d in LDS): rocprofv3 FETCH_SIZE and WRITE_SIZE
 # counter runs of `bench.py --pmc-pass`, fetch scaled by 2.0 and write by 1.0
 # (what the same counters read for a 256 MiB device copy).
-# Source: profiles/apply_cfg2_hbm_traffic.md.
+# Source: profiles/apply_3d_lev7_hbm_traffic.md.
 MEASURED_TRAFFIC = {
     ("fp64", 128, 0): 18585395, ("fp64", 128, 1): 31547699, ("fp64", 128, 2): 28021555,
     ("fp64", 256, 0): 18546790, ("fp64", 256, 1): 31549850, ("fp64", 256, 2): 28034560,
@@ -63,7 +63,7 @@ MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = 369346048
 
 
 def build_operator(seed=2):
-    """cfg2-sized operator in the reference's conventions (old hash + connect_nD)."""
+    """3D SG lev 7 deg 3 operator in the reference's conventions (old hash + connect_nD)."""
     from kmx import coeffs, grid
 
     nd, lev, deg = 3, 7, 3
@@ -398,7 +398,7 @@ def sweep_study(steps=50):
 
 
 def single_instance_leg(plan, tuned, op, xd, yd, steps, warmup):
-    """cfg2-sized single operator: the FP64 drop-in (pair by pair, the
+    """The single 3D lev 7 deg 3 operator: the FP64 drop-in (pair by pair, the
     reference's order of operations), the FP32 pair kernels, and the same
     operator through the dimension sweep of a level-block plan."""
     import kmx

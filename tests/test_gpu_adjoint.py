@@ -1,4 +1,4 @@
-"""Differentiation through the device apply (cfg5 shape): the adjoint kernel
+"""Differentiation through the device apply (3D variable-coefficient advection-diffusion, lev 5 deg 3): the adjoint kernel
 behind a torch.autograd.Function, RK3 with checkpointed recomputation,
 gradients of the final-time L2 misfit w.r.t. the initial coefficients and the
 per-dimension scalings, checked against central finite differences in FP64."""

@@ -49,10 +49,10 @@ def oracle_y(mats, idx, deg, rp, cols, x, threads=8):
     return O.apply_A(inp, x, nthreads=threads)
 
 
-# cfg1: 2D continuity-shaped operator, SG lev 3, deg 2, two terms; all pairs
+# 2D continuity-shaped operator, SG lev 3, deg 2, two terms; all pairs
 # (new hash, the reference default) and connect_nD (old hash)
 @pytest.mark.parametrize("conn", [False, True])
-def test_cfg1_shape_fp64(kmx_gpu, conn):
+def test_2d_lev3_shape_fp64(kmx_gpu, conn):
     mats, idx, rp, cols, x = make_case(2, 3, 2, 2, conn, seed=1)
     _, y = run(kmx_gpu, mats, idx, 2, rp, cols, x, "fp64")
     yo = oracle_y(mats, idx, 2, rp, cols, x)
@@ -83,7 +83,7 @@ def test_shapes_fp64_fp32(kmx_gpu, nd, lev, deg, T, conn):
     assert rel(yg, yo) <= TOL32
 
 
-def test_cfg2_full_size_fp64(kmx_gpu):
+def test_3d_lev7_full_size_fp64(kmx_gpu):
     """3D, lev 7, deg 3, 3 terms, connect_nD (P = 1,903,000): FP64 within 1e-13."""
     import torch
 
@@ -94,11 +94,11 @@ def test_cfg2_full_size_fp64(kmx_gpu):
     y64 = plan.apply(xd, prec="fp64").cpu().numpy()
     yo = oracle_y(mats, idx, 3, rp, cols, x, threads=16)
     r64 = rel(y64, yo)
-    print(f"cfg2 fp64 rel err {r64:.3e}")
+    print(f"3D lev 7 fp64 rel err {r64:.3e}")
     assert r64 <= TOL64
     y32 = plan.apply(xd, prec="fp32").cpu().numpy()
     yg = plan.apply(xd, prec="fp32_grouped").cpu().numpy()
-    print(f"cfg2 fp32 rel err {rel(y32, yo):.3e} grouped {rel(yg, yo):.3e}")
+    print(f"3D lev 7 fp32 rel err {rel(y32, yo):.3e} grouped {rel(yg, yo):.3e}")
     assert rel(y32, yo) <= TOL32 and rel(yg, yo) <= TOL32
     # adjoint identity <A x, w> = <x, A^T w> at full size
     w = np.random.default_rng(5).standard_normal(x.size)

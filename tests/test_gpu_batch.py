@@ -1,4 +1,4 @@
-"""Many instances in one persistent launch (cfg4 shape, reduced size):
+"""Many instances in one persistent launch (the 64-instance adapted-batch shape, reduced size):
 batched apply == per-plan applies == oracle, graph-replayed RK3 == eager RK3,
 and the FP32 batched trajectory stays within 1e-6 of the FP64 one."""
 import numpy as np

@@ -68,10 +68,10 @@ def check_all_splits(kmx, mats, idx, deg, rp, cols, x, want_structured=True, tol
 @pytest.mark.parametrize(
     "nd,lev,deg,T,conn",
     [
-        (2, 3, 2, 2, False), (2, 3, 2, 2, True),   # cfg1 shape: new hash all pairs / old hash connect_nD
+        (2, 3, 2, 2, False), (2, 3, 2, 2, True),   # 2D lev 3 deg 2: new hash all pairs / old hash connect_nD
         (2, 5, 1, 2, True), (2, 5, 4, 2, False), (2, 4, 3, 3, True),
         (3, 4, 2, 3, True), (3, 3, 3, 3, False), (3, 4, 4, 2, True),
-        (4, 3, 2, 4, True), (5, 2, 2, 2, True), (6, 3, 2, 6, True),   # 6D lev 3: cfg3's shape at a lower level
+        (4, 3, 2, 4, True), (5, 2, 2, 2, True), (6, 3, 2, 6, True),   # 6D lev 3: the 6D lev 4 shape at a lower level
     ],
 )
 def test_level_blocks_match_oracle(kmx_gpu, nd, lev, deg, T, conn):
@@ -79,7 +79,7 @@ def test_level_blocks_match_oracle(kmx_gpu, nd, lev, deg, T, conn):
     check_all_splits(kmx_gpu, mats, idx, deg, rp, cols, x)
 
 
-def test_cfg2_size_level_blocks(kmx_gpu):
+def test_3d_lev7_level_blocks(kmx_gpu):
     """3D lev 7 deg 3 with the maximal connect_nD list (P = 1,903,000)."""
     mats, idx, rp, cols, x = case(3, 7, 3, 3, True, seed=2)
     info = check_all_splits(kmx_gpu, mats, idx, 3, rp, cols, x)
@@ -175,7 +175,7 @@ def test_dimension_sweep_matches_oracle(kmx_gpu, nd, lev, deg, T, conn):
     plan.destroy()
 
 
-def test_dimension_sweep_cfg2_size_and_scales(kmx_gpu):
+def test_dimension_sweep_3d_lev7_and_scales(kmx_gpu):
     import torch
 
     mats, idx, rp, cols, x = case(3, 7, 3, 3, True, seed=2)

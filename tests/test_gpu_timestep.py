@@ -110,7 +110,7 @@ def test_backward_euler_refinement_reaches_fp64_residual(kmx_gpu):
     assert rel(f1.cpu().numpy(), ref) <= 1e-8
 
 
-def test_cfg1_continuity2_rk3(kmx_gpu):
+def test_2d_continuity2_rk3(kmx_gpu):
     """2D continuity (pdes/continuity2.m, LF = -1), SG lev 3, deg 2, explicit RK3,
     5 steps, CFL 0.01: device run == oracle run; L2 error vs the exact solution."""
     import torch
