@@ -351,18 +351,11 @@ struct BoxJob {
     long long nb;
     int nslot, inst, blk0;
 };
-__device__ __forceinline__ const BoxJob& box_job(const BoxJob* __restrict__ jobs, int njobs) {
-    int lo = 0, hi = njobs - 1;  // the last job starting at or before this workgroup
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].blk0 <= int(blockIdx.x)) lo = mid;
-        else hi = mid - 1;
-    }
-    return jobs[lo];
-}
+// wg_job[workgroup]: the job the workgroup's 256 box positions belong to.
 template <typename T>
-__global__ void __launch_bounds__(256) k_box_gather_batch(const BoxJob* __restrict__ jobs, int njobs, BatchIO io) {
-    const BoxJob& j = box_job(jobs, njobs);
+__global__ void __launch_bounds__(256) k_box_gather_batch(const BoxJob* __restrict__ jobs,
+                                                          const int* __restrict__ wg_job, BatchIO io) {
+    const BoxJob& j = jobs[wg_job[blockIdx.x]];
     const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
     if (i < j.nb) {
         const int m = j.xmap[i];
@@ -371,18 +364,18 @@ __global__ void __launch_bounds__(256) k_box_gather_batch(const BoxJob* __restri
     }
 }
 // y = the ops' slots summed (in op order), at the present cells
+// (the slots are read before the map, so both loads are in flight together)
 template <typename T>
-__global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict__ jobs, int njobs, BatchIO io) {
-    const BoxJob& j = box_job(jobs, njobs);
+__global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict__ jobs,
+                                                       const int* __restrict__ wg_job, BatchIO io) {
+    const BoxJob& j = jobs[wg_job[blockIdx.x]];
     const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
     if (i < j.nb) {
+        const T* sl = static_cast<const T*>(j.arena[sizeof(T) == 8 ? 0 : 1]) + j.nb + i;
+        double v = 0.0;
+        for (int k = 0; k < j.nslot; ++k) v += double(sl[(long long)k * j.nb]);
         const int m = j.ymap[i];
-        if (m >= 0) {
-            const T* sl = static_cast<const T*>(j.arena[sizeof(T) == 8 ? 0 : 1]) + j.nb + i;
-            double v = 0.0;
-            for (int k = 0; k < j.nslot; ++k) v += double(sl[(long long)k * j.nb]);
-            io.y[j.inst][m] = v;
-        }
+        if (m >= 0) io.y[j.inst][m] = v;
     }
 }
 
@@ -1286,6 +1279,7 @@ struct kmx_sweep_batch {
     std::vector<Phase> phases;
     std::vector<const kmx_plan*> members;
     BoxJob* d_jobs = nullptr;  // separable members: box copy of x first, slot sums into y last
+    int* d_wgjob = nullptr;    // job of each box workgroup
     int njobs = 0, box_wgs = 0;
     UdBatchOp* d_ops = nullptr;
     int* d_wgop = nullptr;  // op of every workgroup, phases in order
@@ -1373,6 +1367,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
         }
         if (flat.size() != wgop.size() * UD_WAVES * UD_REC) throw std::logic_error("sweep batch records");
         std::vector<BoxJob> jobs;
+        std::vector<int> wgjob;
         long long blk = 0;
         for (size_t q = 0; q < members.size(); ++q) {
             const kmx_sweep* sw = members[q]->sw;
@@ -1381,6 +1376,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                                   int(sw->desc.size()), int(q), int(blk)});
             blk += (sw->nb + 255) / 256;
             if (blk >= (1LL << 31)) throw std::runtime_error("sweep batch too large");
+            wgjob.resize(size_t(blk), int(jobs.size() - 1));
         }
         sb->njobs = int(jobs.size());
         sb->box_wgs = int(blk);
@@ -1393,6 +1389,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
         up(wgop.data(), wgop.size(), sb->d_wgop);
         up(flat.data(), flat.size(), sb->d_recs);
         up(jobs.data(), jobs.size(), sb->d_jobs);
+        up(wgjob.data(), wgjob.size(), sb->d_wgjob);
     } catch (...) {
         kmx_sweep_batch_free(sb);
         throw;
@@ -1413,8 +1410,8 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
     }
     const dim3 bg(unsigned(sb->box_wgs)), bb(256);
     if (sb->njobs > 0) {
-        if (slot == 0) hipLaunchKernelGGL(k_box_gather_batch<double>, bg, bb, 0, s, sb->d_jobs, sb->njobs, io);
-        else hipLaunchKernelGGL(k_box_gather_batch<float>, bg, bb, 0, s, sb->d_jobs, sb->njobs, io);
+        if (slot == 0) hipLaunchKernelGGL(k_box_gather_batch<double>, bg, bb, 0, s, sb->d_jobs, sb->d_wgjob, io);
+        else hipLaunchKernelGGL(k_box_gather_batch<float>, bg, bb, 0, s, sb->d_jobs, sb->d_wgjob, io);
     }
     for (const auto& P : sb->phases) {
         if (P.nwg == 0) continue;
@@ -1433,8 +1430,8 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
         }
     }
     if (sb->njobs > 0) {
-        if (slot == 0) hipLaunchKernelGGL(k_box_sum_batch<double>, bg, bb, 0, s, sb->d_jobs, sb->njobs, io);
-        else hipLaunchKernelGGL(k_box_sum_batch<float>, bg, bb, 0, s, sb->d_jobs, sb->njobs, io);
+        if (slot == 0) hipLaunchKernelGGL(k_box_sum_batch<double>, bg, bb, 0, s, sb->d_jobs, sb->d_wgjob, io);
+        else hipLaunchKernelGGL(k_box_sum_batch<float>, bg, bb, 0, s, sb->d_jobs, sb->d_wgjob, io);
     }
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
@@ -1450,6 +1447,7 @@ void kmx_sweep_batch_free(kmx_sweep_batch* sb) {
     if (sb->d_wgop) hipFree(sb->d_wgop);
     if (sb->d_recs) hipFree(sb->d_recs);
     if (sb->d_jobs) hipFree(sb->d_jobs);
+    if (sb->d_wgjob) hipFree(sb->d_wgjob);
     delete sb;
 }
 
