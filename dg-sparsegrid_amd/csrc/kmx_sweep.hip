@@ -457,12 +457,12 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
 // scalar base plus one vector offset and a step costs no vector address or mask
 // work.  Rows and columns past the group read row / column 0 (finite values that
 // reach only outputs the tile does not store); positions past a level, in its
-// last step when deg * cells is not a multiple of 4, read zero.
-template <typename T>
+// last step when deg * cells is not a multiple of 4, read zero.  NB: steps per load
+// batch; PF: the next batch's loads issued before this batch multiplies.
+template <typename T, int NB = 4, bool PF = false>
 __device__ __forceinline__ void sep_tile(const T* __restrict__ mat, T* __restrict__ arena, const int2 rv, const int w,
                                          const int lane, const UdOp& op, T* __restrict__ part) {
     using V = typename SwMfma<T>::V;
-    constexpr int NB = 4;  // steps per whole batch of loads (4 timed fastest of 1, 2, 4, 8, 16)
     const int gsize = __builtin_amdgcn_readlane(rv.x, R_GSIZE);
     const int gbase = __builtin_amdgcn_readlane(rv.x, R_GBASE);
     const bool leader = gsize > 0 && w == gbase;
@@ -518,12 +518,37 @@ __device__ __forceinline__ void sep_tile(const T* __restrict__ mat, T* __restric
                 const T* a0 = amat + (long long)(deg * st + 4 * sin) * lda;  // uniform: first step of the share
                 const T* b0 = bsrc + box + 4 * sin * I;
                 int s = 0;
-                for (; s + NB <= nfull; s += NB) {  // whole batches: every load in flight at once
+                if (!PF) {
+                    for (; s + NB <= nfull; s += NB) {  // whole batches: every load in flight at once
+                        T av[NB], bv[NB];
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) {
+                            av[c] = (a0 + (long long)(4 * (s + c)) * lda)[alane];
+                            bv[c] = (b0 + 4 * (s + c) * I)[blane];
+                        }
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) acc = SwMfma<T>::run(av[c], bv[c], acc);
+                    }
+                } else if (nfull >= NB) {  // whole batches, the next batch's loads in flight while one multiplies
                     T av[NB], bv[NB];
 #pragma unroll
                     for (int c = 0; c < NB; ++c) {
-                        av[c] = (a0 + (long long)(4 * (s + c)) * lda)[alane];
-                        bv[c] = (b0 + 4 * (s + c) * I)[blane];
+                        av[c] = (a0 + (long long)(4 * c) * lda)[alane];
+                        bv[c] = (b0 + 4 * c * I)[blane];
+                    }
+                    for (s = NB; s + NB <= nfull; s += NB) {
+                        T an[NB], bn[NB];
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) {
+                            an[c] = (a0 + (long long)(4 * (s + c)) * lda)[alane];
+                            bn[c] = (b0 + 4 * (s + c) * I)[blane];
+                        }
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) {
+                            acc = SwMfma<T>::run(av[c], bv[c], acc);
+                            av[c] = an[c];
+                            bv[c] = bn[c];
+                        }
                     }
 #pragma unroll
                     for (int c = 0; c < NB; ++c) acc = SwMfma<T>::run(av[c], bv[c], acc);
@@ -578,7 +603,12 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_sep_batch(const UdBatchOp* __
     const int2 rv = recs[((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane];
     const UdBatchOp& o = ops[wg_op[blockIdx.x]];
     constexpr int sl = sizeof(T) == 8 ? 0 : 1;
-    sep_tile<T>(static_cast<const T*>(o.mat[sl]), static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
+    // Load batches (timed on 64 adapted 2D / 3D instances): FP32 two steps with the next
+    // two in flight, ahead of four steps at once (+3 %), four in flight (+2 %) and eight
+    // (+9 %); FP64 four steps at once, ahead of two or four in flight (+2 %) and eight (+13 %).
+    constexpr bool f32 = sizeof(T) == 4;
+    sep_tile<T, f32 ? 2 : 4, f32>(static_cast<const T*>(o.mat[sl]), static_cast<T*>(o.arena[sl]), rv, w, lane, o.a,
+                                  part);
 }
 
 }  // namespace
