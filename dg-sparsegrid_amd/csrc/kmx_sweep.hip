@@ -1393,6 +1393,45 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                     if (wgop.size() >= size_t(1) << 30) throw std::runtime_error("sweep batch too large");
                 }
             }
+            if (P.sep_only && P.nwg > 1) {
+                // Workgroups in classes of their longest wave's steps (powers of two), longer
+                // classes first, emission order (instance by instance) inside a class: the long
+                // tiles of every instance start early without scattering each instance's tiles
+                // over the whole launch.  On 64 adapted 2D / 3D instances: FP32 -2 %, FP64 -8 %;
+                // a strict longest-first order: FP32 +2 %, FP64 -8 %.
+                const size_t rw = size_t(UD_WAVES) * UD_REC;  // ints per workgroup
+                auto rec_int = [&](size_t wg, int w, int k) {  // int k of wave w's record
+                    const size_t b = wg * rw + size_t(w) * UD_REC;
+                    return k < UD_REC / 2 ? flat[b + 2 * size_t(k)] : flat[b + 2 * size_t(k - UD_REC / 2) + 1];
+                };
+                std::vector<int> cls(size_t(P.nwg), 0);
+                for (int g = 0; g < P.nwg; ++g) {
+                    long long most = 0;
+                    for (int w = 0; w < UD_WAVES; ++w) {
+                        const size_t wg = size_t(P.wg0 + g);
+                        if (rec_int(wg, w, R_GSIZE) == 0 || rec_int(wg, w, R_TN) == 0) continue;
+                        const long long st = (long long)(rec_int(wg, w, R_TN) - 1) * rec_int(wg, w, R_SPER) +
+                                             rec_int(wg, w, R_SHI) - rec_int(wg, w, R_SLO);
+                        most = std::max(most, st);
+                    }
+                    while (most > 0) {
+                        ++cls[size_t(g)];
+                        most >>= 1;
+                    }
+                }
+                std::vector<int> idx(size_t(P.nwg));
+                for (int g = 0; g < P.nwg; ++g) idx[size_t(g)] = g;
+                std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cls[size_t(a)] > cls[size_t(b)]; });
+                std::vector<int> wg2(size_t(P.nwg)), fl2(size_t(P.nwg) * rw);
+                for (int g = 0; g < P.nwg; ++g) {
+                    const size_t src = size_t(P.wg0 + idx[size_t(g)]);
+                    wg2[size_t(g)] = wgop[src];
+                    std::copy(flat.begin() + long(src * rw), flat.begin() + long((src + 1) * rw),
+                              fl2.begin() + long(size_t(g) * rw));
+                }
+                std::copy(wg2.begin(), wg2.end(), wgop.begin() + P.wg0);
+                std::copy(fl2.begin(), fl2.end(), flat.begin() + long(size_t(P.wg0) * rw));
+            }
             sb->phases.push_back(P);
         }
         if (flat.size() != wgop.size() * UD_WAVES * UD_REC) throw std::logic_error("sweep batch records");
