@@ -603,12 +603,12 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_sep_batch(const UdBatchOp* __
     const int2 rv = recs[((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane];
     const UdBatchOp& o = ops[wg_op[blockIdx.x]];
     constexpr int sl = sizeof(T) == 8 ? 0 : 1;
-    // Load batches (timed on 64 adapted 2D / 3D instances): FP32 two steps with the next
-    // two in flight, ahead of four steps at once (+3 %), four in flight (+2 %) and eight
-    // (+9 %); FP64 four steps at once, ahead of two or four in flight (+2 %) and eight (+13 %).
+    // Load batches (timed on 64 adapted 2D / 3D instances, eight batches per wave): FP32
+    // four steps with the next four in flight, ahead of two in flight (+3 %), four at once
+    // (+2 %) and eight at once (+7 %); FP64 four steps at once (two or four in flight and
+    // eight at once within +-1 %).
     constexpr bool f32 = sizeof(T) == 4;
-    sep_tile<T, f32 ? 2 : 4, f32>(static_cast<const T*>(o.mat[sl]), static_cast<T*>(o.arena[sl]), rv, w, lane, o.a,
-                                  part);
+    sep_tile<T, 4, f32>(static_cast<const T*>(o.mat[sl]), static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
 }
 
 }  // namespace
