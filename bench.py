@@ -59,7 +59,7 @@ MEASURED_TRAFFIC = {
 # (its three launches together: box copies of x, every dimension's op, slot
 # sums into y), from the same FETCH_SIZE / WRITE_SIZE counter runs and scale
 # factors.  Source: profiles/batch_sweep_hbm_traffic.md.
-MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = 344428954
+MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = 345246106
 
 
 def build_operator(seed=2):

@@ -344,6 +344,7 @@ __global__ void __launch_bounds__(256) k_combine(const double* __restrict__ mat,
 
 // Box copies of x and slot sums of every separable instance of a batch, one
 // launch each: job j covers workgroups [blk0_j, blk0_{j+1}), 256 positions each.
+// Every workgroup reads its own copy of its job (one load, no index to chase).
 struct BoxJob {
     const int* xmap;
     const int* ymap;
@@ -351,11 +352,9 @@ struct BoxJob {
     long long nb;
     int nslot, inst, blk0;
 };
-// wg_job[workgroup]: the job the workgroup's 256 box positions belong to.
 template <typename T>
-__global__ void __launch_bounds__(256) k_box_gather_batch(const BoxJob* __restrict__ jobs,
-                                                          const int* __restrict__ wg_job, BatchIO io) {
-    const BoxJob& j = jobs[wg_job[blockIdx.x]];
+__global__ void __launch_bounds__(256) k_box_gather_batch(const BoxJob* __restrict__ wg_jobs, BatchIO io) {
+    const BoxJob& j = wg_jobs[blockIdx.x];
     const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
     if (i < j.nb) {
         const int m = j.xmap[i];
@@ -366,9 +365,8 @@ __global__ void __launch_bounds__(256) k_box_gather_batch(const BoxJob* __restri
 // y = the ops' slots summed (in op order), at the present cells
 // (the slots are read before the map, so both loads are in flight together)
 template <typename T>
-__global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict__ jobs,
-                                                       const int* __restrict__ wg_job, BatchIO io) {
-    const BoxJob& j = jobs[wg_job[blockIdx.x]];
+__global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict__ wg_jobs, BatchIO io) {
+    const BoxJob& j = wg_jobs[blockIdx.x];
     const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
     if (i < j.nb) {
         const T* sl = static_cast<const T*>(j.arena[sizeof(T) == 8 ? 0 : 1]) + j.nb + i;
@@ -1309,8 +1307,7 @@ struct kmx_sweep_batch {
     };
     std::vector<Phase> phases;
     std::vector<const kmx_plan*> members;
-    BoxJob* d_jobs = nullptr;  // separable members: box copy of x first, slot sums into y last
-    int* d_wgjob = nullptr;    // job of each box workgroup
+    BoxJob* d_wgjobs = nullptr;  // separable members' box copies of x and slot sums: each workgroup's job
     int njobs = 0, box_wgs = 0;
     UdBatchOp* d_ops = nullptr;
     int* d_wgop = nullptr;  // op of every workgroup, phases in order
@@ -1439,8 +1436,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
             sb->phases.push_back(P);
         }
         if (flat.size() != wgop.size() * UD_WAVES * UD_REC) throw std::logic_error("sweep batch records");
-        std::vector<BoxJob> jobs;
-        std::vector<int> wgjob;
+        std::vector<BoxJob> jobs, wgjobs;
         long long blk = 0;
         for (size_t q = 0; q < members.size(); ++q) {
             const kmx_sweep* sw = members[q]->sw;
@@ -1449,7 +1445,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                                   int(sw->desc.size()), int(q), int(blk)});
             blk += (sw->nb + 255) / 256;
             if (blk >= (1LL << 31)) throw std::runtime_error("sweep batch too large");
-            wgjob.resize(size_t(blk), int(jobs.size() - 1));
+            wgjobs.resize(size_t(blk), jobs.back());
         }
         sb->njobs = int(jobs.size());
         sb->box_wgs = int(blk);
@@ -1461,8 +1457,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
         up(ops.data(), ops.size(), sb->d_ops);
         up(wgop.data(), wgop.size(), sb->d_wgop);
         up(flat.data(), flat.size(), sb->d_recs);
-        up(jobs.data(), jobs.size(), sb->d_jobs);
-        up(wgjob.data(), wgjob.size(), sb->d_wgjob);
+        up(wgjobs.data(), wgjobs.size(), sb->d_wgjobs);
     } catch (...) {
         kmx_sweep_batch_free(sb);
         throw;
@@ -1483,8 +1478,8 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
     }
     const dim3 bg(unsigned(sb->box_wgs)), bb(256);
     if (sb->njobs > 0) {
-        if (slot == 0) hipLaunchKernelGGL(k_box_gather_batch<double>, bg, bb, 0, s, sb->d_jobs, sb->d_wgjob, io);
-        else hipLaunchKernelGGL(k_box_gather_batch<float>, bg, bb, 0, s, sb->d_jobs, sb->d_wgjob, io);
+        if (slot == 0) hipLaunchKernelGGL(k_box_gather_batch<double>, bg, bb, 0, s, sb->d_wgjobs, io);
+        else hipLaunchKernelGGL(k_box_gather_batch<float>, bg, bb, 0, s, sb->d_wgjobs, io);
     }
     for (const auto& P : sb->phases) {
         if (P.nwg == 0) continue;
@@ -1503,8 +1498,8 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
         }
     }
     if (sb->njobs > 0) {
-        if (slot == 0) hipLaunchKernelGGL(k_box_sum_batch<double>, bg, bb, 0, s, sb->d_jobs, sb->d_wgjob, io);
-        else hipLaunchKernelGGL(k_box_sum_batch<float>, bg, bb, 0, s, sb->d_jobs, sb->d_wgjob, io);
+        if (slot == 0) hipLaunchKernelGGL(k_box_sum_batch<double>, bg, bb, 0, s, sb->d_wgjobs, io);
+        else hipLaunchKernelGGL(k_box_sum_batch<float>, bg, bb, 0, s, sb->d_wgjobs, io);
     }
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
@@ -1519,8 +1514,7 @@ void kmx_sweep_batch_free(kmx_sweep_batch* sb) {
     if (sb->d_ops) hipFree(sb->d_ops);
     if (sb->d_wgop) hipFree(sb->d_wgop);
     if (sb->d_recs) hipFree(sb->d_recs);
-    if (sb->d_jobs) hipFree(sb->d_jobs);
-    if (sb->d_wgjob) hipFree(sb->d_wgjob);
+    if (sb->d_wgjobs) hipFree(sb->d_wgjobs);
     delete sb;
 }
 
