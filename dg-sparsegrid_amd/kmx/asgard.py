@@ -5,6 +5,7 @@ library:
 
     ftmp = apply_A(pde, opts, A_data, f, deg, Vmax, Emax, imex_flag)   apply_A.m:1
     f    = time_advance(pde, opts, A_data, f, t, dt, deg, hash_table)   time_advance.m:1
+    Meval, nodes = matrix_plot_D(pde, opts, dimension)                matrix_plot_D.m:1
     fr   = wavelet_to_realspace(pde, opts, Meval_D, f, hash_table)      wavelet_to_realspace.m:1
     err, fval = asgard(pde_fn, **opts)                                  asgard.m:3 (RK3/BE subset)
 
@@ -38,6 +39,7 @@ class OPTS:
     max_lev_coeffs: bool = True
     num_steps: int = 5
     quiet: bool = True
+    output_grid: str = "quadrature"  # OPTS.m:30 (matrix_plot_D restates this grid only)
     prec: str = "fp64"  # contraction precision of the device apply (not a reference option)
     extra: dict = field(default_factory=dict)
 
@@ -92,6 +94,13 @@ def time_advance(pde, opts, A_data, f, t, dt, deg, hash_table=None, Vmax=None, E
     else:
         y = TS.rk3_step(lambda v: apply_A(pde, opts, A_data, v, deg), x, t, dt, src)
     return y.cpu().numpy() if host else y
+
+
+def matrix_plot_D(pde, opts, dimension):
+    """matrix_plot_D.m:1 (no input nodes): the evaluation matrix and points of one
+    dimension on opts.output_grid (only 'quadrature', the default), for
+    wavelet_to_realspace.  Returns (Meval, nodes)."""
+    return C.matrix_plot_D(opts.deg, dimension.lev, dimension.min, dimension.max, opts.output_grid)
 
 
 def wavelet_to_realspace(pde, opts, Meval_D, f_wSpace, hash_table):

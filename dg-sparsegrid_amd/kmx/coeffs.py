@@ -133,6 +133,29 @@ def fmwt(deg, lev):
     return F
 
 
+def matrix_plot_D(deg, lev, xmin, xmax, output_grid="quadrature"):
+    """matrix_plot_D.m:1-172 for its default output grid ('quadrature', OPTS.m:30):
+    the evaluation matrix taking a 1D wavelet-space coefficient vector to values
+    at the deg Gauss-Legendre points of each of the 2^lev cells of [xmin, xmax]
+    (deg * 2^lev rows, points in cell order; columns as the wavelet basis).  Cell i
+    holds the orthonormal Legendre values at its points scaled by sqrt(1/h)
+    (:138), columns deg*i .. deg*(i+1)-1 of the real-space basis (:148, :168); the
+    wavelet columns come from the inverse transform, Meval * FMWT^T (:172).
+    Returns (Meval, nodes)."""
+    if output_grid != "quadrature":
+        raise NotImplementedError(f"matrix_plot_D: output_grid {output_grid!r} (only 'quadrature')")
+    n = 2 ** lev
+    h = (xmax - xmin) / n
+    qx, _ = lgwt(deg, -1.0, 1.0)
+    p_val = lin_legendre(qx, deg) * np.sqrt(1.0 / h)
+    M = np.zeros((deg * n, deg * n))
+    nodes = np.zeros(deg * n)
+    for i in range(n):
+        M[deg * i:deg * (i + 1), deg * i:deg * (i + 1)] = p_val
+        nodes[deg * i:deg * (i + 1)] = ((qx + 1) / 2 + i) * h + xmin
+    return M @ fmwt(deg, lev).T, nodes
+
+
 # ------------------------------------------------------------ term classes
 def _one(x, p=None, t=None, dat=None):
     return np.zeros_like(np.asarray(x, dtype=np.float64)) + 1.0

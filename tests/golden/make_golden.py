@@ -44,6 +44,17 @@ PATTERNS = [
     r"quadrature/lgwt_(roots|weights)_\d+_\w+$",
 ]
 
+# Fixtures too large to keep whole, kept as regular samples (the stride is stored
+# beside them under "<key>/stride"): the evaluation matrices of matrix_plot_D.m
+# (gold_matrix_plot_D.m) as every 8th row, and the wavelet_to_realspace vectors
+# (gold_wavelet_to_realspace.m) as every value, or every 8th of the 3D one.
+SAMPLED = {
+    **{f"transformations/matrix_plot_D/continuity_{k}/matrix_plot_D_{d}": 8 for k in (1, 2, 3) for d in range(k)},
+    "transformations/wavelet_to_realspace_continuity_1": 1,
+    "transformations/wavelet_to_realspace_continuity_2": 1,
+    "transformations/wavelet_to_realspace_continuity_3": 8,
+}
+
 
 def read_octave_dat(path):
     """Parse one octave-like text fixture into a 2-D float64 array."""
@@ -90,6 +101,10 @@ def main():
             key = f"{sub}/{fn[:-4]}"
             if any(p.match(key) for p in pats):
                 out[key] = read_octave_dat(os.path.join(d, fn))
+    for key, stride in SAMPLED.items():
+        a = read_octave_dat(os.path.join(GOLD, key + ".dat"))
+        out[key] = a[::stride].copy()
+        out[key + "/stride"] = np.asarray([stride], np.int64)
     np.savez_compressed(OUT, **out)
     print(f"wrote {len(out)} fixtures to {OUT} ({os.path.getsize(OUT)} bytes)")
 

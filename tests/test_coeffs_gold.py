@@ -33,6 +33,44 @@ def test_fmwt(gold, key):
     assert np.max(np.abs(F @ F.T - np.eye(F.shape[0]))) <= 1e-12  # orthogonal (the deg-5 filters reach 2.3e-13)
 
 
+# gold_matrix_plot_D.m:1-3 (lev, deg); domains pdes/continuity1.m:16, continuity2.m:27-29,
+# continuity3.m:24-34
+MEVAL_GOLD = [("continuity_1", 7, 7, [(-1, 1)]),
+              ("continuity_2", 6, 7, [(-1, 1), (-2, 2)]),
+              ("continuity_3", 5, 6, [(-1, 1), (-2, 2), (-3, 3)])]
+
+
+@pytest.mark.parametrize("name,deg,lev,doms", MEVAL_GOLD)
+def test_matrix_plot_D_matches_gold(gold, name, deg, lev, doms):
+    """matrix_plot_D.m (default 'quadrature' grid) against the reference's evaluation
+    matrices, every 8th row stored; a planted mistake (the first dimension's cell
+    width in every dimension) is caught where the domains differ."""
+    for d, (a, b) in enumerate(doms):
+        key = f"transformations/matrix_plot_D/{name}/matrix_plot_D_{d}"
+        stride = int(gold[key + "/stride"][0])
+        ref = gold[key]
+        M, nodes = K.matrix_plot_D(deg, lev, a, b)
+        assert M.shape == (deg * 2 ** lev, deg * 2 ** lev)
+        scale = np.max(np.abs(ref))
+        assert np.max(np.abs(M[::stride] - ref)) <= 1e-12 * scale  # the deg-6 filters reach 5.6e-13
+        assert nodes.min() > a and nodes.max() < b
+        if (a, b) != doms[0]:
+            bad, _ = K.matrix_plot_D(deg, lev, *doms[0])
+            assert np.max(np.abs(bad[::stride] - ref)) > 1e-3 * scale
+
+
+@pytest.mark.parametrize("deg,lev,a,b", [(2, 3, -1.0, 1.0), (4, 4, -2.0, 5.0), (7, 2, 0.0, 3.0)])
+def test_matrix_plot_D_gauss_orthonormality(deg, lev, a, b):
+    """The wavelet basis is orthonormal on [a, b], and deg Gauss points per cell
+    integrate products of degree < deg polynomials exactly, so with the Gauss
+    weights (cell width / 2 times the reference-cell weights) Meval^T W Meval = I."""
+    M, _ = K.matrix_plot_D(deg, lev, a, b)
+    _, w = K.lgwt(deg, -1.0, 1.0)
+    h = (b - a) / 2 ** lev
+    W = np.tile(w * h / 2, 2 ** lev)
+    assert np.max(np.abs(M.T @ (W[:, None] * M) - np.eye(M.shape[1]))) <= 1e-11  # deg-7 filters: 3.8e-12
+
+
 def test_lgwt(gold):
     for n, a, b, tag in [(10, -1, 1, "10_neg1_1"), (32, -5, 2, "32_neg5_2")]:
         x, w = K.lgwt(n, a, b)

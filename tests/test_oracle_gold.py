@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from kmx import coeffs as K
 
 
 # ---------------------------------------------------------------- PERMUTE/
@@ -243,3 +244,27 @@ def test_wavelet_to_realspace_closed_form(nd, lev, deg):
     bad = idx.copy()
     bad[-1, 0] = bad[-1, 0] % (2 ** lev) + 1
     assert np.linalg.norm(O.wavelet_to_realspace(meval, bad, deg, fw) - ref) > 1e-6 * np.linalg.norm(ref)
+
+
+# gold_wavelet_to_realspace.m:1-3 (lev, deg), max_lev 8 (OPTS.m:24), sparse grid;
+# domains pdes/continuity1.m:16, continuity2.m:27-29, continuity3.m:24-34
+REALSPACE_GOLD = [("continuity_1", 7, 8, [(-1, 1)]),
+                  ("continuity_2", 5, 4, [(-1, 1), (-2, 2)]),
+                  ("continuity_3", 4, 3, [(-1, 1), (-2, 2), (-3, 3)])]
+
+
+@pytest.mark.parametrize("name,deg,lev,doms", REALSPACE_GOLD)
+def test_wavelet_to_realspace_gold(gold, name, deg, lev, doms):
+    """wavelet_to_realspace.m end to end against the reference's gold vectors: the
+    evaluation matrices of matrix_plot_D.m (restated in kmx.coeffs), the new-hash
+    element table and f = 2 * (0, 1, ..., N deg^d - 1) (gold_wavelet_to_realspace.m:
+    36-42).  The 3D vector is stored as every 8th value."""
+    key = f"transformations/wavelet_to_realspace_{name}"
+    stride = int(gold[key + "/stride"][0])
+    ref = gold[key].ravel()
+    idx = O.hash_table_sparse_nd([lev] * len(doms), 8, "SG")["idx1d"]
+    meval = [K.matrix_plot_D(deg, lev, a, b)[0] for (a, b) in doms]
+    fw = 2.0 * np.arange(idx.shape[0] * deg ** len(doms))
+    fr = O.wavelet_to_realspace(meval, idx, deg, fw)
+    assert fr.size == (deg * 2 ** lev) ** len(doms)
+    assert np.linalg.norm(fr[::stride] - ref) <= 1e-13 * np.linalg.norm(ref)
