@@ -74,3 +74,27 @@ def test_asgard_mirror_wavelet_to_realspace(kmx_gpu):
     fw = rng.standard_normal(table.num_elem * 4)
     fr = A.wavelet_to_realspace(pde, opts, meval, fw, table)
     assert rel(fr, O.wavelet_to_realspace(meval, table.idx1d, 2, fw)) <= 1e-13
+
+
+# gold_wavelet_to_realspace.m:1,3 (lev, deg); max_lev 8 (OPTS.m:24); domains
+# pdes/continuity1.m:16, continuity3.m:24-34
+@pytest.mark.parametrize("name,deg,lev,doms", [("continuity_1", 7, 8, [(-1, 1)]),
+                                               ("continuity_3", 4, 3, [(-1, 1), (-2, 2), (-3, 3)])])
+def test_asgard_realspace_matches_gold(kmx_gpu, gold, name, deg, lev, doms):
+    """The reference-named entries end to end on the GPU: matrix_plot_D's evaluation
+    matrices, then wavelet_to_realspace of f = 2 * (0, 1, ..., N deg^d - 1)
+    (gold_wavelet_to_realspace.m:36-42) against the reference's gold vector (the
+    3D one stored as every 8th value)."""
+    from kmx import asgard as A
+    from kmx import coeffs as K
+    from kmx import grid as G
+
+    opts = A.OPTS(lev=lev, deg=deg)
+    pde = K.PDE(dims=[K.Dimension(a, b, lev) for (a, b) in doms], terms=[])
+    table = G.hash_table_sparse_nd([lev] * len(doms), 8, "SG")
+    meval = [A.matrix_plot_D(pde, opts, dim)[0] for dim in pde.dims]
+    fw = 2.0 * np.arange(table.num_elem * deg ** len(doms))
+    fr = A.wavelet_to_realspace(pde, opts, meval, fw, table)
+    key = f"transformations/wavelet_to_realspace_{name}"
+    ref = gold[key].ravel()
+    assert rel(fr[:: int(gold[key + "/stride"][0])], ref) <= 1e-13
