@@ -6,7 +6,8 @@ computed with the oracle apply (apply_A.m + time_advance.m end to end).
 The gold trajectories (gold/generated-inputs/time_advance/*, gold_time_advance.m)
 were written by a reference revision whose continuity PDEs used the central
 flux (LF = 0); the current pdes/continuity2.m passes LF = -1.  The runs below
-therefore set LF = 0; the LF != 0 flux terms are exercised by the GPU parity
+therefore set LF = 0; the LF != 0 flux term is pinned by its closed form on
+periodic meshes (test_div_flux_term_closed_form) and exercised by the GPU parity
 tests through the same builder."""
 import numpy as np
 import pytest
@@ -102,6 +103,35 @@ def test_operator_identities():
     m8 = K.restrict(K.get_coeff_mats(pde, 2, 0.0, 8), pde, 2)
     for a, b in zip(sum(m3, []), sum(m8, [])):
         assert np.max(np.abs(a - b)) <= 1e-11 * max(1.0, np.max(np.abs(a)))
+
+
+@pytest.mark.parametrize("deg,lev,a,b", [(2, 3, -1.0, 1.0), (3, 4, -2.0, 3.0), (4, 2, 0.0, 1.0)])
+def test_div_flux_term_closed_form(deg, lev, a, b):
+    """The Lax-Friedrichs part of coeff_matrix.m's DIV (periodic, constant g),
+    pinned by what the mathematics fixes rather than by gold files (none use
+    LF != 0): the central-flux matrix D0 is skew; S = D(LF) - D0 is symmetric,
+    linear in LF and in |g|, has rank = number of cell faces (one jump per face),
+    and for the upwind choice (g = -1, LF = -1) is negative semidefinite with its
+    most negative eigenvalue -|g| |LF| deg (deg + 1) / h: orthonormal Legendre
+    traces r_k = sqrt((2k+1)/h) and l_k = (-1)^k r_k give, in the Fourier mode of
+    the periodic mesh that maximises it, |g| |LF| / 2 * (|r|^2 + |l|^2 + 2 |r.l|)."""
+    dim = K.Dimension(a, b, lev)
+
+    def D(g, LF):
+        return K.coeff_matrix(deg, 0.0, dim, K.DIV(K._const(g), LF, "P", "P"), {}, lev)[0]
+
+    D0 = D(-1.0, 0.0)
+    S = D(-1.0, -1.0) - D0
+    tol = 1e-12 * np.abs(S).max()
+    assert np.abs(D0 + D0.T).max() <= 1e-11
+    assert np.abs(S - S.T).max() <= tol
+    assert np.abs((D(-1.0, 1.0) - D0) + S).max() <= tol
+    assert np.abs((D(-2.0, -1.0) - D(-2.0, 0.0)) - 2 * S).max() <= 2 * tol
+    assert np.linalg.matrix_rank(S, tol=1e-9 * np.abs(S).max()) == 2 ** lev
+    ev = np.linalg.eigvalsh(S)
+    h = (b - a) / 2 ** lev
+    assert ev.max() <= 1e-11
+    assert abs(ev.min() + deg * (deg + 1) / h) <= 1e-11 * deg * (deg + 1) / h
 
 
 def _setup(pde, deg, grid_type="SG", max_lev=8):
