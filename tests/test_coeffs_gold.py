@@ -134,6 +134,43 @@ def test_div_flux_term_closed_form(deg, lev, a, b):
     assert abs(ev.min() + deg * (deg + 1) / h) <= 1e-11 * deg * (deg + 1) / h
 
 
+def _project(u, deg, lev, a, b):
+    """Real-space DG coefficients of u: per cell, Gauss quadrature of u times the
+    orthonormal Legendre basis sqrt(1/h) P~_k (exact for the polynomials below)."""
+    n = 2 ** lev
+    h = (b - a) / n
+    qx, qw = K.lgwt(deg + 4, -1.0, 1.0)
+    phi = K.lin_legendre(qx, deg) * np.sqrt(1.0 / h)
+    out = np.zeros(deg * n)
+    for i in range(n):
+        out[deg * i:deg * (i + 1)] = (qw * u(a + (qx + 1) / 2 * h + i * h)) @ phi * h / 2
+    return out
+
+
+@pytest.mark.parametrize("deg,lev,a,b", [(3, 3, -1.0, 1.0), (4, 2, 0.0, 2.0)])
+@pytest.mark.parametrize("LF", [0.0, -1.0])
+def test_div_boundary_branches_exact_on_polynomials(deg, lev, a, b, LF):
+    """coeff_matrix.m:212-262, the homogeneous parts of the Neumann and Dirichlet
+    boundary fluxes (the inhomogeneous values go to boundary_condition_vector.m,
+    not restated): with 'N' the boundary face takes the interior trace and with 'D'
+    it takes zero, both consistent, so for constant g the DG derivative is exact
+    on every polynomial of degree < deg (N) or on one vanishing at both ends (D):
+    D P(u) = P(g u').  A periodic matrix on a non-periodic u is not exact (the
+    check discriminates the boundary handling)."""
+    g = 1.5
+    dim = K.Dimension(a, b, lev)
+    cases = {"N": (lambda x: x ** 2 - 0.3 * x, lambda x: 2 * x - 0.3),
+             "D": (lambda x: (x - a) * (b - x), lambda x: (b - x) - (x - a))}
+    for bc, (u, du) in cases.items():
+        R = K.coeff_matrix(deg, 0.0, dim, K.DIV(K._const(g), LF, bc, bc), {}, lev)[1]
+        ref = _project(lambda x: g * du(x), deg, lev, a, b)
+        assert np.abs(R @ _project(u, deg, lev, a, b) - ref).max() <= 1e-12 * np.abs(ref).max(), bc
+    u, du = cases["N"]
+    Rp = K.coeff_matrix(deg, 0.0, dim, K.DIV(K._const(g), LF, "P", "P"), {}, lev)[1]
+    ref = _project(lambda x: g * du(x), deg, lev, a, b)
+    assert np.abs(Rp @ _project(u, deg, lev, a, b) - ref).max() > 1e-3 * np.abs(ref).max()
+
+
 def _setup(pde, deg, grid_type="SG", max_lev=8):
     lev_vec = [d.lev for d in pde.dims]
     tab = G.hash_table_sparse_nd(lev_vec, max_lev, grid_type)
