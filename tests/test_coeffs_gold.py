@@ -60,6 +60,22 @@ def test_matrix_plot_D_matches_gold(gold, name, deg, lev, doms):
             assert np.max(np.abs(bad[::stride] - ref)) > 1e-3 * scale
 
 
+def test_asgard_matrix_plot_D_mirror():
+    """The reference-named entry (matrix_plot_D(pde, opts, dimension)) reads deg
+    from opts and lev / domain from the dimension; output grids it does not
+    restate raise instead of returning another grid's matrix."""
+    from kmx import asgard as A
+
+    opts = A.OPTS(lev=3, deg=3)
+    dim = K.Dimension(-2.0, 2.0, 4)
+    M, nodes = A.matrix_plot_D(None, opts, dim)
+    M2, nodes2 = K.matrix_plot_D(3, 4, -2.0, 2.0)
+    assert np.array_equal(M, M2) and np.array_equal(nodes, nodes2)
+    opts.output_grid = "uniform"
+    with pytest.raises(NotImplementedError):
+        A.matrix_plot_D(None, opts, dim)
+
+
 @pytest.mark.parametrize("deg,lev,a,b", [(2, 3, -1.0, 1.0), (4, 4, -2.0, 5.0), (7, 2, 0.0, 3.0)])
 def test_matrix_plot_D_gauss_orthonormality(deg, lev, a, b):
     """The wavelet basis is orthonormal on [a, b], and deg Gauss points per cell
