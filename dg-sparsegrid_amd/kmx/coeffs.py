@@ -297,12 +297,40 @@ def coeff_matrix(deg, t, dim, pterm, params, lev):
     return F @ unrot @ F.T, unrot
 
 
+_PTERM_CACHE = {}
+
+
+def _params_key(params):
+    try:
+        return tuple(sorted(params.items()))
+    except TypeError:
+        return ("id", id(params))
+
+
 def pterm_coeff_matrix(deg, t, dim, pterm, params, lev):
     """pterm_coeff_matrix.m:1-26: the pterm's RHS matrix L moved to the RHS with
-    its own LHS mass matrix M: mat = M \\ L."""
+    its own LHS mass matrix M: mat = M \\ L.
+
+    The result is a function of the pterm's fields, the dimension's domain and
+    moment_dV, deg, lev, t and params only, so it is built once per distinct
+    combination and reused (the functions g / LHS_mass_g / dV / moment_dV are
+    keyed by identity and kept alive by the cache, so an id is never reused
+    while its entry exists).  Callers get read-only arrays."""
+    fns = (pterm.g, pterm.LHS_mass_g, pterm.dV, dim.moment_dV)
+    key = (deg, float(t), float(dim.min), float(dim.max), lev, pterm.type, float(pterm.LF), pterm.BCL, pterm.BCR,
+           pterm.IBCL, pterm.IBCR, tuple(id(f) for f in fns), _params_key(params))
+    hit = _PTERM_CACHE.get(key)
+    if hit is not None and all(a is b for a, b in zip(hit[0], fns)):
+        return hit[1]
     L, L_un = coeff_matrix(deg, t, dim, pterm, params, lev)
     M, M_un = coeff_matrix(deg, t, dim, MASS(pterm.LHS_mass_g, None, dim.moment_dV), params, lev)
-    return np.linalg.solve(M, L), np.linalg.solve(M_un, L_un), M
+    out = (np.linalg.solve(M, L), np.linalg.solve(M_un, L_un), M)
+    for a in out:
+        a.setflags(write=False)
+    if len(_PTERM_CACHE) > 512:
+        _PTERM_CACHE.clear()
+    _PTERM_CACHE[key] = (fns, out)
+    return out
 
 
 def sd_term_coeff_matrix(deg, t, dim, pterms, params, lev):
@@ -376,7 +404,9 @@ def md_eval_function(pde, deg, idx1d, md_funcs, t):
 
 
 # ------------------------------------------------------------- PDE specs
+@functools.lru_cache(maxsize=None)
 def _const(c):
+    """The constant function c (one object per value, so equal pterms share their matrices)."""
     return lambda x, p=None, t=None, dat=None: np.zeros_like(np.asarray(x, dtype=np.float64)) + c
 
 
@@ -448,6 +478,16 @@ def diffusion_nd(nd=2, lev=3):
     return PDE(dims, terms, [], sol, sol, {"parameter1": 0, "parameter2": 1}, set_dt, f"diffusion{nd}")
 
 
+@functools.lru_cache(maxsize=None)
+def _varcoeff_funcs(nu):
+    """g(x) = 1 + 0.5 sin(pi x), -g and nu g (one set of objects per nu, so PDEs
+    of any dimension share their 1D matrices)."""
+    gvar = lambda x, p=None, t=None, dat=None: 1.0 + 0.5 * np.sin(np.pi * np.asarray(x, dtype=np.float64))  # noqa: E731
+    gneg = lambda x, p=None, t=None, dat=None: -gvar(x)  # noqa: E731
+    gdif = lambda x, p=None, t=None, dat=None: nu * gvar(x)  # noqa: E731
+    return gvar, gneg, gdif
+
+
 def advection_diffusion_varcoeff(nd=3, lev=3, nu=0.02):
     """Variable-coefficient advection-diffusion on [-1,1]^nd, periodic: per dim
     an upwind DIV(-g_i, LF=-1) term and an LDG diffusion chain
@@ -456,9 +496,7 @@ def advection_diffusion_varcoeff(nd=3, lev=3, nu=0.02):
     advective CFL stable (|lambda_diff| ~ nu * 2.8e6 at lev 5, deg 3)."""
     levs = np.broadcast_to(np.asarray(lev), (nd,))
     dims = [Dimension(-1, 1, int(levs[d]), name=f"x{d + 1}") for d in range(nd)]
-    gvar = lambda x, p=None, t=None, dat=None: 1.0 + 0.5 * np.sin(np.pi * np.asarray(x, dtype=np.float64))  # noqa: E731
-    gneg = lambda x, p=None, t=None, dat=None: -gvar(x)  # noqa: E731
-    gdif = lambda x, p=None, t=None, dat=None: nu * gvar(x)  # noqa: E731
+    gvar, gneg, gdif = _varcoeff_funcs(float(nu))
     terms = []
     for k in range(nd):
         terms.append([[DIV(gneg, -1, "P", "P")] if d == k else [MASS()] for d in range(nd)])
