@@ -2,23 +2,22 @@
 """Operator-apply benchmark on one MI355X.
 
 Headline workload: 64 independent adapted 2D/3D advection-diffusion
-instances (levels 4-8 in 2D / 4-6 in 3D, deg 2-4, term scales U[0.5, 2],
-one seeded refinement sweep keeping each child with probability 0.6, all
-element pairs: 63.1 M pairs), advanced by an RK3 step replayed from a HIP
-graph.  A "step" is one RK3 step of every instance (three batched applies and
+instances (levels 4-8, deg 2-4, term scales U[0.5, 2], one seeded refinement
+sweep keeping each child with probability 0.6, all element pairs: 608 M
+pairs), advanced by an RK3 step replayed from a HIP graph.  A "step" is one RK3 step of every instance (three batched applies and
 the FP64 stage updates); `value` is instance-applies/s with the applies in
 mixed precision (FP32 on the matrix cores) on the merged dimension sweeps,
 `speedup_vs_fp64_plain` its ratio to the same steps with the plain FP64 pair
 path, and `rk3_rel_l2_vs_fp64_plain` the final-state error between the two.
 
-Also reported: the same batching config over its full level range (3D levels
-up to 8, merged sweeps only; --no-full-range skips), the single 3D operator (SG
+Also reported: FP32 against FP64 on the same merged sweeps (the precision gain
+alone), the single 3D operator (SG
 lev 7, deg 3, 3 terms, connect_nD, P = 1,903,000) in FP64 / FP32 pair by pair
 and through the dimension sweep (--no-single skips), the 6D lev 4 deg 2 precision study with FP32 / BF16
 errors and a refined backward Euler step (--no-precision skips), and the CPU
 oracle on the host cores for both workloads (--no-cpu skips).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-single] [--no-precision] [--no-full-range]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-single] [--no-precision]
 
 Prints ONE JSON line.
 """
@@ -59,7 +58,18 @@ MEASURED_TRAFFIC = {
 # (its three launches together: box copies of x, every dimension's op, slot
 # sums into y), from the same FETCH_SIZE / WRITE_SIZE counter runs and scale
 # factors.  Source: profiles/batch_sweep_hbm_traffic.md.
-MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = 345246106
+MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = None
+TRAFFIC_SOURCE = None
+
+
+def host_threads():
+    """Threads for the CPU baseline: every core this process may run on, bounded by
+    OMP_NUM_THREADS when the host sets it (the GPU box allots 16 cores per GPU)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
 
 
 def build_operator(seed=2):
@@ -108,7 +118,7 @@ def cpu_baseline(op, budget_s=12.0):
     """The oracle (C++ FP64 restatement of apply_A.m) on the host cores."""
     import oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     inp = oracle.ApplyInputs(op["mats"], op["table"].idx1d, op["deg"], op["rp"], op["cols"])
     y = oracle.apply_A(inp, op["x"], nthreads=threads)  # warm-up + reference result
     n, t0 = 0, time.perf_counter()
@@ -130,32 +140,23 @@ def _cells(l):
     return 1 if l == 0 else 1 << (l - 1)
 
 
-def separable_sweep_model(inst, elem):
+def separable_sweep_model(plans, elem):
     """Work model of one batched apply on separable sweeps (every term acting in
-    one dimension): per instance and dimension one GEMM per group of level
-    tuples agreeing off that dimension (2 n^2 c flops: n positions of the group's
-    levels along the dimension, c columns), reading the dimension's combined 1D
-    matrix once (its largest leading block), the box copy of x once and writing
-    its slot once; plus the box copy of x (map, x, copy) and the slot sum (map,
-    slots, y).  Returns (flops, algorithmic bytes)."""
-    flops = nbytes = 0
-    for it in inst:
-        D, deg = it["nd"], it["deg"]
-        S = set(map(tuple, np.asarray(it["table"].lev).reshape(-1, D).tolist()))
-        nb = sum(int(np.prod([deg * _cells(l) for l in L])) for L in S)
-        for d in range(D):
-            rests = {}
-            for L in S:
-                R = L[:d] + L[d + 1:]
-                rests[R] = max(rests.get(R, -1), L[d])
-            for R, m in rests.items():
-                n = deg << m
-                flops += 2 * n * n * int(np.prod([deg * _cells(l) for l in R]))
-            nmax = deg << max(L[d] for L in S)
-            nbytes += nmax * nmax * elem + 2 * nb * elem
-        dof = it["table"].num_elem * deg ** D
+    one dimension), from the plans' own sweep set-up (kmx_plan_sweep_info): the
+    useful flops (kept blocks x group columns), and the algorithmic bytes -- each
+    dimension's block-sparse combined operator read once (its stored values), the
+    box copy of x read and the dimension's slot written once per dimension, the
+    box copy of x (map, x, copy) and the slot sum (map, slots, y).  Returns
+    (flops, algorithmic bytes, operator bytes)."""
+    flops = nbytes = opbytes = 0
+    for p in plans:
+        info = p.sweep_info()
+        D, nb, dof = p.ndim, info["box_values"], p.dof
+        flops += info["flops"]
+        opbytes += info["operator_values"] * elem
+        nbytes += info["operator_values"] * elem + D * 2 * nb * elem
         nbytes += nb * (4 + elem) + dof * 8 + nb * (4 + D * elem) + dof * 8
-    return flops, nbytes
+    return flops, nbytes, opbytes
 
 
 def time_batch_apply(sysm, prec, warmup, steps, per_graph=10):
@@ -189,14 +190,31 @@ def time_batch_apply(sysm, prec, warmup, steps, per_graph=10):
     return e0.elapsed_time(e1) / (reps * per_graph)
 
 
-def batched_leg(steps=50, warmup=5, inst=None):
-    """64 adapted 2D/3D advection-diffusion instances (all element pairs), one
-    batched apply per RK3 stage, whole step replayed from a HIP graph.  Runs:
-    "fp64" / "fp32" = every instance pair by pair in the persistent launch (the
-    FP64 one is the plain baseline); "fp32_sweep" / "fp64_sweep" = every instance
-    on the merged dimension sweeps (level-block plans, one launch per phase for
-    all instances; same terms, other summation order).  Errors: final state of
-    each run against the FP64 pair-path state after the same steps."""
+def timed_steps(g, n):
+    """ms per RK3 step over n graph replays (HIP events on the replay stream)."""
+    import torch
+
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(n):
+        g.step()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / n
+
+
+def batched_leg(steps=50, warmup=5, inst=None, err_steps=6):
+    """64 adapted 2D/3D advection-diffusion instances (levels 4-8, all element
+    pairs), one batched apply per RK3 stage, whole step replayed from a HIP graph.
+    Runs: "fp64" / "fp32" = every instance pair by pair in the persistent launch
+    (the FP64 one is the plain baseline); "fp32_sweep" / "fp64_sweep" = every
+    instance on the merged block-sparse separable sweeps (same terms, other
+    summation order).  Every run first advances err_steps steps from the same
+    initial states (timed: the pair paths' figure); the final states are compared
+    with the FP64 pair path's.  The sweep runs then take warmup + steps further
+    steps, the timed steps giving their figure."""
     import torch
 
     from kmx import batch as BT
@@ -206,87 +224,52 @@ def batched_leg(steps=50, warmup=5, inst=None):
         inst = BT.make_instances(count=64, seed=0)
     out = {"instances": len(inst), "build_s": round(time.perf_counter() - t0, 1), "runs": {}}
     states = {}
-    for name, prec, sweep in (("fp64", "fp64", None), ("fp32", "fp32", None),
-                              ("fp32_sweep", "fp32", ("fp32",)), ("fp64_sweep", "fp64", ("fp64",))):
+    plans_sweep = None
+    for name, prec, sweep in (("fp32_sweep", "fp32", ("fp32",)), ("fp64_sweep", "fp64", ("fp64",)),
+                              ("fp64", "fp64", None), ("fp32", "fp32", None)):
+        t1 = time.perf_counter()
         sysm = BT.BatchedSystem(inst, sweep=sweep)
-        apply_ms = time_batch_apply(sysm, prec, 3, max(10, steps // 2))
+        setup_s = time.perf_counter() - t1
+        pairs = sweep is None
+        apply_ms = time_batch_apply(sysm, prec, 1 if pairs else 3, 3 if pairs else max(10, steps // 2),
+                                    per_graph=1 if pairs else 10)
         g = BT.BatchRK3Graph(sysm, prec=prec)
-        for _ in range(warmup):
-            g.step()
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(steps):
-            g.step()
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / steps
+        ms_err = timed_steps(g, err_steps)
         states[name] = sysm.f.cpu().numpy()
-        fl = sysm.flops_by_prec[prec]
-        run = {"ms_per_rk3_step": round(ms, 4),
-               "instance_applies_per_s": round(3 * len(inst) / (ms * 1e-3), 1),
-               "ms_per_batched_apply": round(apply_ms, 4),
-               "flops_per_batched_apply": fl,
-               "TFLOPs_in_apply": round(fl / (apply_ms * 1e-3) / 1e12, 3)}
-        if sweep:
+        run = {"setup_s": round(setup_s, 1), "ms_per_batched_apply": round(apply_ms, 4),
+               "flops_per_batched_apply": sysm.flops_by_prec[prec]}
+        if pairs:
+            run["ms_per_rk3_step"] = round(ms_err, 4)
+            run["timed_steps"] = err_steps
+        else:
+            for _ in range(warmup):
+                g.step()
+            ms = timed_steps(g, steps)
+            run["ms_per_rk3_step"] = round(ms, 4)
+            run["timed_steps"] = steps
             run["merged_sweep"] = sysm.sweep[prec]
+            if plans_sweep is None:
+                plans_sweep = sysm
+        run["instance_applies_per_s"] = round(3 * len(inst) / (run["ms_per_rk3_step"] * 1e-3), 1)
+        run["TFLOPs_in_apply"] = round(run["flops_per_batched_apply"] / (apply_ms * 1e-3) / 1e12, 3)
         out["runs"][name] = run
         out["pairs"] = int(sysm.pairs)
         out["dof"] = int(sysm.dof)
-        del g, sysm
+        out["pair_path_flops_per_apply"] = int(sysm.flops)
+        del g
+        if sysm is not plans_sweep:
+            del sysm
+        print(f"bench: batched {name}: {run}", file=sys.stderr, flush=True)
     for name in ("fp32", "fp32_sweep", "fp64_sweep"):
         d = states[name] - states["fp64"]
         out["runs"][name]["rk3_rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(states["fp64"]))
         out["runs"][name]["speedup_vs_fp64_plain"] = round(
             out["runs"]["fp64"]["ms_per_rk3_step"] / out["runs"][name]["ms_per_rk3_step"], 3)
-    out["rk3_steps"] = steps + warmup
-    return out, inst
-
-
-def batched_full_range_leg(steps=20, warmup=3):
-    """The batching config over its full level range: 64 adapted instances with
-    levels 4-8 in 2D and in 3D (make_instances seed 0, lev3d=(4, 8); 3D level-8
-    instances reach N = 8,869 elements and 78.7 M element pairs each).  Only the
-    merged dimension sweeps run: FP32 (timed) and FP64 (its reference
-    trajectory); the plain FP64 pair path would cost ~0.6 TFLOP per batched apply."""
-    import torch
-
-    from kmx import batch as BT
-
-    t0 = time.perf_counter()
-    inst = BT.make_instances(count=64, seed=0, lev3d=(4, 8))
-    out = {"instances": len(inst), "build_s": round(time.perf_counter() - t0, 1), "runs": {}}
-    states = {}
-    for name, prec in (("fp32_sweep", "fp32"), ("fp64_sweep", "fp64")):
-        sysm = BT.BatchedSystem(inst, sweep=(prec,))
-        apply_ms = time_batch_apply(sysm, prec, 3, max(10, steps // 2))
-        g = BT.BatchRK3Graph(sysm, prec=prec)
-        for _ in range(warmup):
-            g.step()
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(steps):
-            g.step()
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / steps
-        states[name] = sysm.f.cpu().numpy()
-        out["runs"][name] = {"ms_per_rk3_step": round(ms, 4),
-                             "instance_applies_per_s": round(3 * len(inst) / (ms * 1e-3), 1),
-                             "ms_per_batched_apply": round(apply_ms, 4),
-                             "flops_per_batched_apply": sysm.flops_by_prec[prec],
-                             "merged_sweep": sysm.sweep[prec]}
-        out["pairs"] = int(sysm.pairs)
-        out["dof"] = int(sysm.dof)
-        out["pair_path_flops_per_apply"] = int(sysm.flops)
-        del g, sysm
-    d = states["fp32_sweep"] - states["fp64_sweep"]
-    out["runs"]["fp32_sweep"]["rk3_rel_l2_vs_fp64_sweep"] = float(np.linalg.norm(d) / np.linalg.norm(states["fp64_sweep"]))
-    out["rk3_steps"] = steps + warmup
-    return out
+    # the precision gain alone: FP32 against FP64 on the same merged sweeps
+    out["fp32_over_fp64_same_sweep"] = round(
+        out["runs"]["fp64_sweep"]["ms_per_rk3_step"] / out["runs"]["fp32_sweep"]["ms_per_rk3_step"], 3)
+    out["rk3_error_steps"] = err_steps
+    return out, inst, plans_sweep
 
 
 def cpu_baseline_batch(inst, budget_s=12.0):
@@ -294,7 +277,7 @@ def cpu_baseline_batch(inst, budget_s=12.0):
     applying every instance of the batch on the host cores: instance-applies/s."""
     import oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     inputs = [oracle.ApplyInputs([[it["scales"][t] * it["mats"][t][0]] + list(it["mats"][t][1:])
                                   for t in range(len(it["mats"]))], it["table"].idx1d, it["deg"]) for it in inst]
     n, passes, t0 = 0, 0, time.perf_counter()
@@ -454,8 +437,6 @@ def main():
                     help="skip the 64-instance legs (profiling runs of the single-operator kernels: "
                          "the line then has no value)")
     ap.add_argument("--no-precision", action="store_true", help="skip the 6D precision leg")
-    ap.add_argument("--no-full-range", action="store_true",
-                    help="skip the 64-instance leg over the full level range (3D levels up to 8, sweeps only)")
     ap.add_argument("--sweep-study", action="store_true",
                     help="only time the pair kernels against the opt-in dimension-sweep path (one JSON line)")
     ap.add_argument("--pmc-pass", action="store_true",
@@ -525,32 +506,29 @@ def main():
         return
     # the headline: 64 adapted instances, mixed precision (FP32) on the merged
     # dimension sweeps, against the plain FP64 pair path on the same batch
-    batch, inst = batched_leg(steps=a.steps, warmup=a.warmup)
+    batch, inst, sweep_sys = batched_leg(steps=a.steps, warmup=a.warmup)
     head = batch["runs"]["fp32_sweep"]
     ms = head["ms_per_rk3_step"]
     apply_ms = head["ms_per_batched_apply"]
     tflops = head["flops_per_batched_apply"] / (apply_ms * 1e-3) / 1e12
+    mflops, alg_bytes, op_bytes = separable_sweep_model(sweep_sys.plans, 4)
+    achieved = alg_bytes / (apply_ms * 1e-3) / 1e9
     traffic = MEASURED_TRAFFIC_BATCH_SWEEP_FP32
-    mflops, alg_bytes = separable_sweep_model(inst, 4)
-    if mflops == head["flops_per_batched_apply"]:  # every instance on the separable sweep: HBM-bound
-        achieved = alg_bytes / (apply_ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "algorithmic_bytes_per_apply": alg_bytes,
-                    "intensity_flops_per_byte": round(mflops / alg_bytes, 3)}
-    else:
-        roofline = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": traffic}
-    roofline.update({
-        "traffic_GBs": round(traffic / (apply_ms * 1e-3) / 1e9, 1) if traffic else None,
-        "kernel": "one FP32 batched apply = k_box_gather_batch<float> + k_sep_batch<float> "
-                  "(v_mfma_f32_16x16x4_f32; k_ud_batch<float,*> for general sweeps) + k_box_sum_batch<float>; "
-                  "achieved = algorithmic bytes (or flops) per apply / apply time; traffic = measured HBM "
-                  "bytes per apply",
-        "flops_per_apply": head["flops_per_batched_apply"],
-        "mfma_TFLOPs": round(tflops, 3),
-        "ms_per_apply": apply_ms,
-    })
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": TRAFFIC_SOURCE,
+                "algorithmic_bytes_per_apply": alg_bytes, "operator_bytes_per_apply": op_bytes,
+                "intensity_flops_per_byte": round(mflops / alg_bytes, 3),
+                "traffic_GBs": round(traffic / (apply_ms * 1e-3) / 1e9, 1) if traffic else None,
+                "kernel": "one FP32 batched apply = k_box_gather_batch<float> + k_sep_batch<float> "
+                          "(v_mfma_f32_16x16x4_f32 over block-sparse row panels) + k_box_sum_batch<float>; "
+                          "achieved = algorithmic bytes per apply / apply time (HIP events around graph-replayed "
+                          "applies on the launch stream)",
+                "flops_per_apply": mflops,
+                "mfma_flops_per_apply": sum(p.sweep_info()["mfma_flops"] for p in sweep_sys.plans),
+                "mfma_TFLOPs": round(tflops, 3),
+                "ms_per_apply": apply_ms}
+    del sweep_sys
     out = {
         "metric": METRIC,
         "value": head["instance_applies_per_s"],
@@ -564,19 +542,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: 64 seeded adapted instances (make_instances seed 0), initial states ~ N(0,1)",
-        "config": {"workload": "64 adapted 2D/3D advection-diffusion instances, lev 4-8 (2D) / 4-6 (3D), "
+        "config": {"workload": "64 adapted 2D/3D advection-diffusion instances, lev 4-8 (2D and 3D), "
                                "deg 2-4, term scales U[0.5,2], refinement keep prob 0.6, all element pairs "
                                "(%d pairs, %d dof); one RK3 step (3 batched applies) per step, HIP graph"
                                % (batch["pairs"], batch["dof"]),
-                   "variant": "fp32 merged dimension sweeps (matrix cores), FP64 state and RK3 update",
+                   "variant": "fp32 merged block-sparse separable sweeps (matrix cores), FP64 state and RK3 update",
                    "instances": batch["instances"]},
         "speedup_vs_fp64_plain": head["speedup_vs_fp64_plain"],
         "rk3_rel_l2_vs_fp64_plain": head["rk3_rel_l2_vs_fp64"],
+        "fp32_over_fp64_same_sweep": batch["fp32_over_fp64_same_sweep"],
         "batched": batch,
         "roofline": roofline,
     }
-    if not a.no_full_range:
-        out["batched_full_range"] = batched_full_range_leg()
     if not a.no_single:
         single, y64 = single_instance_leg(plan, tuned, op, xd, yd, steps=200, warmup=10)
         out["single_instance"] = single

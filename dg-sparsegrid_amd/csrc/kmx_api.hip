@@ -4,6 +4,7 @@
 #include "kmx_internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -201,6 +202,26 @@ int kmx_plan_create(const kmx_plan_desc* desc, kmx_plan** out) {
             p->d_blk32 = upload(f.data(), f.size());
         }
         p->d_idx1d = upload(idx.data(), idx.size());
+        p->h_idx1d = idx;
+        // factors declared the identity: checked against the matrices given
+        if (desc->identity) {
+            p->declared_identity.assign(desc->identity, desc->identity + size_t(T) * D);
+            for (int t = 0; t < T; ++t)
+                for (int k = 0; k < D; ++k) {
+                    if (!p->declared_identity[size_t(t) * D + k]) continue;
+                    const double* M = desc->mats[t * D + k];
+                    const long long n = p->n[k];
+                    double dev = 0.0;
+                    for (long long c = 0; c < n; ++c)
+                        for (long long r = 0; r < n; ++r) dev = std::max(dev, std::fabs(M[c * n + r] - (r == c ? 1.0 : 0.0)));
+                    if (dev > 1e-8) {
+                        char msg[160];
+                        std::snprintf(msg, sizeof msg, "term %d dim %d is declared the identity but max |A - I| = %.3g",
+                                      t, k, dev);
+                        throw ArgError(msg);
+                    }
+                }
+        }
         std::vector<int> ord = lpt_order(rp, N);
         p->d_row_order = upload(ord.data(), ord.size());
         if (!p->all_pairs) {
@@ -352,6 +373,7 @@ int kmx_plan_update_term(kmx_plan* p, int term, int dim, const double* mat, void
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         kmx_blocks_update_term(p, term, dim, mat, s);
         kmx_sweep_term_updated(p, term, dim);
+        ++p->version;
         return KMX_OK;
     } catch (const std::exception& e) {
         set_error(std::string("kmx_plan_update_term: ") + e.what());
@@ -365,6 +387,7 @@ int kmx_plan_set_term_scale(kmx_plan* p, int term, double alpha) {
         return KMX_E_ARG;
     }
     p->scale[term] = alpha;
+    ++p->version;
     return kmx_sweep_scales_updated(p, term);
 }
 

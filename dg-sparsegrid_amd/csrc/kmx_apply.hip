@@ -328,7 +328,7 @@ extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
             rc = set_split(p, prec, bestb);
             if (rc != KMX_OK) break;
             // 3. the dimension-sweep path against the best of the above
-            if (p->sw) {
+            if (p->sw && (prec == KMX_PREC_FP32 || kmx_sweep_fp64_ok(p))) {
                 p->var[prec_slot(prec)].sweep = 1;
                 float ms;
                 rc = time_calls([&] { return kmx_launch_apply(p, false, x, y, prec, s); }, s, e0, e1, ms);
@@ -372,7 +372,21 @@ extern "C" int kmx_plan_set_sweep(kmx_plan* p, int prec, int on) {
         set_error("kmx_plan_set_sweep: this plan has no dimension-sweep path");
         return KMX_E_UNSUPPORTED;
     }
+    if (on && prec == KMX_PREC_FP64 && !kmx_sweep_fp64_ok(p)) {
+        set_error("kmx_plan_set_sweep: the separable sweep takes near-identity factors as the identity, which "
+                  "FP64 applies would see (declare them in kmx_plan_desc.identity to allow it)");
+        return KMX_E_UNSUPPORTED;
+    }
     p->var[prec_slot(prec)].sweep = on ? 1 : 0;
+    return KMX_OK;
+}
+
+extern "C" int kmx_plan_sweep_info(const kmx_plan* p, kmx_sweep_info* info) {
+    if (!p || !info) {
+        set_error("kmx_plan_sweep_info: bad arguments");
+        return KMX_E_ARG;
+    }
+    kmx_sweep_get_info(p, info);
     return KMX_OK;
 }
 
@@ -381,7 +395,7 @@ extern "C" int kmx_plan_get_sweep(const kmx_plan* p, int prec, int* available, i
         set_error("kmx_plan_get_sweep: bad arguments");
         return KMX_E_ARG;
     }
-    if (available) *available = p->sw ? 1 : 0;
+    if (available) *available = (p->sw && (prec == KMX_PREC_FP32 || kmx_sweep_fp64_ok(p))) ? 1 : 0;
     if (active) *active = (p->sw && p->var[prec_slot(prec)].sweep) ? 1 : 0;
     if (flops) *flops = kmx_sweep_flops(p);
     return KMX_OK;
@@ -444,10 +458,13 @@ struct kmx_batch {
     std::vector<kmx_plan*> plans;
     BatchSet set[2];  // [0] FP64, [1] FP32
     bool has32 = true;
-    // merged dimension sweeps (kmx_batch_set_sweep): members[q] = plan index of io slot q
-    kmx_sweep_batch* sweep = nullptr;
-    std::vector<int> members;
+    // merged dimension sweeps per precision (kmx_batch_set_sweep): members[s][q] =
+    // plan index of io slot q
+    kmx_sweep_batch* sweep[2] = {nullptr, nullptr};
+    std::vector<int> members[2];
     bool sweep_on[2] = {false, false};
+    // plan versions the tables above were built from (kmx_plan::version)
+    std::vector<unsigned long long> version;
 };
 
 // The switch's register budget is that of its largest case: blocks of up to 27
@@ -471,7 +488,7 @@ static hipError_t build_set(kmx_batch* b, BatchSet& bs, bool fp64) {
     std::vector<long long> rp;
     std::vector<char> skip(b->plans.size(), 0);  // plans served by the merged sweeps
     if (b->sweep_on[fp64 ? 0 : 1])
-        for (int i : b->members) skip[size_t(i)] = 1;
+        for (int i : b->members[fp64 ? 0 : 1]) skip[size_t(i)] = 1;
     for (int i = 0; i < int(b->plans.size()); ++i) {
         kmx_plan* p = b->plans[i];
         if (!fp64 && !p->d_blk32) continue;
@@ -553,7 +570,59 @@ extern "C" int kmx_batch_create(kmx_plan* const* plans, int count, kmx_batch** o
         kmx_batch_destroy(b);
         return KMX_E_HIP;
     }
+    for (kmx_plan* p : b->plans) b->version.push_back(p->version);
     *out = b;
+    return KMX_OK;
+}
+
+// The batch's merged sweep for one precision: every plan whose sweep serves that
+// precision, each plan once (one arena per plan), at most KMX_BATCH_MAX.
+static int build_batch_sweep(kmx_batch* b, int slot) {
+    kmx_sweep_batch_free(b->sweep[slot]);
+    b->sweep[slot] = nullptr;
+    b->members[slot].clear();
+    std::vector<kmx_plan*> mp;
+    std::vector<int> mi;
+    for (int i = 0; i < int(b->plans.size()) && int(mi.size()) < KMX_BATCH_MAX; ++i) {
+        kmx_plan* p = b->plans[i];
+        if (!p->sw || (slot == 0 && !kmx_sweep_fp64_ok(p)) || (slot == 1 && !p->d_blk32)) continue;
+        if (std::find(mp.begin(), mp.end(), p) != mp.end()) continue;
+        mp.push_back(p);
+        mi.push_back(i);
+    }
+    if (mp.empty()) {
+        set_error("kmx_batch_set_sweep: no plan of the batch has a dimension-sweep path for this precision");
+        return KMX_E_UNSUPPORTED;
+    }
+    try {
+        b->sweep[slot] = kmx_sweep_batch_build(mp);
+    } catch (const std::exception& ex) {
+        set_error(std::string("kmx_batch_set_sweep: ") + ex.what());
+        return KMX_E_HIP;
+    }
+    b->members[slot] = mi;
+    return KMX_OK;
+}
+
+// Term matrices or scales of a plan changed since the batch's tables were built:
+// rebuild them from the plans' current state (synchronous; not under graph capture).
+static int refresh_batch(kmx_batch* b) {
+    bool changed = false;
+    for (size_t i = 0; i < b->plans.size(); ++i) changed = changed || b->plans[i]->version != b->version[i];
+    if (!changed) return KMX_OK;
+    for (int slot = 0; slot < 2; ++slot) {
+        if (b->sweep_on[slot]) {
+            const int rc = build_batch_sweep(b, slot);
+            if (rc != KMX_OK) return rc;
+        }
+        free_set(b->set[slot]);
+        const hipError_t e = slot == 0 ? build_set<double>(b, b->set[0], true) : build_set<float>(b, b->set[1], false);
+        if (e != hipSuccess) {
+            set_error(std::string("kmx_batch_apply: refreshing the batch: ") + hipGetErrorString(e));
+            return KMX_E_HIP;
+        }
+    }
+    for (size_t i = 0; i < b->plans.size(); ++i) b->version[i] = b->plans[i]->version;
     return KMX_OK;
 }
 
@@ -578,14 +647,19 @@ extern "C" int kmx_batch_apply(kmx_batch* b, const double* const* xs, double* co
             return KMX_E_ARG;
         }
     const int slot = prec == KMX_PREC_FP64 ? 0 : 1;
+    {
+        const int rc = refresh_batch(b);
+        if (rc != KMX_OK) return rc;
+    }
     if (b->sweep_on[slot]) {
         BatchIO io;
         std::memset(&io, 0, sizeof(io));
-        for (size_t q = 0; q < b->members.size(); ++q) {
-            io.x[q] = xs[b->members[q]];
-            io.y[q] = ys[b->members[q]];
+        const std::vector<int>& mem = b->members[slot];
+        for (size_t q = 0; q < mem.size(); ++q) {
+            io.x[q] = xs[mem[q]];
+            io.y[q] = ys[mem[q]];
         }
-        const int rc = kmx_sweep_batch_apply(b->sweep, slot, io, s);
+        const int rc = kmx_sweep_batch_apply(b->sweep[slot], slot, io, s);
         if (rc != KMX_OK) return rc;
     }
     BatchSet& bs = b->set[slot];
@@ -617,7 +691,8 @@ extern "C" int kmx_batch_destroy(kmx_batch* b) {
     if (!b) return KMX_OK;
     free_set(b->set[0]);
     free_set(b->set[1]);
-    kmx_sweep_batch_free(b->sweep);
+    kmx_sweep_batch_free(b->sweep[0]);
+    kmx_sweep_batch_free(b->sweep[1]);
     delete b;
     return KMX_OK;
 }
@@ -629,33 +704,14 @@ extern "C" int kmx_batch_set_sweep(kmx_batch* b, int prec, int on) {
     }
     const int slot = prec_slot(prec);
     if (bool(on) == b->sweep_on[slot]) return KMX_OK;
-    if (on && !b->sweep) {
-        std::vector<kmx_plan*> mp;
-        std::vector<int> mi;
-        for (int i = 0; i < int(b->plans.size()) && int(mi.size()) < KMX_BATCH_MAX; ++i) {
-            kmx_plan* p = b->plans[i];
-            if (!p->sw || std::find(mp.begin(), mp.end(), p) != mp.end()) continue;  // one arena per plan
-            mp.push_back(p);
-            mi.push_back(i);
-        }
-        if (mp.empty()) {
-            set_error("kmx_batch_set_sweep: no plan of the batch has a dimension-sweep path");
-            return KMX_E_UNSUPPORTED;
-        }
-        try {
-            b->sweep = kmx_sweep_batch_build(mp);
-        } catch (const std::exception& ex) {
-            set_error(std::string("kmx_batch_set_sweep: ") + ex.what());
-            return KMX_E_HIP;
-        }
-        b->members = mi;
+    if (on) {
+        const int rc = build_batch_sweep(b, slot);
+        if (rc != KMX_OK) return rc;
+    } else {
+        kmx_sweep_batch_free(b->sweep[slot]);
+        b->sweep[slot] = nullptr;
+        b->members[slot].clear();
     }
-    if (on && slot == 1)
-        for (int i : b->members)
-            if (!b->plans[size_t(i)]->d_blk32) {
-                set_error("kmx_batch_set_sweep: a plan has no FP32 blocks");
-                return KMX_E_ARG;
-            }
     b->sweep_on[slot] = on != 0;
     free_set(b->set[slot]);
     const hipError_t e = slot == 0 ? build_set<double>(b, b->set[0], true) : build_set<float>(b, b->set[1], false);
@@ -671,9 +727,10 @@ extern "C" int kmx_batch_get_sweep(const kmx_batch* b, int prec, int* members, i
         set_error("kmx_batch_get_sweep: bad arguments");
         return KMX_E_ARG;
     }
-    const bool act = b->sweep_on[prec_slot(prec)];
-    if (members) *members = act ? int(b->members.size()) : 0;
+    const int slot = prec_slot(prec);
+    const bool act = b->sweep_on[slot];
+    if (members) *members = act ? int(b->members[slot].size()) : 0;
     if (active) *active = act ? 1 : 0;
-    if (phases) *phases = act ? kmx_sweep_batch_phases(b->sweep) : 0;
+    if (phases) *phases = act ? kmx_sweep_batch_phases(b->sweep[slot]) : 0;
     return KMX_OK;
 }

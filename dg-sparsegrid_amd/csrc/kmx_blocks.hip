@@ -705,7 +705,7 @@ static void build_impl(kmx_plan* p, const std::vector<int>& idx, std::vector<lon
     const int D = p->D, deg = p->deg, N = p->N, T = p->T;
     kmx_blockset* bs = p->bs;
     auto fail = [&](const char* why) { bs->ok = false; bs->why = why; };
-    if (D < 2 || deg > 4 || N == 0) return fail("ndim < 2, deg > 4 or empty plan");
+    if (D < 2 || deg > KMX_MAX_DEG || N == 0) return fail("ndim < 2 or empty plan");
     int Lmax[KMX_MAX_DIM];
     for (int k = 0; k < D; ++k) {
         const int nb = p->nb[k];
@@ -947,6 +947,7 @@ int kmx_blocks_split(kmx_plan* p, int slot, double beta) {
         free_split(sp);
         sp.beta = beta;
         if (std::isinf(beta) && beta > 0) return KMX_OK;
+        if (p->deg > 4) return KMX_OK;  // the mode-product kernels take deg 1..4; the structure serves the sweep
         const int D = p->D, deg = p->deg, N = p->N, T = p->T, nblk = bs->nblk;
         double chain = 2.0 * D;
         for (int k = 0; k <= D; ++k) chain *= deg;  // 2 d deg^(d+1) per pair and term

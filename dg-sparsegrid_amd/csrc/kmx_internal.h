@@ -156,11 +156,17 @@ struct kmx_plan {
     double* d_blkT64 = nullptr;
     float* d_blkT32 = nullptr;
 
-    // host copies for kmx_plan_update_term
+    // host copies for kmx_plan_update_term and the sweep set-up
+    std::vector<int> h_idx1d;  // N x D, 0-based
     std::vector<double> h_blk64;
     std::vector<double> h_blkT64;
     double* d_x_host_stage = nullptr;  // kmx_apply_host staging
     double* d_y_host_stage = nullptr;
+
+    // factors the caller declared the identity (kmx_plan_desc.identity; empty: none)
+    std::vector<unsigned char> declared_identity;
+    // bumped by every change of a term matrix or scale (batches refresh their copies)
+    unsigned long long version = 0;
 
     kmx_blockset* bs = nullptr;  // level-block path (null when disabled)
     kmx_sweep* sw = nullptr;     // dimension-sweep path (null when the structure does not allow it)
@@ -193,6 +199,10 @@ void kmx_sweep_build(kmx_plan* p);
 int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s);
 void kmx_sweep_free(kmx_plan* p);
 double kmx_sweep_flops(const kmx_plan* p);
+// whether FP64 applies may use the plan's sweep (a separable sweep whose undeclared
+// identity factors are exact to 4 ulps, or a general sweep)
+bool kmx_sweep_fp64_ok(const kmx_plan* p);
+void kmx_sweep_get_info(const kmx_plan* p, kmx_sweep_info* out);
 // after kmx_plan_update_term(term, dim) (the FP64 block copy already updated)
 void kmx_sweep_term_updated(kmx_plan* p, int term, int dim);
 // after kmx_plan_set_term_scale: the plan's own launch tables take the new scales

@@ -94,6 +94,8 @@ enum {
     UD_BOX = 64,
     UD_REC = 128
 };
+// a separable op's record: its panel's runs in the plan's run table instead of a level range
+enum { R_RUN0 = R_LA, R_NRUN = R_LB };
 constexpr int UD_MAXLEV = UD_REC - UD_BOX - 1;  // highest level a record holds a box for
 constexpr int UD_WAVES = 8;                     // waves per workgroup
 
@@ -317,31 +319,6 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_op(const T* __restrict__ m
     ud_tile<T, PART, FINAL, NBR>(mat, arena, rv, w, lane, op, x, xmap, y, ymap, part);
 }
 
-// The combined 1D operator of one dimension of a separable sweep: out = sum over
-// the dimension's terms of scale * masked matrix (FP64, column-major n x n, terms
-// in order), and its FP32 copy.
-struct CombineArgs {
-    long long off[KMX_MAX_TERMS];
-    double scale[KMX_MAX_TERMS];
-    int nt, n;
-    long long n2;
-};
-// (and its transpose, for the adjoint apply: element (r, c) of the combined
-// matrix lands at (c, r))
-__global__ void __launch_bounds__(256) k_combine(const double* __restrict__ mat, CombineArgs a,
-                                                 double* __restrict__ out64, float* __restrict__ out32,
-                                                 double* __restrict__ outT64, float* __restrict__ outT32) {
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < a.n2; i += (long long)gridDim.x * 256) {
-        double v = 0.0;
-        for (int k = 0; k < a.nt; ++k) v += a.scale[k] * mat[a.off[k] + i];
-        out64[i] = v;
-        if (out32) out32[i] = float(v);
-        const long long r = i % a.n, c = i / a.n, it = c + r * a.n;
-        outT64[it] = v;
-        if (outT32) outT32[it] = float(v);
-    }
-}
-
 // Box copies of x and slot sums of every separable instance of a batch, one
 // launch each: job j covers workgroups [blk0_j, blk0_{j+1}), 256 positions each.
 // Every workgroup reads its own copy of its job (one load, no index to chase).
@@ -402,9 +379,15 @@ struct UdBatchOp {
     void* arena[2];
     const int* xmap;
     const int* ymap;
+    const int4* runs;  // separable ops: the plan's run table (forward or transposed)
     int inst;
     int kind;
 };
+
+template <typename T, int NB>
+__device__ __forceinline__ void sep_tile(const T* __restrict__ mat, const int4* __restrict__ runs,
+                                         T* __restrict__ arena, const int2 rv, const int w, const int lane,
+                                         const UdOp& op, T* __restrict__ part);
 
 // One phase of a batch: ops of many instances, each on whole workgroups
 // (wg_op[workgroup] = its op); the workgroup's op kind picks the tile body.
@@ -434,7 +417,7 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
             ud_tile<T, UD_FULL, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part);
             break;
         case UK_UP_FINAL: ud_tile<T, UD_UP, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
-        case UK_SEP: sep_tile<T>(mat, arena, rv, w, lane, o.a, part); break;
+        case UK_SEP: sep_tile<T, 8>(mat, o.runs, arena, rv, w, lane, o.a, part); break;
         default:
             if constexpr (MAXNBR == 4) {
                 if (o.kind == UK_DOWN4)
@@ -448,18 +431,26 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
     }
 }
 
-// One wave of a separable op (UD_SEP, one combined term, scale folded in): the
-// same tile and record as ud_tile, with the K loop split by level.  Within a level
-// the A and B elements of step s sit at a wave-uniform offset (4 s columns of A, 4 s
-// positions of the source box) from this lane's own fixed offset, so a load is a
-// scalar base plus one vector offset and a step costs no vector address or mask
-// work.  Rows and columns past the group read row / column 0 (finite values that
-// reach only outputs the tile does not store); positions past a level, in its
-// last step when deg * cells is not a multiple of 4, read zero.  NB: steps per load
-// batch; PF: the next batch's loads issued before this batch multiplies.
-template <typename T, int NB = 4, bool PF = false>
-__device__ __forceinline__ void sep_tile(const T* __restrict__ mat, T* __restrict__ arena, const int2 rv, const int w,
-                                         const int lane, const UdOp& op, T* __restrict__ part) {
+// One wave of a separable op (UD_SEP): one 16 x 16 output tile of one group (rows
+// m0 .. m0 + 15 of the group's positions along the op dimension, columns f0 .. f0 + 15
+// of its O x I columns) from the compact row-panel storage of the dimension's
+// combined operator (its terms summed, scales included).  The storage keeps, per
+// 16-row panel, only the steps of four source positions that meet a kept block;
+// they come as runs (level, first position in the level, positions, storage
+// offset), and the tile uses the panel's runs of the levels its group holds.
+// Element (row li, position 4 s + lk) of a run's step s sits at offset
+// 64 s + 16 lk + li = 64 s + lane, so each step is one contiguous 64-value read
+// per wave; B comes from the box of the run's level at the lane's column.  The
+// wave's share is steps [slo, shi), counted over runs run0 .. run0 + nrun - 1; NB
+// steps are addressed on the scalar unit (crossing runs as needed), then all
+// their loads are issued, then their matrix operations.  Rows past the group
+// read the panel's other rows (finite; never stored), rows past the matrix read
+// the storage's zero padding; positions past a run's end (its last step when
+// the run's positions are not a multiple of 4) read zero A and B is masked.
+template <typename T, int NB>
+__device__ __forceinline__ void sep_tile(const T* __restrict__ mat, const int4* __restrict__ runs,
+                                         T* __restrict__ arena, const int2 rv, const int w, const int lane,
+                                         const UdOp& op, T* __restrict__ part) {
     using V = typename SwMfma<T>::V;
     const int gsize = __builtin_amdgcn_readlane(rv.x, R_GSIZE);
     const int gbase = __builtin_amdgcn_readlane(rv.x, R_GBASE);
@@ -468,11 +459,9 @@ __device__ __forceinline__ void sep_tile(const T* __restrict__ mat, T* __restric
     int oaddr[4] = {0, 0, 0, 0};
     bool ook[4] = {false, false, false, false};
     if (gsize > 0) {
-        const int m0 = __builtin_amdgcn_readlane(rv.x, R_M0), f0 = __builtin_amdgcn_readlane(rv.x, R_F0);
+        const int f0 = __builtin_amdgcn_readlane(rv.x, R_F0);
         const int O = __builtin_amdgcn_readlane(rv.x, R_O), I = __builtin_amdgcn_readlane(rv.x, R_I);
         const int li = lane & 15, lk = lane >> 4;
-        const int lr = __shfl(rv.x, R_ROWLEV + li);  // level of A row m0 + li (-1: past the group)
-        const int r0 = lr >= 0 ? m0 + li : 0;
         const int f = f0 + li;
         const bool fok = f < O * I;
         const int fc = fok ? f : 0;
@@ -490,88 +479,63 @@ __device__ __forceinline__ void sep_tile(const T* __restrict__ mat, T* __restric
             }
         const int tn = __builtin_amdgcn_readlane(rv.x, R_TN);
         if (tn > 0) {
-            const int deg = op.deg, lda = op.lda;
-            const int s1 = __builtin_amdgcn_readlane(rv.x, R_SHI);
-            int l = __builtin_amdgcn_readlane(rv.x, R_LA), rem = __builtin_amdgcn_readlane(rv.x, R_SLO);
-            for (;;) {  // level and step of this wave's first step
-                const int ns = (deg * (l ? 1 << (l - 1) : 1) + 3) >> 2;
-                if (rem < ns) break;
-                rem -= ns;
-                ++l;
-            }
-            int q = __builtin_amdgcn_readlane(rv.x, R_SLO);  // steps done, over the tile's K range
-            int sin = rem;                                   // step inside level l
-            const T* amat = mat + op.a_off[0];
+            const int deg = op.deg;
+            const int run0 = __builtin_amdgcn_readlane(rv.x, R_RUN0);
+            const int rend = run0 + __builtin_amdgcn_readlane(rv.x, R_NRUN);
+            const int slo = __builtin_amdgcn_readlane(rv.x, R_SLO), shi = __builtin_amdgcn_readlane(rv.x, R_SHI);
+            const T* amat = mat + op.a_off[0] + lane;
             const T* bsrc = arena + op.in_base;
-            const int alane = r0 + lk * lda;  // this lane's A offset at step 0 of any level (plus the level's columns)
-            while (q < s1) {
-                const int st = l ? 1 << (l - 1) : 0;
-                const int nl = deg * (l ? st : 1);  // positions of level l
-                const int nsl = (nl + 3) >> 2;
-                const int nst = min(nsl - sin, s1 - q);  // steps of level l in this wave's share
-                const bool tail = (nl & 3) && sin + nst == nsl;  // the level's partial last step is among them
-                const int nfull = tail ? nst - 1 : nst;
+            // the tile's runs, one per lane (64 at a time)
+            int c0 = run0;
+            int4 myr = runs[c0 + min(lane, rend - 1 - c0)];
+            int rr = run0, nst = 0, npos = 0, a = 0, b = 0;
+            auto load_run = [&]() {
+                if (rr - c0 >= 64) {
+                    c0 = rr;
+                    myr = runs[c0 + min(lane, rend - 1 - c0)];
+                }
+                const int k = rr - c0;
+                const int l = __builtin_amdgcn_readlane(myr.x, k);
+                const int pos0 = __builtin_amdgcn_readlane(myr.y, k);
+                npos = __builtin_amdgcn_readlane(myr.z, k);
+                a = __builtin_amdgcn_readlane(myr.w, k);
+                nst = (npos + 3) >> 2;
+                const int nl = deg * (l ? 1 << (l - 1) : 1);
                 const int box = __builtin_amdgcn_readlane(rv.y, min(l, UD_MAXLEV));
-                const int blane = (fo * nl + lk) * I + fi;  // this lane's B offset at step 0 of level l
-                const T* a0 = amat + (long long)(deg * st + 4 * sin) * lda;  // uniform: first step of the share
-                const T* b0 = bsrc + box + 4 * sin * I;
-                int s = 0;
-                if (!PF) {
-                    for (; s + NB <= nfull; s += NB) {  // whole batches: every load in flight at once
-                        T av[NB], bv[NB];
+                b = box + (fo * nl + pos0 + lk) * I + fi;
+            };
+            load_run();
+            int sin = slo;  // step inside run rr
+            while (sin >= nst) {
+                sin -= nst;
+                ++rr;
+                load_run();
+            }
+            for (int q = slo; q < shi; q += NB) {
+                int ao[NB], bo[NB];
+                bool pa[NB], pb[NB];
 #pragma unroll
-                        for (int c = 0; c < NB; ++c) {
-                            av[c] = (a0 + (long long)(4 * (s + c)) * lda)[alane];
-                            bv[c] = (b0 + 4 * (s + c) * I)[blane];
-                        }
-#pragma unroll
-                        for (int c = 0; c < NB; ++c) acc = SwMfma<T>::run(av[c], bv[c], acc);
-                    }
-                } else if (nfull >= NB) {  // whole batches, the next batch's loads in flight while one multiplies
-                    T av[NB], bv[NB];
-#pragma unroll
-                    for (int c = 0; c < NB; ++c) {
-                        av[c] = (a0 + (long long)(4 * c) * lda)[alane];
-                        bv[c] = (b0 + 4 * c * I)[blane];
-                    }
-                    for (s = NB; s + NB <= nfull; s += NB) {
-                        T an[NB], bn[NB];
-#pragma unroll
-                        for (int c = 0; c < NB; ++c) {
-                            an[c] = (a0 + (long long)(4 * (s + c)) * lda)[alane];
-                            bn[c] = (b0 + 4 * (s + c) * I)[blane];
-                        }
-#pragma unroll
-                        for (int c = 0; c < NB; ++c) {
-                            acc = SwMfma<T>::run(av[c], bv[c], acc);
-                            av[c] = an[c];
-                            bv[c] = bn[c];
+                for (int c = 0; c < NB; ++c) {
+                    const bool on = q + c < shi;
+                    pa[c] = on;
+                    pb[c] = on && 4 * sin + lk < npos;
+                    ao[c] = on ? a + 64 * sin : 0;
+                    bo[c] = pb[c] ? b + 4 * sin * I : 0;
+                    if (on && ++sin == nst) {
+                        sin = 0;
+                        if (q + c + 1 < shi) {
+                            ++rr;
+                            load_run();
                         }
                     }
-#pragma unroll
-                    for (int c = 0; c < NB; ++c) acc = SwMfma<T>::run(av[c], bv[c], acc);
                 }
-                for (; s < nfull; s += 2) {  // the rest (short low levels) two steps at a time
-                    T av[2], bv[2];
+                T av[NB], bv[NB];
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        const int sc = min(s + c, nfull - 1);  // a step past the share repeats a valid one
-                        av[c] = (a0 + (long long)(4 * sc) * lda)[alane];
-                        bv[c] = (b0 + 4 * sc * I)[blane];
-                    }
+                for (int c = 0; c < NB; ++c) av[c] = amat[ao[c]];
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) acc = SwMfma<T>::run(s + c < nfull ? av[c] : T(0), bv[c], acc);
-                }
-                if (tail) {  // positions nl - (nl & 3) .. nl - 1 of the level, the rest zero
-                    const int pl = 4 * (nsl - 1) + lk;  // this lane's position in the level
-                    const bool pin = pl < nl;
-                    const T av = pin ? amat[(long long)(deg * st + pl) * lda + r0] : T(0);
-                    const T bv = pin ? bsrc[box + (fo * nl + pl) * I + fi] : T(0);
-                    acc = SwMfma<T>::run(av, bv, acc);
-                }
-                q += nst;
-                sin = 0;
-                ++l;
+                for (int c = 0; c < NB; ++c) bv[c] = bsrc[bo[c]];
+#pragma unroll
+                for (int c = 0; c < NB; ++c) acc = SwMfma<T>::run(pa[c] ? av[c] : T(0), pb[c] ? bv[c] : T(0), acc);
             }
         }
         if (gsize > 1 && !leader)
@@ -601,12 +565,7 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_sep_batch(const UdBatchOp* __
     const int2 rv = recs[((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane];
     const UdBatchOp& o = ops[wg_op[blockIdx.x]];
     constexpr int sl = sizeof(T) == 8 ? 0 : 1;
-    // Load batches (timed on 64 adapted 2D / 3D instances, eight batches per wave): FP32
-    // four steps with the next four in flight, ahead of two in flight (+3 %), four at once
-    // (+2 %) and eight at once (+7 %); FP64 four steps at once (two or four in flight and
-    // eight at once within +-1 %).
-    constexpr bool f32 = sizeof(T) == 4;
-    sep_tile<T, 4, f32>(static_cast<const T*>(o.mat[sl]), static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
+    sep_tile<T, 8>(static_cast<const T*>(o.mat[sl]), o.runs, static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
 }
 
 }  // namespace
@@ -649,6 +608,24 @@ struct HostTile {
 
 }  // namespace
 
+// Compact row-panel storage of the combined operators of a separable sweep, one
+// orientation (0: the operator, 1: its transpose, for the adjoint apply).  Op k's
+// panel p (rows 16 p .. 16 p + 15 of its n x n combined matrix) has runs
+// runs[run_base[k] + prun[k][p] ..  run_base[k] + prun[k][p + 1]) in level order:
+// (level, first position inside the level, positions, storage offset from
+// a_base[k]); a run holds 4 * ceil(positions / 4) positions x 16 rows, position
+// major, zero past the run's positions and past the matrix rows.
+struct SepSide {
+    std::vector<int4> runs;
+    std::vector<int> run_base;
+    std::vector<std::vector<int>> prun;
+    std::vector<long long> a_base;
+    long long a_elems = 0;
+    int4* d_runs = nullptr;
+    double* d_a64 = nullptr;
+    float* d_a32 = nullptr;
+};
+
 struct kmx_sweep {
     int D = 0;
     long long nb = 0;     // values per intermediate (every box of every level tuple)
@@ -667,6 +644,7 @@ struct kmx_sweep {
     int* d_recs = nullptr;
     void* d_arena[2] = {nullptr, nullptr};  // FP64, FP32
     long long flops = 0;                    // useful flops per apply (all terms)
+    long long mfma_flops = 0;               // matrix-core flops issued per apply (tiles and steps as run)
     // what the work records are emitted from (a batch emits them again with its own split)
     struct OpDesc {
         int part;
@@ -677,20 +655,27 @@ struct kmx_sweep {
     };
     std::vector<OpDesc> desc;
     // separable form: every term acts in at most one dimension (its other 1D
-    // factors are the identity), so the apply is one whole-operator op per
-    // dimension, x to y; identity[t * D + k] records the factors taken as the
-    // identity, and stale is set when kmx_plan_update_term replaces one of them
-    // by another matrix
+    // factors are the identity, or declared so), so the apply is one whole-operator
+    // op per dimension; identity[t * D + k] records the factors taken as the
+    // identity, and stale is set when kmx_plan_update_term replaces one of them by
+    // another matrix.  fp64 is false when some such factor is only within
+    // identity_tol of the identity (then only FP32 applies may use the sweep);
+    // identity_dev is the largest max |A - I| over the undeclared ones.
     bool separable = false;
     std::vector<char> identity;
     bool stale = false;
-    kmx_sweep_batch* self = nullptr;  // separable: the plan's own launches (a batch of one)
-    // separable: op k's terms folded into one matrix (scales included) at comb_off[k]
-    double* d_comb64 = nullptr;
-    float* d_comb32 = nullptr;
-    double* d_combT64 = nullptr;  // their transposes (the adjoint apply), same offsets
-    float* d_combT32 = nullptr;
-    std::vector<long long> comb_off;
+    bool fp64 = true;
+    double identity_dev = 0.0;
+    double drop_tol = 0.0;
+    long long kept_blocks = 0, total_blocks = 0;
+    // separable: per op, the kept deg x deg blocks of its combined matrix (nb_d x nb_d)
+    std::vector<std::vector<unsigned char>> keep;
+    SepSide side[2];
+    // storage and launch tables replaced by a rebuild, kept until the plan is
+    // destroyed (a graph captured earlier may still read them)
+    std::vector<SepSide> retired;
+    std::vector<kmx_sweep_batch*> retired_batches;
+    kmx_sweep_batch* self = nullptr;   // separable: the plan's own launches (a batch of one)
     kmx_sweep_batch* selfT = nullptr;  // separable: the adjoint apply's launches
     std::vector<Group> groups;
     std::vector<int> group0;  // groups of op dimension d: [group0[d], group0[d+1])
@@ -706,26 +691,49 @@ int sweep_wave_batches(int dflt) {
     return dflt;
 }
 
+double env_double(const char* name, double dflt) {
+    if (const char* e = std::getenv(name)) return std::atof(e);
+    return dflt;
+}
+
+// Runs of panel p of separable op k (orientation `side`) whose level is <= m: a
+// prefix of the panel's runs.  Returns their count and total steps.
+void panel_prefix(const SepSide& sd, int k, int p, int m, int& nrun, int& steps) {
+    nrun = 0;
+    steps = 0;
+    const int r0 = sd.run_base[size_t(k)];
+    for (int q = sd.prun[size_t(k)][size_t(p)]; q < sd.prun[size_t(k)][size_t(p) + 1]; ++q) {
+        const int4& ru = sd.runs[size_t(r0 + q)];
+        if (ru.x > m) break;
+        ++nrun;
+        steps += (ru.z + 3) / 4;
+    }
+}
+
 // Work records of every op of a sweep.  A tile (16 rows x 16 columns x nbr
 // branches) whose K steps exceed wave_batches load batches is split over up to
 // UD_WAVES waves of one workgroup: a lone plan (latency-bound, few tiles) uses
 // one batch per wave, a batch of plans (many tiles) four (general sweeps) or eight
 // (separable ones).  Widest tiles first
 // (power-of-two wave counts then never straddle a workgroup), longest first.
+// Separable ops take their steps from the runs of orientation `side`.
 void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector<int>>& recs,
-                  std::vector<kmx_sweep::Op>& ops) {
+                  std::vector<kmx_sweep::Op>& ops, int side = 0) {
     const int deg = sw.deg;
     recs.clear();
     ops.clear();
+    int sep_k = 0;  // separable op index (the ops of a separable sweep are all UD_SEP, in order)
     for (const kmx_sweep::OpDesc& od : sw.desc) {
         const int T = od.a.T;  // the op's terms
         const int part = od.part;
         const bool fin = od.fin;
+        const bool sep = part == UD_SEP;
         const long long nbranch = od.nbranch;
         const int nbr = int(std::min<long long>(nbranch, sw.max_nbr));  // branches per tile (a power of two)
         kmx_sweep::Op op{part, fin, od.a, int(recs.size() / UD_WAVES), 0, nbr, od.terms};
+        if (sep) op.a.a_off[0] = sw.side[side].a_base[size_t(sep_k)];
         const int d = od.a.d;
-        const bool multi = fin || part == UD_SEP;  // one tile sums every term of the op
+        const bool multi = fin || sep;  // one tile sums every term of the op
         const int nterm = multi ? T : 1;
         const int batch = ud_split_batch(nbr) * std::max(1, wave_batches);
         std::vector<HostTile> ot;
@@ -739,7 +747,16 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
                         int la = 0, lb = g.m;
                         if (part == UD_DOWN) la = host_level(m0, deg);
                         if (part == UD_UP) lb = host_level(std::min(n, m0 + 16) - 1, deg) - 1;
-                        const int sper = steps_of(la, lb, deg);
+                        int sper = 0, run0 = 0, nrun = 0;
+                        if (sep) {
+                            panel_prefix(sw.side[side], sep_k, m0 / 16, g.m, nrun, sper);
+                            run0 = sw.side[side].run_base[size_t(sep_k)] +
+                                   sw.side[side].prun[size_t(sep_k)][size_t(m0 / 16)];
+                            la = run0;
+                            lb = nrun;
+                        } else {
+                            sper = steps_of(la, lb, deg);
+                        }
                         const int steps = nterm * sper;
                         int gsize = 1;  // waves sharing the tile: at most `batch` steps each
                         while (gsize < UD_WAVES && (steps + gsize - 1) / gsize > batch) gsize *= 2;
@@ -771,7 +788,12 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
                 r[R_GBASE] = slot;
                 r[R_MAGIC] = int(g.magic);
                 const int q0 = std::min(h.steps, w * qn), q1 = std::min(h.steps, (w + 1) * qn);
-                if (q1 > q0) {
+                if (sep) {  // steps counted over the tile's runs
+                    r[R_TLO] = 0;
+                    r[R_SLO] = q0;
+                    r[R_SHI] = q1;
+                    r[R_TN] = q1 > q0 ? 1 : 0;
+                } else if (q1 > q0) {
                     r[R_TLO] = h.t + q0 / h.sper;
                     r[R_SLO] = q0 % h.sper;
                     r[R_TN] = (q1 - 1) / h.sper - q0 / h.sper + 1;
@@ -797,19 +819,23 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
         while (recs.size() % UD_WAVES) recs.push_back(std::vector<int>(UD_REC, 0));  // empty slots
         op.wg1 = int(recs.size() / UD_WAVES);
         ops.push_back(op);
+        if (sep) ++sep_k;
     }
 }
 
 
 // Every work record: its tile inside a group of its op, its slots inside the
-// arena, its rows and boxes as the group defines them (host checks, before upload).
+// arena, its rows and boxes as the group defines them, a separable tile's runs
+// inside its panel, its group's levels and the op's storage (host checks, before
+// upload).
 void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& recs,
-                   const std::vector<kmx_sweep::Op>& ops) {
+                   const std::vector<kmx_sweep::Op>& ops, int side = 0) {
     const int deg = sw.deg;
     const long long nb = sw.nb, half = 1LL << (sw.D - 1);
     const std::vector<Group>& groups = sw.groups;
     const std::vector<int>& group0 = sw.group0;
     auto slot_ok = [&](long long off) { return off >= 0 && off + nb <= sw.arena; };
+    int sep_k = 0;
     for (const auto& op : ops) {
         const int T = op.a.T;
         // (a separable op's terms are folded into its one combined matrix)
@@ -822,6 +848,14 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
         const long long nbranch = (op.final || sep) ? 1 : (op.part == UD_FULL ? half : (1LL << op.a.d));
         if (op.nbr < 1 || op.nbr > 4 || nbranch % op.nbr || (op.final && op.nbr != 1))
             throw std::logic_error("sweep branches per tile");
+        const SepSide& sd = sw.side[side];
+        long long a_lo = 0, a_hi = 0;
+        if (sep) {
+            if (size_t(sep_k) >= sd.a_base.size() || op.a.a_off[0] != sd.a_base[size_t(sep_k)])
+                throw std::logic_error("sweep separable storage offset");
+            a_lo = sd.a_base[size_t(sep_k)];
+            a_hi = size_t(sep_k) + 1 < sd.a_base.size() ? sd.a_base[size_t(sep_k) + 1] : sd.a_elems;
+        }
         for (int wg = op.wg0; wg < op.wg1; ++wg)
             for (int w = 0; w < UD_WAVES; ++w) {
                 const std::vector<int>& r = recs[size_t(wg) * UD_WAVES + w];
@@ -844,16 +878,10 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
                 if (!gp) throw std::logic_error("sweep record group");
                 const Group& g = *gp;
                 const int n = deg << g.m;
-                const int sper = steps_of(r[R_LA], r[R_LB], deg);
                 if (r[R_M0] < 0 || r[R_M0] >= n || r[R_M0] % 16 || r[R_F0] < 0 || r[R_F0] >= g.O * g.I ||
-                    r[R_LA] < 0 || r[R_LB] > g.m || r[R_SPER] != sper || r[R_B] < 0 || r[R_B] % op.nbr ||
-                    r[R_B] + op.nbr > nbranch ||
-                    r[R_TOUT] < 0 || r[R_TOUT] >= T || r[R_MAGIC] != int(g.magic))
+                    r[R_B] < 0 || r[R_B] % op.nbr || r[R_B] + op.nbr > nbranch || r[R_TOUT] < 0 || r[R_TOUT] >= T ||
+                    r[R_MAGIC] != int(g.magic))
                     throw std::logic_error("sweep record outside its group");
-                if (r[R_TN] < 0 || (r[R_TN] > 0 && (r[R_TLO] < 0 || r[R_TLO] + r[R_TN] > T || r[R_SLO] < 0 ||
-                                                    r[R_SLO] >= sper || r[R_SHI] < 1 || r[R_SHI] > sper ||
-                                                    (!op.final && !sep && (r[R_TN] != 1 || r[R_TLO] != r[R_TOUT])))))
-                    throw std::logic_error("sweep record steps");
                 for (int i = 0; i < 16; ++i) {
                     const int row = r[R_M0] + i;
                     const int l = row < n ? host_level(row, deg) : -1;
@@ -864,11 +892,40 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
                 }
                 for (int l = 0; l <= g.m; ++l)
                     if (r[UD_BOX + l] != g.box[l]) throw std::logic_error("sweep record boxes");
-                if (sep) {  // one box copy of x in, the running sum in (after the first op) and out
+                if (sep) {
+                    // one box copy of x in, the op's own slot out
                     if (op.a.x0 || op.final || !slot_ok(op.a.in_base) || !slot_ok(op.a.out_base))
                         throw std::logic_error("sweep separable slots outside the arena");
+                    // the runs: the panel's own, levels of the group, positions inside their level,
+                    // storage inside the op's, and the wave's steps inside them
+                    const int p = r[R_M0] / 16;
+                    const int rb = sd.run_base[size_t(sep_k)];
+                    const std::vector<int>& pr = sd.prun[size_t(sep_k)];
+                    if (size_t(p) + 1 >= pr.size() || r[R_RUN0] != rb + pr[size_t(p)] || r[R_NRUN] < 0 ||
+                        r[R_NRUN] > pr[size_t(p) + 1] - pr[size_t(p)])
+                        throw std::logic_error("sweep separable record runs");
+                    long long steps = 0;
+                    for (int q = r[R_RUN0]; q < r[R_RUN0] + r[R_NRUN]; ++q) {
+                        const int4& ru = sd.runs[size_t(q)];
+                        const long long nl = deg * cells_of(ru.x);
+                        const long long nst = (ru.z + 3) / 4;
+                        if (ru.x < 0 || ru.x > g.m || ru.x > UD_MAXLEV || ru.y < 0 || ru.y % 4 || ru.z < 1 ||
+                            ru.y + ru.z > nl || a_lo + ru.w < a_lo || a_lo + ru.w + 64 * nst > a_hi || ru.w % 64)
+                            throw std::logic_error("sweep separable run");
+                        steps += nst;
+                    }
+                    if (r[R_SPER] != steps || r[R_TN] != (r[R_SHI] > r[R_SLO] ? 1 : 0) || r[R_SLO] < 0 ||
+                        r[R_SHI] < r[R_SLO] || r[R_SHI] > steps)
+                        throw std::logic_error("sweep separable record steps");
                     continue;
                 }
+                const int sper = steps_of(r[R_LA], r[R_LB], deg);
+                if (r[R_LA] < 0 || r[R_LB] > g.m || r[R_SPER] != sper)
+                    throw std::logic_error("sweep record outside its group");
+                if (r[R_TN] < 0 || (r[R_TN] > 0 && (r[R_TLO] < 0 || r[R_TLO] + r[R_TN] > T || r[R_SLO] < 0 ||
+                                                    r[R_SLO] >= sper || r[R_SHI] < 1 || r[R_SHI] > sper ||
+                                                    (!op.final && (r[R_TN] != 1 || r[R_TLO] != r[R_TOUT])))))
+                    throw std::logic_error("sweep record steps");
                 for (int bj = r[R_B]; bj < r[R_B] + op.nbr; ++bj) {  // every branch of the tile
                     for (int t = r[R_TLO]; t < r[R_TLO] + r[R_TN]; ++t)
                         if (!(op.a.x0 && bj == 0) && !slot_ok(op.a.in_base + ((long long)bj * T + t) * nb))
@@ -883,55 +940,236 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
                     }
                 }
             }
+        if (sep) ++sep_k;
     }
 }
 
-}  // namespace
-
-// Whether 1D factor (t, k) of the plan is the identity: every diagonal block
-// within 4 ulps of I and every other block within 4 ulps of 0 (the FP64 copy the
-// plan keeps, block-row-major).
-bool identity_factor(const kmx_plan* p, int t, int k) {
-    const double tol = 4.0 * 2.220446049250313e-16;
+// max |A - I| over every entry of 1D factor (t, k) (the FP64 copy the plan
+// keeps, block-row-major).
+double identity_dev(const kmx_plan* p, int t, int k) {
     const int deg = p->deg, nbk = p->nb[k];
     const double* m = p->h_blk64.data() + p->off[t * p->D + k];
+    double dev = 0.0;
     for (long long I = 0; I < nbk; ++I)
         for (long long J = 0; J < nbk; ++J) {
             const double* b = m + (I * nbk + J) * deg * deg;
             for (int a = 0; a < deg; ++a)
                 for (int c = 0; c < deg; ++c)
-                    if (std::fabs(b[a * deg + c] - (I == J && a == c ? 1.0 : 0.0)) > tol) return false;
+                    dev = std::max(dev, std::fabs(b[a * deg + c] - (I == J && a == c ? 1.0 : 0.0)));
         }
-    return true;
+    return dev;
 }
 
-// (Re)computes the combined matrices of a separable sweep (every op, or op `only`)
-// from the plan's masked matrices and term scales.  Waits for the device first
-// (queued applies may read them) and after.
-int combine_matrices(kmx_plan* p, int only = -1) {
-    kmx_sweep* sw = p->sw;
-    if (hipDeviceSynchronize() != hipSuccess) return KMX_E_HIP;
-    for (size_t k = 0; k < sw->desc.size(); ++k) {
-        if (only >= 0 && int(k) != only) continue;
-        const kmx_sweep::OpDesc& od = sw->desc[k];
-        CombineArgs a{};
-        a.nt = int(od.terms.size());
-        const int d = od.a.d;
-        for (int i = 0; i < a.nt; ++i) {
-            a.off[i] = p->off[od.terms[size_t(i)] * p->D + d];
-            a.scale[i] = p->scale[od.terms[size_t(i)]];
-        }
-        a.n = p->n[d];
-        a.n2 = (long long)p->n[d] * p->n[d];
-        const unsigned g = unsigned(std::min<long long>((a.n2 + 255) / 256, 4096));
-        const long long o = sw->comb_off[k];
-        hipLaunchKernelGGL(k_combine, dim3(g), dim3(256), 0, 0, p->bs->d_mat64, a, sw->d_comb64 + o,
-                           sw->d_comb32 ? sw->d_comb32 + o : nullptr, sw->d_combT64 + o,
-                           sw->d_combT32 ? sw->d_combT32 + o : nullptr);
-    }
-    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return KMX_E_HIP;
-    return KMX_OK;
+// Entry (r, c) of factor (t, d), masked by dimension d's 1D connections.
+inline double masked_entry(const kmx_plan* p, int t, int d, long long r, long long c) {
+    const int deg = p->deg, nbk = p->nb[d];
+    const long long I = r / deg, J = c / deg;
+    if (!p->bs->mask[size_t(d)][size_t(I * nbk + J)]) return 0.0;
+    return p->h_blk64[size_t(p->off[t * p->D + d] + ((I * nbk + J) * deg + r % deg) * deg + c % deg)];
 }
+
+// Kept blocks of separable op k: block (I, J) of its combined matrix is kept when
+// some term of the op has a connected block there whose largest entry exceeds
+// drop_tol times the largest connected entry of that term's matrix (drop_tol = 0:
+// every block with a nonzero entry).  Independent of the term scales.
+std::vector<unsigned char> sep_pattern(const kmx_plan* p, const kmx_sweep& sw, int k) {
+    const kmx_sweep::OpDesc& od = sw.desc[size_t(k)];
+    const int d = od.a.d, deg = p->deg, nbk = p->nb[d];
+    std::vector<unsigned char> keep(size_t(nbk) * nbk, 0);
+    for (int t : od.terms) {
+        const double* m = p->h_blk64.data() + p->off[t * p->D + d];
+        const std::vector<unsigned char>& mask = p->bs->mask[size_t(d)];
+        double big = 0.0;
+        std::vector<double> bmax(size_t(nbk) * nbk, 0.0);
+        for (long long q = 0; q < (long long)nbk * nbk; ++q) {
+            if (!mask[size_t(q)]) continue;
+            double v = 0.0;
+            for (int e = 0; e < deg * deg; ++e) v = std::max(v, std::fabs(m[q * deg * deg + e]));
+            bmax[size_t(q)] = v;
+            big = std::max(big, v);
+        }
+        const double cut = sw.drop_tol * big;
+        for (size_t q = 0; q < bmax.size(); ++q)
+            if (bmax[q] > cut) keep[q] = 1;
+    }
+    return keep;
+}
+
+// Runs of every op of orientation `s` from the kept-block patterns (orientation 1
+// reads them transposed); storage offsets in units of 64 values.
+void sep_runs(const kmx_plan* p, kmx_sweep& sw, int s) {
+    SepSide& sd = sw.side[s];
+    sd.runs.clear();
+    sd.run_base.clear();
+    sd.prun.clear();
+    sd.a_base.clear();
+    const int deg = p->deg;
+    long long abase = 0;
+    for (size_t k = 0; k < sw.desc.size(); ++k) {
+        const int d = sw.desc[k].a.d, nbk = p->nb[d], n = p->n[d];
+        const std::vector<unsigned char>& keep = sw.keep[k];
+        auto kept = [&](int I, int J) { return keep[s == 0 ? size_t(I) * nbk + J : size_t(J) * nbk + I] != 0; };
+        int L = 0;
+        while ((deg << L) < n) ++L;
+        if ((deg << L) != n) throw std::logic_error("separable op: matrix size is not deg * 2^L");
+        const int npan = (n + 15) / 16;
+        sd.run_base.push_back(int(sd.runs.size()));
+        sd.a_base.push_back(abase);
+        std::vector<int> pr(size_t(npan) + 1, 0);
+        long long aoff = 0;
+        std::vector<unsigned char> colneed(static_cast<size_t>(nbk), 0);
+        for (int pnl = 0; pnl < npan; ++pnl) {
+            pr[size_t(pnl)] = int(sd.runs.size()) - sd.run_base.back();
+            const int I0 = (16 * pnl) / deg, I1 = std::min(n - 1, 16 * pnl + 15) / deg;
+            for (int J = 0; J < nbk; ++J) {
+                bool any = false;
+                for (int I = I0; I <= I1 && !any; ++I) any = kept(I, J);
+                colneed[size_t(J)] = any ? 1 : 0;
+            }
+            for (int l = 0; l <= L; ++l) {
+                const long long k0 = deg * start_of(l), nl = deg * cells_of(l);
+                const long long nst = (nl + 3) / 4;
+                long long s0 = 0;
+                while (s0 < nst) {
+                    auto need = [&](long long st) {
+                        const long long a = k0 + 4 * st, b = k0 + std::min(nl, 4 * st + 4) - 1;
+                        for (long long J = a / deg; J <= b / deg; ++J)
+                            if (colneed[size_t(J)]) return true;
+                        return false;
+                    };
+                    if (!need(s0)) {
+                        ++s0;
+                        continue;
+                    }
+                    long long s1 = s0 + 1;
+                    while (s1 < nst && need(s1)) ++s1;
+                    const long long pos0 = 4 * s0, npos = std::min(nl, 4 * s1) - pos0;
+                    if (aoff > (1LL << 30)) throw std::runtime_error("separable op storage beyond 32-bit offsets");
+                    sd.runs.push_back(make_int4(l, int(pos0), int(npos), int(aoff)));
+                    aoff += 64 * (s1 - s0);
+                    s0 = s1;
+                }
+            }
+        }
+        pr[size_t(npan)] = int(sd.runs.size()) - sd.run_base.back();
+        sd.prun.push_back(std::move(pr));
+        abase += std::max<long long>(aoff, 64);  // an op without runs still owns 64 values (its tiles' dummy reads)
+        if (sd.runs.size() > size_t(1) << 30) throw std::runtime_error("separable op runs beyond 32-bit indices");
+    }
+    sd.a_elems = abase;
+}
+
+// The combined values (terms summed in order, scales included) of op `only` (or
+// every op) of orientation `s` into its compact storage, FP64 and FP32, and the run
+// table; allocates the device buffers on first use.  Waits for the device first
+// (queued applies may read them).
+void sep_values(const kmx_plan* p, kmx_sweep& sw, int s, int only = -1) {
+    SepSide& sd = sw.side[s];
+    const int deg = p->deg;
+    (void)deg;
+    if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
+    const bool want32 = p->bs->d_mat32 != nullptr;
+    if (!sd.d_runs) {
+        if (hipMalloc(&sd.d_runs, std::max<size_t>(1, sd.runs.size()) * sizeof(int4)) != hipSuccess ||
+            hipMalloc(&sd.d_a64, size_t(std::max(1LL, sd.a_elems)) * sizeof(double)) != hipSuccess ||
+            (want32 && hipMalloc(&sd.d_a32, size_t(std::max(1LL, sd.a_elems)) * sizeof(float)) != hipSuccess))
+            throw std::bad_alloc();
+        if (!sd.runs.empty() &&
+            hipMemcpy(sd.d_runs, sd.runs.data(), sd.runs.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("hipMemcpy failed");
+    }
+    for (size_t k = 0; k < sw.desc.size(); ++k) {
+        if (only >= 0 && int(k) != only) continue;
+        const kmx_sweep::OpDesc& od = sw.desc[k];
+        const int d = od.a.d, n = p->n[d];
+        const long long lo = sd.a_base[k], hi = k + 1 < sd.a_base.size() ? sd.a_base[k + 1] : sd.a_elems;
+        std::vector<double> v64(size_t(hi - lo), 0.0);
+        const std::vector<int>& pr = sd.prun[k];
+        for (size_t pnl = 0; pnl + 1 < pr.size(); ++pnl)
+            for (int q = pr[pnl]; q < pr[pnl + 1]; ++q) {
+                const int4& ru = sd.runs[size_t(sd.run_base[k] + q)];
+                const long long k0 = p->deg * start_of(ru.x) + ru.y;
+                for (int j = 0; j < ru.z; ++j)
+                    for (int li = 0; li < 16; ++li) {
+                        const long long r = 16 * (long long)pnl + li;
+                        if (r >= n) continue;
+                        const long long c = k0 + j;
+                        double v = 0.0;
+                        for (int t : od.terms)
+                            v += p->scale[t] * (s == 0 ? masked_entry(p, t, d, r, c) : masked_entry(p, t, d, c, r));
+                        v64[size_t(ru.w + 16 * j + li)] = v;
+                    }
+            }
+        if (!v64.empty() && hipMemcpy(sd.d_a64 + lo, v64.data(), v64.size() * sizeof(double), hipMemcpyHostToDevice) !=
+                                hipSuccess)
+            throw std::runtime_error("hipMemcpy failed");
+        if (want32 && !v64.empty()) {
+            std::vector<float> v32(v64.begin(), v64.end());
+            if (hipMemcpy(sd.d_a32 + lo, v32.data(), v32.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+                throw std::runtime_error("hipMemcpy failed");
+        }
+    }
+}
+
+void sep_side_free(SepSide& sd) {
+    if (sd.d_runs) hipFree(sd.d_runs);
+    if (sd.d_a64) hipFree(sd.d_a64);
+    if (sd.d_a32) hipFree(sd.d_a32);
+    sd.d_runs = nullptr;
+    sd.d_a64 = nullptr;
+    sd.d_a32 = nullptr;
+}
+
+// Patterns, runs and values of both orientations, the flop counts, and the plan's
+// own launch tables (a batch of one, forward and adjoint).
+void sep_build(kmx_plan* p, kmx_sweep& sw) {
+    sw.keep.clear();
+    sw.kept_blocks = sw.total_blocks = 0;
+    for (size_t k = 0; k < sw.desc.size(); ++k) {
+        sw.keep.push_back(sep_pattern(p, sw, int(k)));
+        for (unsigned char c : sw.keep.back()) sw.kept_blocks += c;
+        sw.total_blocks += (long long)sw.keep.back().size();
+    }
+    for (int s = 0; s < 2; ++s) {
+        if (sw.side[s].d_runs) {
+            sw.retired.push_back(sw.side[s]);
+            sw.side[s].d_runs = nullptr;
+            sw.side[s].d_a64 = nullptr;
+            sw.side[s].d_a32 = nullptr;
+        }
+        sep_runs(p, sw, s);
+        sep_values(p, sw, s);
+    }
+    // flops: kept blocks of each group's leading block, times its columns (useful);
+    // tiles x steps (issued on the matrix cores)
+    const int deg = p->deg;
+    sw.flops = sw.mfma_flops = 0;
+    for (size_t k = 0; k < sw.desc.size(); ++k) {
+        const int d = sw.desc[k].a.d, nbk = p->nb[d];
+        for (int gi = sw.group0[d]; gi < sw.group0[d + 1]; ++gi) {
+            const Group& g = sw.groups[size_t(gi)];
+            const long long cols = (long long)g.O * g.I;
+            const int nbm = int((deg << g.m) / deg);
+            long long kb = 0;
+            for (int I = 0; I < nbm; ++I)
+                for (int J = 0; J < nbm; ++J) kb += sw.keep[k][size_t(I) * nbk + J];
+            sw.flops += 2LL * kb * deg * deg * cols;
+            const int n = deg << g.m;
+            for (int m0 = 0; m0 < n; m0 += 16) {
+                int nrun, steps;
+                panel_prefix(sw.side[0], int(k), m0 / 16, g.m, nrun, steps);
+                sw.mfma_flops += 2LL * 16 * 16 * 4 * steps * ((cols + 15) / 16);
+            }
+        }
+    }
+    if (sw.self) sw.retired_batches.push_back(sw.self);
+    if (sw.selfT) sw.retired_batches.push_back(sw.selfT);
+    sw.self = sw.selfT = nullptr;
+    sw.self = kmx_sweep_batch_build({p}, sweep_wave_batches(1));
+    sw.selfT = kmx_sweep_batch_build({p}, sweep_wave_batches(1), true);
+}
+
+}  // namespace
 
 void kmx_sweep_build(kmx_plan* p) {
     kmx_blockset* bs = p->bs;
@@ -1060,46 +1298,73 @@ void kmx_sweep_build(kmx_plan* p) {
         long long flops = 0;
         // separable terms: each acts in one dimension (a term that acts in none is
         // taken as acting in the first); the terms of dimension d share one whole-
-        // operator op on d that reads x and writes (first op) or adds to y
+        // operator op on d.  A factor counts as the identity when the caller declared
+        // it so or it is within identity_tol of I; FP64 applies need every undeclared
+        // one within 4 ulps.  The identity dimensions' connections must hold every
+        // present 1D index (mask[I][I] = 1), since the separable form keeps only J = I
+        // there.
+        const double exact_tol = 4.0 * 2.220446049250313e-16;
+        const double near_tol = env_double("KMX_SWEEP_IDENTITY_TOL", 1e-11);
         std::vector<char> ident(size_t(T) * D, 0);
         std::vector<int> active(T, 0);
         bool separable = !p->h_blk64.empty();
         if (const char* e = std::getenv("KMX_SWEEP_SEPARABLE")) separable = separable && std::atoi(e) != 0;
+        bool fp64 = true;
+        double idev = 0.0;
         for (int t = 0; t < T && separable; ++t) {
             int na = 0;
             for (int k = 0; k < D; ++k) {
-                ident[size_t(t) * D + k] = identity_factor(p, t, k) ? 1 : 0;
+                const bool declared = !p->declared_identity.empty() && p->declared_identity[size_t(t) * D + k];
+                const double dev = declared ? 0.0 : identity_dev(p, t, k);
+                ident[size_t(t) * D + k] = (declared || dev <= near_tol) ? 1 : 0;
                 if (!ident[size_t(t) * D + k]) {
                     active[t] = k;
                     ++na;
+                } else if (!declared) {
+                    idev = std::max(idev, dev);
+                    if (dev > exact_tol) fp64 = false;
                 }
             }
             separable = na <= 1;
         }
+        if (separable) {  // every present 1D index connected to itself in the identity dimensions
+            std::vector<char> idim(D, 0);
+            for (int t = 0; t < T; ++t)
+                for (int k = 0; k < D; ++k)
+                    if (ident[size_t(t) * D + k] && k != active[t]) idim[k] = 1;
+            for (int e = 0; e < p->N && separable; ++e)
+                for (int k = 0; k < D; ++k) {
+                    if (!idim[k]) continue;
+                    const int I = p->h_idx1d[size_t(e) * D + k];
+                    if (!bs->mask[size_t(k)][size_t(I) * p->nb[k] + I]) {
+                        separable = false;
+                        break;
+                    }
+                }
+        }
         if (separable) {
             sw->separable = true;
             sw->identity = ident;
+            sw->fp64 = fp64;
+            sw->identity_dev = idev;
+            sw->drop_tol = env_double("KMX_SWEEP_DROP_TOL", 1e-15);
             int nsep = 0;
-            long long comb_elems = 0;
             for (int d = 0; d < D; ++d) {
                 std::vector<int> ts;
                 for (int t = 0; t < T; ++t)
                     if (active[t] == d) ts.push_back(t);
                 if (ts.empty()) continue;
                 UdOp a = base_op(d);
-                a.T = 1;  // the dimension's terms, folded into one combined matrix
+                a.T = 1;  // the dimension's terms, folded into one combined operator
                 for (int i = 0; i < KMX_MAX_TERMS; ++i) {
-                    a.a_off[i] = i == 0 ? comb_elems : 0;
+                    a.a_off[i] = 0;  // op i's storage offset, set when its runs are laid out
                     a.scale[i] = 1.0;
                 }
-                sw->comb_off.push_back(comb_elems);
-                comb_elems += (long long)p->n[d] * p->n[d];
                 a.x0 = 0;
-                a.in_base = 0;                         // slot 0: the box copy of x
+                a.in_base = 0;                            // slot 0: the box copy of x
                 a.out_base = (1 + nsep) * (long long)nb;  // slot 1 + k: op k's sum
                 ++nsep;
                 add_op(UD_SEP, false, a, 1, ts);
-                flops += fl_full[d];
             }
         } else if (D == 1) {
             UdOp a = base_op(0);
@@ -1139,12 +1404,12 @@ void kmx_sweep_build(kmx_plan* p) {
         }
         if (sw->separable) sw->arena = (1 + (long long)sw->desc.size()) * nb;  // x's box copy, one slot per op
         sw->flops = flops;
+        sw->mfma_flops = flops;
         sw->deg = deg;
         sw->T = T;
         sw->max_nbr = max_nbr;
         sw->groups = groups;
         sw->group0 = group0;
-        emit_records(*sw, sweep_wave_batches(1), recs, sw->ops);
         // ---- box maps: x -> boxes (read side) and boxes -> y
         std::vector<int> xmap(size_t(nb), -1);
         std::vector<int> ymap(size_t(nb), -1);
@@ -1178,14 +1443,6 @@ void kmx_sweep_build(kmx_plan* p) {
                 ymap[size_t(pos)] = e * p->B + a;
             }
         }
-        check_records(*sw, recs, sw->ops);
-        // records interleaved so that lane j reads ints j and 64 + j as one int2
-        std::vector<int> flat(recs.size() * UD_REC);
-        for (size_t i = 0; i < recs.size(); ++i)
-            for (int j = 0; j < UD_REC / 2; ++j) {
-                flat[i * UD_REC + 2 * j] = recs[i][j];
-                flat[i * UD_REC + 2 * j + 1] = recs[i][UD_REC / 2 + j];
-            }
         auto up = [](const auto* h, size_t n, auto*& d) {
             if (hipMalloc(&d, std::max<size_t>(1, n) * sizeof(*h)) != hipSuccess) throw std::bad_alloc();
             if (n && hipMemcpy(d, h, n * sizeof(*h), hipMemcpyHostToDevice) != hipSuccess)
@@ -1193,28 +1450,27 @@ void kmx_sweep_build(kmx_plan* p) {
         };
         up(xmap.data(), xmap.size(), sw->d_xmap);
         up(ymap.data(), ymap.size(), sw->d_ymap);
-        up(flat.data(), flat.size(), sw->d_recs);
         if (hipMalloc(&sw->d_arena[0], size_t(sw->arena) * sizeof(double)) != hipSuccess) throw std::bad_alloc();
         if (bs->d_mat32 && hipMalloc(&sw->d_arena[1], size_t(sw->arena) * sizeof(float)) != hipSuccess)
             throw std::bad_alloc();
         p->sw = sw;
         if (sw->separable) {
-            const long long ce = sw->comb_off.empty() ? 0 : sw->comb_off.back() + (long long)p->n[sw->desc.back().a.d] *
-                                                                                   p->n[sw->desc.back().a.d];
-            for (size_t k = 0; k < sw->desc.size(); ++k)
-                if (sw->desc[k].a.a_off[0] != sw->comb_off[k] ||
-                    sw->comb_off[k] + (long long)p->n[sw->desc[k].a.d] * p->n[sw->desc[k].a.d] > ce)
-                    throw std::logic_error("sweep combined matrix offsets");
-            const size_t cn = size_t(std::max(1LL, ce));
-            if (hipMalloc(&sw->d_comb64, cn * sizeof(double)) != hipSuccess ||
-                hipMalloc(&sw->d_combT64, cn * sizeof(double)) != hipSuccess)
-                throw std::bad_alloc();
-            if (bs->d_mat32 && (hipMalloc(&sw->d_comb32, cn * sizeof(float)) != hipSuccess ||
-                                hipMalloc(&sw->d_combT32, cn * sizeof(float)) != hipSuccess))
-                throw std::bad_alloc();
-            if (combine_matrices(p) != KMX_OK) throw std::runtime_error("combining the separable terms failed");
-            sw->self = kmx_sweep_batch_build({p}, sweep_wave_batches(1));
-            sw->selfT = kmx_sweep_batch_build({p}, sweep_wave_batches(1), true);
+            // the box copy of x: absent cells stay zero (the gathers write present cells only)
+            if (hipMemset(sw->d_arena[0], 0, size_t(nb) * sizeof(double)) != hipSuccess ||
+                (sw->d_arena[1] && hipMemset(sw->d_arena[1], 0, size_t(nb) * sizeof(float)) != hipSuccess))
+                throw std::runtime_error("hipMemset failed");
+            sep_build(p, *sw);
+        } else {
+            emit_records(*sw, sweep_wave_batches(1), recs, sw->ops);
+            check_records(*sw, recs, sw->ops);
+            // records interleaved so that lane j reads ints j and 64 + j as one int2
+            std::vector<int> flat(recs.size() * UD_REC);
+            for (size_t i = 0; i < recs.size(); ++i)
+                for (int j = 0; j < UD_REC / 2; ++j) {
+                    flat[i * UD_REC + 2 * j] = recs[i][j];
+                    flat[i * UD_REC + 2 * j + 1] = recs[i][UD_REC / 2 + j];
+                }
+            up(flat.data(), flat.size(), sw->d_recs);
         }
     } catch (const std::exception&) {
         p->sw = sw;
@@ -1228,6 +1484,11 @@ int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream
         if (!sw->self) {
             set_error("kmx_apply: separable sweep without its launch tables");
             return KMX_E_ARG;
+        }
+        if (slot == 0 && !sw->fp64) {
+            set_error("kmx_apply: this plan's separable sweep takes near-identity factors as the identity "
+                      "(FP32 applies only); declare them in kmx_plan_desc.identity or switch the FP64 sweep off");
+            return KMX_E_UNSUPPORTED;
         }
         BatchIO io;
         std::memset(&io, 0, sizeof(io));
@@ -1245,7 +1506,7 @@ int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream
         using T = std::remove_pointer_t<decltype(arena)>;
         for (const auto& op : sw->ops) {
             UdOp a = op.a;
-            if (op.final)  // the ops that scale their terms (a separable op's scales are in its matrix)
+            if (op.final)  // the ops that scale their terms
                 for (int i = 0; i < a.T; ++i) a.scale[i] = p->scale[op.terms[size_t(i)]];
             const dim3 grid(unsigned(op.wg1 - op.wg0)), block(UD_WAVES * 64);
             const int2* rc = reinterpret_cast<const int2*>(sw->d_recs) + size_t(op.wg0) * UD_WAVES * (UD_REC / 2);
@@ -1283,10 +1544,10 @@ void kmx_sweep_free(kmx_plan* p) {
     if (!sw) return;
     kmx_sweep_batch_free(sw->self);
     kmx_sweep_batch_free(sw->selfT);
-    if (sw->d_comb64) hipFree(sw->d_comb64);
-    if (sw->d_comb32) hipFree(sw->d_comb32);
-    if (sw->d_combT64) hipFree(sw->d_combT64);
-    if (sw->d_combT32) hipFree(sw->d_combT32);
+    sep_side_free(sw->side[0]);
+    sep_side_free(sw->side[1]);
+    for (SepSide& r : sw->retired) sep_side_free(r);
+    for (kmx_sweep_batch* b : sw->retired_batches) kmx_sweep_batch_free(b);
     if (sw->d_xmap) hipFree(sw->d_xmap);
     if (sw->d_ymap) hipFree(sw->d_ymap);
     if (sw->d_recs) hipFree(sw->d_recs);
@@ -1297,6 +1558,25 @@ void kmx_sweep_free(kmx_plan* p) {
 }
 
 double kmx_sweep_flops(const kmx_plan* p) { return p->sw ? double(p->sw->flops) : 0.0; }
+
+bool kmx_sweep_fp64_ok(const kmx_plan* p) { return p->sw && (!p->sw->separable || p->sw->fp64); }
+
+void kmx_sweep_get_info(const kmx_plan* p, kmx_sweep_info* o) {
+    std::memset(o, 0, sizeof(*o));
+    const kmx_sweep* sw = p->sw;
+    if (!sw) return;
+    o->available = 1;
+    o->separable = sw->separable ? 1 : 0;
+    o->fp64 = kmx_sweep_fp64_ok(p) ? 1 : 0;
+    o->identity_dev = sw->identity_dev;
+    o->drop_tol = sw->drop_tol;
+    o->flops = double(sw->flops);
+    o->mfma_flops = double(sw->mfma_flops);
+    o->kept_blocks = sw->separable ? sw->kept_blocks : 0;
+    o->total_blocks = sw->separable ? sw->total_blocks : 0;
+    o->operator_values = sw->separable ? sw->side[0].a_elems : p->mat_elems;
+    o->box_values = sw->nb;
+}
 
 
 // ------------------------------------------------------------------ batches
@@ -1320,10 +1600,10 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
     for (const kmx_plan* p : members)
         if (transpose && (!p || !p->sw || !p->sw->separable))
             throw std::invalid_argument("the adjoint sweep needs the separable form");
-    // load batches per wave (0: a batch's defaults): separable members eight (64 adapted
-    // 2D / 3D instances, FP32: 4 +9 %, 12 +-0, 16 +3 %, 32 +22 %), general members four
+    // load batches per wave (0: a batch's defaults): separable members eight, general members four
     const int wb_sep = wave_batches > 0 ? wave_batches : sweep_wave_batches(8);
     const int wb_gen = wave_batches > 0 ? wave_batches : sweep_wave_batches(4);
+    const int side = transpose ? 1 : 0;
     for (size_t q = 0; q < members.size(); ++q) {
         const kmx_plan* p = members[q];
         if (!p || !p->sw || p->sw->desc.empty()) throw std::invalid_argument("sweep batch member without a sweep");
@@ -1337,9 +1617,10 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
     std::vector<int> ngroups(members.size());
     int nph = 0;
     for (size_t q = 0; q < members.size(); ++q) {
-        emit_records(*members[q]->sw, members[q]->sw->separable ? wb_sep : wb_gen, recs[q], mops[q]);
-        check_records(*members[q]->sw, recs[q], mops[q]);
-        ngroups[q] = members[q]->sw->separable ? 1 : int(mops[q].size());
+        const kmx_sweep& sw = *members[q]->sw;
+        emit_records(sw, sw.separable ? wb_sep : wb_gen, recs[q], mops[q], side);
+        check_records(sw, recs[q], mops[q], side);
+        ngroups[q] = sw.separable ? 1 : int(mops[q].size());
         nph = std::max(nph, ngroups[q]);
     }
     auto* sb = new kmx_sweep_batch();
@@ -1360,14 +1641,13 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                     const kmx_sweep::Op& op = mops[q][k];
                     UdBatchOp o{};
                     o.a = op.a;
-                    if (op.final)  // the ops that scale their terms (a separable op's scales are in its matrix)
+                    if (op.final)  // the ops that scale their terms (a separable op's scales are in its storage)
                         for (int i = 0; i < o.a.T; ++i) o.a.scale[i] = p->scale[op.terms[size_t(i)]];
                     const bool sep = op.part == UD_SEP;
-                    o.mat[0] = sep ? static_cast<const void*>(transpose ? sw->d_combT64 : sw->d_comb64)
-                                   : p->bs->d_mat64;
-                    o.mat[1] = sep ? static_cast<const void*>(transpose ? sw->d_combT32 : sw->d_comb32)
-                                   : p->bs->d_mat32;
-                    if (sep && !sw->d_comb32) sb->has32 = false;
+                    o.mat[0] = sep ? static_cast<const void*>(sw->side[side].d_a64) : p->bs->d_mat64;
+                    o.mat[1] = sep ? static_cast<const void*>(sw->side[side].d_a32) : p->bs->d_mat32;
+                    o.runs = sep ? sw->side[side].d_runs : nullptr;
+                    if (sep && !sw->side[side].d_a32) sb->has32 = false;
                     o.arena[0] = sw->d_arena[0];
                     o.arena[1] = sw->d_arena[1];
                     if (!sw->d_arena[1] || !p->bs->d_mat32) sb->has32 = false;
@@ -1398,8 +1678,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                 // Workgroups in classes of their longest wave's steps (powers of two), longer
                 // classes first, emission order (instance by instance) inside a class: the long
                 // tiles of every instance start early without scattering each instance's tiles
-                // over the whole launch.  On 64 adapted 2D / 3D instances: FP32 -2 %, FP64 -8 %;
-                // a strict longest-first order: FP32 +2 %, FP64 -8 %.
+                // over the whole launch.
                 const size_t rw = size_t(UD_WAVES) * UD_REC;  // ints per workgroup
                 auto rec_int = [&](size_t wg, int w, int k) {  // int k of wave w's record
                     const size_t b = wg * rw + size_t(w) * UD_REC;
@@ -1411,8 +1690,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                     for (int w = 0; w < UD_WAVES; ++w) {
                         const size_t wg = size_t(P.wg0 + g);
                         if (rec_int(wg, w, R_GSIZE) == 0 || rec_int(wg, w, R_TN) == 0) continue;
-                        const long long st = (long long)(rec_int(wg, w, R_TN) - 1) * rec_int(wg, w, R_SPER) +
-                                             rec_int(wg, w, R_SHI) - rec_int(wg, w, R_SLO);
+                        const long long st = rec_int(wg, w, R_SHI) - rec_int(wg, w, R_SLO);
                         most = std::max(most, st);
                     }
                     while (most > 0) {
@@ -1523,28 +1801,50 @@ int kmx_sweep_batch_phases(const kmx_sweep_batch* sb) { return sb ? int(sb->phas
 void kmx_sweep_term_updated(kmx_plan* p, int term, int dim) {
     kmx_sweep* sw = p->sw;
     if (!sw || !sw->separable) return;
-    if (sw->identity[size_t(term) * p->D + dim] && !identity_factor(p, term, dim)) sw->stale = true;
-    for (size_t k = 0; k < sw->desc.size(); ++k)
-        if (sw->desc[k].a.d == dim &&
-            std::find(sw->desc[k].terms.begin(), sw->desc[k].terms.end(), term) != sw->desc[k].terms.end())
-            if (combine_matrices(p, int(k)) != KMX_OK) throw std::runtime_error("combining the separable terms failed");
+    const bool declared = !p->declared_identity.empty() && p->declared_identity[size_t(term) * p->D + dim];
+    if (sw->identity[size_t(term) * p->D + dim] && !declared) {
+        const double dev = identity_dev(p, term, dim);
+        if (dev > env_double("KMX_SWEEP_IDENTITY_TOL", 1e-11)) sw->stale = true;
+        else if (dev > 4.0 * 2.220446049250313e-16) sw->fp64 = false;
+        sw->identity_dev = std::max(sw->identity_dev, dev);
+    }
+    for (size_t k = 0; k < sw->desc.size(); ++k) {
+        if (sw->desc[k].a.d != dim ||
+            std::find(sw->desc[k].terms.begin(), sw->desc[k].terms.end(), term) == sw->desc[k].terms.end())
+            continue;
+        if (sep_pattern(p, *sw, int(k)) == sw->keep[k]) {  // same kept blocks: new values in place
+            sep_values(p, *sw, 0, int(k));
+            sep_values(p, *sw, 1, int(k));
+        } else {  // the kept blocks moved: new runs, storage and launch tables
+            sep_build(p, *sw);
+        }
+    }
 }
 
 int kmx_sweep_scales_updated(kmx_plan* p, int term) {
     kmx_sweep* sw = p->sw;
     if (!sw || !sw->separable) return KMX_OK;
-    for (size_t k = 0; k < sw->desc.size(); ++k)  // the one op whose combined matrix holds the term
-        if (std::find(sw->desc[k].terms.begin(), sw->desc[k].terms.end(), term) != sw->desc[k].terms.end()) {
-            const int rc = combine_matrices(p, int(k));
-            if (rc != KMX_OK) set_error("kmx_plan_set_term_scale: recombining the separable terms failed");
-            return rc;
-        }
+    try {
+        for (size_t k = 0; k < sw->desc.size(); ++k)  // the one op whose storage holds the term
+            if (std::find(sw->desc[k].terms.begin(), sw->desc[k].terms.end(), term) != sw->desc[k].terms.end()) {
+                sep_values(p, *sw, 0, int(k));
+                sep_values(p, *sw, 1, int(k));
+            }
+    } catch (const std::exception& e) {
+        set_error(std::string("kmx_plan_set_term_scale: refreshing the separable operator failed: ") + e.what());
+        return KMX_E_HIP;
+    }
     return KMX_OK;
 }
 
 bool kmx_sweep_has_transpose(const kmx_plan* p) { return p->sw && p->sw->selfT; }
 
 int kmx_sweep_apply_transpose(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s) {
+    if (slot == 0 && !p->sw->fp64) {
+        set_error("kmx_apply_transpose: this plan's separable sweep takes near-identity factors as the identity "
+                  "(FP32 applies only)");
+        return KMX_E_UNSUPPORTED;
+    }
     BatchIO io;
     std::memset(&io, 0, sizeof(io));
     io.x[0] = x;

@@ -57,6 +57,23 @@ class PlanDesc(C.Structure):
         ("cols", C.POINTER(C.c_int)),
         ("index_base", C.c_int),
         ("flags", C.c_uint),
+        ("identity", C.POINTER(C.c_ubyte)),
+    ]
+
+
+class SweepInfo(C.Structure):
+    _fields_ = [
+        ("available", C.c_int),
+        ("separable", C.c_int),
+        ("fp64", C.c_int),
+        ("identity_dev", C.c_double),
+        ("drop_tol", C.c_double),
+        ("flops", C.c_double),
+        ("mfma_flops", C.c_double),
+        ("kept_blocks", C.c_int64),
+        ("total_blocks", C.c_int64),
+        ("operator_values", C.c_int64),
+        ("box_values", C.c_int64),
     ]
 
 
@@ -94,6 +111,7 @@ PROTOTYPES = {
     "kmx_plan_set_sweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "kmx_plan_get_sweep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_double)]),
+    "kmx_plan_sweep_info": (C.c_int, [C.c_void_p, C.POINTER(SweepInfo)]),
     "kmx_realspace_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
                                        C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "kmx_realspace_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -7,8 +7,7 @@ batched apply launch (kmx_batch_apply) per RK3 stage, and the whole RK3 step
 once in a HIP graph and replayed.
 
 `make_instances` draws the instances from a seed: d in {2, 3}, deg in {2, 3, 4},
-level uniform in [4, 8] for d = 2 and [4, 6] for d = 3, term scales uniform in
-[0.5, 2], and an adapted grid obtained from the sparse grid by one refinement
+level uniform in [4, 8], term scales uniform in [0.5, 2], and an adapted grid obtained from the sparse grid by one refinement
 sweep that keeps each method-1 child (get_child_elements.m) with probability
 0.6.  Adapted grids use all element pairs (the reference does not combine
 adaptivity with connectivity, OPTS.m:175-177).
@@ -31,7 +30,7 @@ def _operator(nd, lev_build, deg, nu):
     return _MATS[key]
 
 
-def make_instances(count=64, seed=0, keep_prob=0.6, lev2d=(4, 8), lev3d=(4, 6), degs=(2, 3, 4), nu=0.02,
+def make_instances(count=64, seed=0, keep_prob=0.6, lev2d=(4, 8), lev3d=(4, 8), degs=(2, 3, 4), nu=0.02,
                    cfl=0.01):
     rng = np.random.default_rng(seed)
     out = []

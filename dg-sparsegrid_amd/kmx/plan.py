@@ -64,11 +64,15 @@ class Plan:
     adjoint   keep the transposed pair list (KMX_PLAN_ADJOINT)
     level_blocks  also build the level-block structure (KMX_PLAN_LEVEL_BLOCKS):
               block pairs of the pair list may then run as mode products
-              (set_split / autotune); off, every apply runs pair by pair
+              (set_split / autotune) and the dimension sweep may be built;
+              off, every apply runs pair by pair
+    identity  None, or T x d booleans declaring factors the identity (MASS
+              pterms with g = 1): the separable sweep takes them as exactly I
+              (kmx_plan_desc.identity)
     """
 
     def __init__(self, mats, idx1d, deg, row_ptr=None, cols=None, index_base=1, fp32=True, adjoint=False,
-                 level_blocks=False):
+                 level_blocks=False, identity=None):
         _torch().cuda.current_device()  # make sure the HIP context exists
         self.T = len(mats)
         self.ndim = len(mats[0])
@@ -102,6 +106,9 @@ class Plan:
         desc.index_base = int(index_base)
         desc.flags = ((_lib.PLAN_FP32 if fp32 else 0) | (_lib.PLAN_ADJOINT if adjoint else 0)
                       | (_lib.PLAN_LEVEL_BLOCKS if level_blocks else 0))
+        if identity is not None:
+            ident = np.ascontiguousarray(np.asarray(identity, dtype=np.uint8).reshape(self.T * self.ndim))
+            desc.identity = ident.ctypes.data_as(C.POINTER(C.c_ubyte))
         h = C.c_void_p()
         check(lib().kmx_plan_create(C.byref(desc), C.byref(h)))
         self._h = h
@@ -174,6 +181,15 @@ class Plan:
         av, ac, fl = C.c_int(), C.c_int(), C.c_double()
         check(lib().kmx_plan_get_sweep(self.handle, _prec(prec), C.byref(av), C.byref(ac), C.byref(fl)))
         return {"available": bool(av.value), "active": bool(ac.value), "flops": fl.value}
+
+    def sweep_info(self):
+        """The dimension sweep's form and storage (kmx_plan_sweep_info)."""
+        info = _lib.SweepInfo()
+        check(lib().kmx_plan_sweep_info(self.handle, C.byref(info)))
+        out = {f: getattr(info, f) for f, _ in _lib.SweepInfo._fields_}
+        for f in ("available", "separable", "fp64"):
+            out[f] = bool(out[f])
+        return out
 
     def split(self, prec="fp64"):
         st, beta = C.c_int(), C.c_double()

@@ -73,6 +73,12 @@ typedef struct {
     const int* cols;           /* [row_ptr[N]] connected element ids per row       */
     int index_base;            /* 1: elem_idx1d and cols are 1-based (MATLAB); 0: C */
     unsigned flags;            /* KMX_PLAN_* */
+    /* [T*ndim] or NULL: identity[t*ndim+d] = 1 declares factor (t, d) the identity
+     * -- a MASS pterm with g = 1 whose M\M (pterm_coeff_matrix.m:20) is I up to
+     * roundoff.  The pair-by-pair path still applies the matrix as given; the
+     * separable dimension sweep takes it as exactly I, for FP64 applies too.
+     * kmx_plan_create rejects a declared factor with max |A - I| > 1e-8. */
+    const unsigned char* identity;
 } kmx_plan_desc;
 
 /* Build a resident plan (coefficient blocks, element table, pair list stay on
@@ -85,9 +91,11 @@ int kmx_plan_create(const kmx_plan_desc* desc, kmx_plan** plan);
 int kmx_plan_update_term(kmx_plan* plan, int term, int dim, const double* mat, void* stream);
 
 /* Scale term t's contribution by alpha (default 1).  Used for per-term
- * coefficient scalings (batched instances, sensitivities).  On a plan whose
- * dimension sweep has the separable form it waits for the device and refreshes
- * the sweep's launch table; not under graph capture. */
+ * coefficient scalings (batched instances, sensitivities).  The new scale is used
+ * by every later apply of the plan, and by every batch holding the plan from its
+ * next kmx_batch_apply on (which then refreshes the batch's copies, synchronously).
+ * On a plan whose dimension sweep has the separable form it waits for the device
+ * and rewrites the sweep's operator storage in place.  Not under graph capture. */
 int kmx_plan_set_term_scale(kmx_plan* plan, int term, double alpha);
 
 int kmx_plan_destroy(kmx_plan* plan);
@@ -138,6 +146,34 @@ int kmx_plan_get_split(const kmx_plan* plan, int prec, int* structured, double* 
 int kmx_plan_set_sweep(kmx_plan* plan, int prec, int on);
 int kmx_plan_get_sweep(const kmx_plan* plan, int prec, int* available, int* active, double* flops);
 
+/* The separable form of the sweep (every term acts in one dimension; its other
+ * factors are the identity).  A factor counts as the identity when declared so
+ * (kmx_plan_desc.identity) or within 1e-11 of I entrywise; FP64 applies may use
+ * the form only when every undeclared such factor is I to 4 ulps (fp64 = 1),
+ * FP32 applies always (the difference, identity_dev times the other factors, is
+ * far below FP32 rounding; the reference's own MASS factors M\M deviate by
+ * 1e-14 .. 3e-13).  Each dimension's terms are summed, scales included, into one
+ * combined operator kept block-sparse: a deg x deg block is dropped when its
+ * largest entry is at most drop_tol (1e-15) times the largest entry of every
+ * term's matrix that has it -- in the wavelet basis the blocks between cells of
+ * disjoint support are zero up to the transform's roundoff (1e-20 .. 1e-17 of
+ * the largest entry), so what is dropped is roundoff.  No counterpart in the
+ * reference. */
+typedef struct {
+    int available;          /* the plan has a dimension-sweep path             */
+    int separable;          /* ... in the separable form                        */
+    int fp64;               /* FP64 applies may use it                          */
+    double identity_dev;    /* largest max |A - I| of the undeclared identity factors */
+    double drop_tol;        /* block drop tolerance (separable form)            */
+    double flops;           /* useful flops per apply (kept blocks)             */
+    double mfma_flops;      /* matrix-core flops issued per apply               */
+    int64_t kept_blocks;    /* kept / all deg x deg blocks of the combined operators */
+    int64_t total_blocks;
+    int64_t operator_values; /* values of operator storage an apply reads        */
+    int64_t box_values;     /* values of one intermediate (every level box)     */
+} kmx_sweep_info;
+int kmx_plan_sweep_info(const kmx_plan* plan, kmx_sweep_info* info);
+
 /* num_pairs = P, block = deg^d, dof = N*deg^d. */
 int kmx_plan_info(const kmx_plan* plan, int64_t* num_pairs, int* block, int64_t* dof);
 
@@ -172,9 +208,10 @@ int kmx_batch_destroy(kmx_batch* batch);
  * the sweeps of all such plans share one launch per phase (a 3D plan has five
  * phases: two down, one whole operator, two up); the other plans keep the
  * persistent pair launch.  A plan listed twice runs its sweep once per apply
- * for its first slot only; the others stay on the pair path.  Term scales are
- * those set when the sweep is first switched on.  on = 0 goes back to the pair
- * path.  Synchronous (allocates); not under graph capture.  KMX_E_UNSUPPORTED
+ * for its first slot only; the others stay on the pair path.  FP64 merges only
+ * the plans whose sweep allows FP64 (kmx_plan_sweep_info fp64).  Term scales and
+ * matrices are the plans' current ones at every apply (kmx_plan_set_term_scale).
+ * on = 0 goes back to the pair path.  Synchronous (allocates); not under graph capture.  KMX_E_UNSUPPORTED
  * when no plan has a sweep path.  get: plans in the merged sweep, whether
  * prec uses it, and its launches per apply.  No counterpart in the reference. */
 int kmx_batch_set_sweep(kmx_batch* batch, int prec, int on);
