@@ -190,6 +190,9 @@ def time_batch_apply(sysm, prec, warmup, steps, per_graph=10):
     return e0.elapsed_time(e1) / (reps * per_graph)
 
 
+STEPS_PER_REPLAY = 10  # most fused RK3 steps per graph replay (the states are read in once per replay)
+
+
 def timed_steps(g, n):
     """ms per RK3 step over n graph replays (HIP events on the replay stream)."""
     import torch
@@ -211,10 +214,12 @@ def batched_leg(steps=50, warmup=5, inst=None, err_steps=6):
     Runs: "fp64" / "fp32" = every instance pair by pair in the persistent launch
     (the FP64 one is the plain baseline); "fp32_sweep" / "fp64_sweep" = every
     instance on the merged block-sparse separable sweeps (same terms, other
-    summation order).  Every run first advances err_steps steps from the same
-    initial states (timed: the pair paths' figure); the final states are compared
-    with the FP64 pair path's.  The sweep runs then take warmup + steps further
-    steps, the timed steps giving their figure."""
+    summation order) with the stage arithmetic fused in (kmx_batch_rk3).  Every
+    run first advances err_steps steps from the same initial states (timed: the
+    pair paths' figure); the final states are compared with the FP64 pair path's.
+    The sweep runs then take warmup + steps further steps, STEPS_PER_REPLAY per
+    graph replay, the timed steps giving their figure; the FP32 sweep also times
+    the unfused step (applies + torch vector updates)."""
     import torch
 
     from kmx import batch as BT
@@ -233,7 +238,9 @@ def batched_leg(steps=50, warmup=5, inst=None, err_steps=6):
         pairs = sweep is None
         apply_ms = time_batch_apply(sysm, prec, 1 if pairs else 3, 3 if pairs else max(10, steps // 2),
                                     per_graph=1 if pairs else 10)
-        g = BT.BatchRK3Graph(sysm, prec=prec)
+        # pair paths: RK3 from three batched applies and FP64 vector updates per step;
+        # sweeps: the fused step (kmx_batch_rk3, two launches per stage)
+        g = BT.BatchRK3Graph(sysm, prec=prec) if pairs else BT.BatchRK3Fused(sysm, prec=prec)
         ms_err = timed_steps(g, err_steps)
         states[name] = sysm.f.cpu().numpy()
         run = {"setup_s": round(setup_s, 1), "ms_per_batched_apply": round(apply_ms, 4),
@@ -242,12 +249,22 @@ def batched_leg(steps=50, warmup=5, inst=None, err_steps=6):
             run["ms_per_rk3_step"] = round(ms_err, 4)
             run["timed_steps"] = err_steps
         else:
-            for _ in range(warmup):
-                g.step()
-            ms = timed_steps(g, steps)
+            spr = max(d for d in range(1, STEPS_PER_REPLAY + 1) if steps % d == 0)  # exactly `steps` timed
+            gk = BT.BatchRK3Fused(sysm, prec=prec, steps_per_replay=spr)
+            for _ in range(max(1, warmup // spr)):
+                gk.step()
+            ms = timed_steps(gk, steps // spr) / spr
             run["ms_per_rk3_step"] = round(ms, 4)
             run["timed_steps"] = steps
+            run["steps_per_graph_replay"] = spr
             run["merged_sweep"] = sysm.sweep[prec]
+            if name == "fp32_sweep":  # the same steps unfused: applies + torch vector updates
+                gu = BT.BatchRK3Graph(sysm, prec=prec)
+                for _ in range(warmup):
+                    gu.step()
+                run["ms_per_rk3_step_unfused"] = round(timed_steps(gu, steps), 4)
+                del gu
+            del gk
             if plans_sweep is None:
                 plans_sweep = sysm
         run["instance_applies_per_s"] = round(3 * len(inst) / (run["ms_per_rk3_step"] * 1e-3), 1)
@@ -440,9 +457,10 @@ def main():
     ap.add_argument("--sweep-study", action="store_true",
                     help="only time the pair kernels against the opt-in dimension-sweep path (one JSON line)")
     ap.add_argument("--pmc-pass", action="store_true",
-                    help="for counter runs: one 256 MiB device copy (calibration), 3 applies per precision, "
-                         "3 merged-sweep FP32 batched applies, then 3 level-block applies per precision of the "
-                         "6D plan (unless --no-precision)")
+                    help="for kernel-trace and counter runs: one 256 MiB device copy (calibration), 3 pair-path "
+                         "applies per precision of the single 3D operator (unless --no-single), 3 merged-sweep "
+                         "batched applies and 3 fused RK3 steps per precision of the 64-instance batch, then 3 "
+                         "level-block applies per precision of the 6D plan (unless --no-precision)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {a.gpus} requested); the operator apply is not "
@@ -470,16 +488,22 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        for prec in ("fp64", "fp32"):
-            for _ in range(3):
-                plan.apply(xd, yd, prec=prec)
-        torch.cuda.synchronize()
-        sysm = BT.BatchedSystem(BT.make_instances(count=64, seed=0), sweep=("fp32",))
+        if not a.no_single:  # the pair kernels of the single 3D operator
+            for prec in ("fp64", "fp32"):
+                for _ in range(3):
+                    plan.apply(xd, yd, prec=prec)
+            torch.cuda.synchronize()
+        # the 64-instance batch on the merged separable sweeps: 3 batched applies per
+        # precision (box copy, ops, slot sum), then 3 fused RK3 steps per precision
+        sysm = BT.BatchedSystem(BT.make_instances(count=64, seed=0), sweep=("fp32", "fp64"))
         xb = sysm.f.clone()
         yb = torch.empty_like(xb)
-        for _ in range(3):
-            sysm.apply(xb, yb, "fp32")
-        torch.cuda.synchronize()
+        for prec in ("fp32", "fp64"):
+            for _ in range(3):
+                sysm.apply(xb, yb, prec)
+            torch.cuda.synchronize()
+            sysm.rk3(3, prec, f=xb)
+            torch.cuda.synchronize()
         if not a.no_precision:  # matrix-core kernels: the 6D level-block plan, every block pair factorized
             from kmx import coeffs as K
             from kmx import grid as G

@@ -722,6 +722,36 @@ extern "C" int kmx_batch_set_sweep(kmx_batch* b, int prec, int on) {
     return KMX_OK;
 }
 
+extern "C" int kmx_batch_rk3(kmx_batch* b, double* const* fs, const double* dts, int nsteps, int prec, void* stream) {
+    if (!b || !fs || !dts || nsteps < 0 || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
+        set_error("kmx_batch_rk3: bad arguments");
+        return KMX_E_ARG;
+    }
+    if (nsteps == 0 || b->plans.empty()) return KMX_OK;
+    const int slot = prec_slot(prec);
+    {
+        const int rc = refresh_batch(b);
+        if (rc != KMX_OK) return rc;
+    }
+    const std::vector<int>& mem = b->members[slot];
+    if (!b->sweep_on[slot] || mem.size() != b->plans.size()) {
+        set_error("kmx_batch_rk3: every plan of the batch must be a member of its merged sweep for this precision "
+                  "(kmx_batch_set_sweep; distinct plans)");
+        return KMX_E_UNSUPPORTED;
+    }
+    std::vector<double*> f(mem.size());
+    std::vector<double> dt(mem.size());
+    for (size_t q = 0; q < mem.size(); ++q) {
+        f[q] = fs[mem[q]];
+        dt[q] = dts[mem[q]];
+        if (!f[q] && b->plans[size_t(mem[q])]->N > 0) {
+            set_error("kmx_batch_rk3: null state");
+            return KMX_E_ARG;
+        }
+    }
+    return kmx_sweep_batch_rk3(b->sweep[slot], slot, f.data(), dt.data(), nsteps, reinterpret_cast<hipStream_t>(stream));
+}
+
 extern "C" int kmx_batch_get_sweep(const kmx_batch* b, int prec, int* members, int* active, int* phases) {
     if (!b || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
         set_error("kmx_batch_get_sweep: bad arguments");

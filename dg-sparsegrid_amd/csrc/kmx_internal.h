@@ -224,5 +224,9 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
 bool kmx_sweep_has_transpose(const kmx_plan* p);
 int kmx_sweep_apply_transpose(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s);
 int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const kmx::BatchIO& io, hipStream_t s);
+// nsteps fused RK3 steps of a batch whose members are all separable (fs / dts by
+// member); allocates the stage workspace on its first call only
+int kmx_sweep_batch_rk3(kmx_sweep_batch* sb, int slot, double* const* fs, const double* dts, int nsteps,
+                        hipStream_t s);
 void kmx_sweep_batch_free(kmx_sweep_batch* sb);
 int kmx_sweep_batch_phases(const kmx_sweep_batch* sb);

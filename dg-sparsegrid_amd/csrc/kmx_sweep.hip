@@ -72,6 +72,9 @@ struct SwMfma<float> {
     static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// positions of level l along a dimension: deg * cells(l)
+__device__ __forceinline__ int deg_cells(int deg, int l) { return deg * (l ? 1 << (l - 1) : 1); }
+
 // UD_SEP: one dimension's op of a separable sweep: the whole operators of its terms,
 // summed, from the box copy of x (slot in_base) into the dimension's own slot
 // (out_base); the slots are summed into y afterwards.
@@ -95,7 +98,10 @@ enum {
     UD_REC = 128
 };
 // a separable op's record: its panel's runs in the plan's run table instead of a level range
-enum { R_RUN0 = R_LA, R_NRUN = R_LB };
+enum { R_RUN0 = R_LA, R_NRUN = R_LB, R_NCT = R_TLO };
+// a separable tile whose panel has at most SEP_KMAX steps may sweep R_NCT column
+// tiles of its group with the panel's A values held in registers
+constexpr int SEP_KMAX = 16;
 constexpr int UD_MAXLEV = UD_REC - UD_BOX - 1;  // highest level a record holds a box for
 constexpr int UD_WAVES = 8;                     // waves per workgroup
 
@@ -327,6 +333,7 @@ struct BoxJob {
     const int* ymap;
     void* arena[2];  // FP64, FP32 arena of the plan: slot 0 x's box copy, slots 1..nslot the ops' sums
     long long nb;
+    long long koff;  // the instance's first value in the batch's RK3 stage workspace
     int nslot, inst, blk0;
 };
 template <typename T>
@@ -351,6 +358,58 @@ __global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict_
         for (int k = 0; k < j.nslot; ++k) v += double(sl[(long long)k * j.nb]);
         const int m = j.ymap[i];
         if (m >= 0) io.y[j.inst][m] = v;
+    }
+}
+
+// Explicit RK3 (time_advance.m:151-204: a21 = 1/2, a31 = -1, a32 = 2,
+// b = 1/6, 2/3, 1/6) fused around the merged separable sweeps.  Per stage one
+// launch of the ops and one of this kernel, which, at every present cell of every
+// instance, sums the ops' slots (dimension order, FP64) into the stage derivative
+// k_s and writes what the next launch reads: stage 1 k1 and the box copy of
+// f + dt a21 k1, stage 2 k2 and the box copy of f + dt (a31 k1 + a32 k2), stage 3
+// the new state f + dt (b1 k1 + b2 k2 + b3 k3) and (next_box) its box copy for the
+// next step; stage 0 only copies f into the box.  Absent cells of the box copy are
+// never written (zero since the plan's set-up).
+constexpr double RK_A21 = 0.5, RK_A31 = -1.0, RK_A32 = 2.0;
+constexpr double RK_B1 = 1.0 / 6.0, RK_B2 = 2.0 / 3.0, RK_B3 = 1.0 / 6.0;
+struct Rk3Args {
+    double* f[KMX_BATCH_MAX];
+    double dt[KMX_BATCH_MAX];
+    double* k1;  // stage workspace, instance values at koff
+    double* k2;
+};
+template <typename T, int STAGE>
+__global__ void __launch_bounds__(256) k_box_rk3(const BoxJob* __restrict__ wg_jobs, Rk3Args a, int next_box) {
+    const BoxJob& j = wg_jobs[blockIdx.x];
+    const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
+    if (i >= j.nb) return;
+    T* box = static_cast<T*>(j.arena[sizeof(T) == 8 ? 0 : 1]);
+    double k = 0.0;
+    if (STAGE > 0) {  // the slots first, so their loads and the map's are in flight together
+        const T* sl = box + j.nb + i;
+        for (int q = 0; q < j.nslot; ++q) k += double(sl[(long long)q * j.nb]);
+    }
+    const int m = j.ymap[i];
+    if (m < 0) return;
+    double* f = a.f[j.inst];
+    const double fm = f[m];
+    if (STAGE == 0) {
+        box[i] = T(fm);
+        return;
+    }
+    const double dt = a.dt[j.inst];
+    double* k1 = a.k1 + j.koff;
+    double* k2 = a.k2 + j.koff;
+    if (STAGE == 1) {
+        k1[m] = k;
+        box[i] = T(fm + dt * RK_A21 * k);
+    } else if (STAGE == 2) {
+        k2[m] = k;
+        box[i] = T(fm + dt * (RK_A31 * k1[m] + RK_A32 * k));
+    } else {
+        const double fn = fm + dt * (RK_B1 * k1[m] + RK_B2 * k2[m] + RK_B3 * k);
+        f[m] = fn;
+        if (next_box) box[i] = T(fn);
     }
 }
 
@@ -388,6 +447,9 @@ template <typename T, int NB>
 __device__ __forceinline__ void sep_tile(const T* __restrict__ mat, const int4* __restrict__ runs,
                                          T* __restrict__ arena, const int2 rv, const int w, const int lane,
                                          const UdOp& op, T* __restrict__ part);
+template <typename T>
+__device__ __forceinline__ void sep_tile_cols(const T* __restrict__ mat, const int4* __restrict__ runs,
+                                              T* __restrict__ arena, const int2 rv, const int lane, const UdOp& op);
 
 // One phase of a batch: ops of many instances, each on whole workgroups
 // (wg_op[workgroup] = its op); the workgroup's op kind picks the tile body.
@@ -417,7 +479,12 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
             ud_tile<T, UD_FULL, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part);
             break;
         case UK_UP_FINAL: ud_tile<T, UD_UP, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
-        case UK_SEP: sep_tile<T, 8>(mat, o.runs, arena, rv, w, lane, o.a, part); break;
+        case UK_SEP:
+            if (__builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
+                sep_tile_cols<T>(mat, o.runs, arena, rv, lane, o.a);
+            else
+                sep_tile<T, 8>(mat, o.runs, arena, rv, w, lane, o.a, part);
+            break;
         default:
             if constexpr (MAXNBR == 4) {
                 if (o.kind == UK_DOWN4)
@@ -553,6 +620,106 @@ __device__ __forceinline__ void sep_tile(const T* __restrict__ mat, const int4* 
         if (ook[qq]) out[oaddr[qq]] = acc[qq];
 }
 
+// One wave of a separable op over R_NCT consecutive 16-column tiles of one group
+// and one row panel whose kept steps (at most SEP_KMAX, from one run list, not
+// split over waves) are loaded once: the panel's A values and the lane's B
+// offsets without the column term stay in registers, and each column tile then
+// costs its B loads, its matrix operations and its stores.  Two column tiles are
+// in flight at a time.  Holds one workgroup barrier, like sep_tile.
+template <typename T>
+__device__ __forceinline__ void sep_tile_cols(const T* __restrict__ mat, const int4* __restrict__ runs,
+                                              T* __restrict__ arena, const int2 rv, const int lane, const UdOp& op) {
+    using V = typename SwMfma<T>::V;
+    const int f0 = __builtin_amdgcn_readlane(rv.x, R_F0), nct = __builtin_amdgcn_readlane(rv.x, R_NCT);
+    const int O = __builtin_amdgcn_readlane(rv.x, R_O), I = __builtin_amdgcn_readlane(rv.x, R_I);
+    const unsigned magic = unsigned(__builtin_amdgcn_readlane(rv.x, R_MAGIC));
+    const int li = lane & 15, lk = lane >> 4;
+    int rb[4], rn[4];
+    bool rok[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int row = SwMfma<T>::row(lane, q);
+        rb[q] = __shfl(rv.x, R_ROWBASE + row);
+        rn[q] = __shfl(rv.x, R_ROWNL + row);
+        rok[q] = __shfl(rv.x, R_ROWLEV + row) >= 0;
+    }
+    const int ns = __builtin_amdgcn_readlane(rv.x, R_SHI);  // the panel's steps (from step 0)
+    const int run0 = __builtin_amdgcn_readlane(rv.x, R_RUN0), nrun = __builtin_amdgcn_readlane(rv.x, R_NRUN);
+    const T* amat = mat + op.a_off[0] + lane;
+    const T* bsrc = arena + op.in_base;
+    const int4 myr = runs[run0 + min(lane, nrun - 1)];  // nrun <= SEP_KMAX
+    T av[SEP_KMAX];
+    int bp[SEP_KMAX], nli[SEP_KMAX];
+    bool pb[SEP_KMAX];
+    {
+        int rr = 0, sin = 0, nst = 0, npos = 0, a = 0, base = 0, nl = 0;
+        auto load_run = [&]() {
+            const int l = __builtin_amdgcn_readlane(myr.x, rr);
+            const int pos0 = __builtin_amdgcn_readlane(myr.y, rr);
+            npos = __builtin_amdgcn_readlane(myr.z, rr);
+            a = __builtin_amdgcn_readlane(myr.w, rr);
+            nst = (npos + 3) >> 2;
+            nl = deg_cells(op.deg, l);
+            base = __builtin_amdgcn_readlane(rv.y, min(l, UD_MAXLEV)) + (pos0 + lk) * I;
+        };
+        load_run();
+#pragma unroll
+        for (int c = 0; c < SEP_KMAX; ++c) {
+            const bool on = c < ns;
+            pb[c] = on && 4 * sin + lk < npos;
+            av[c] = on ? amat[a + 64 * sin] : T(0);
+            bp[c] = pb[c] ? base + 4 * sin * I : 0;
+            nli[c] = nl * I;
+            if (on && ++sin == nst && c + 1 < ns) {
+                ++rr;
+                sin = 0;
+                load_run();
+            }
+        }
+    }
+    T* out = arena + op.out_base;
+    for (int ct = 0; ct < nct; ct += 2) {
+        const bool two = ct + 1 < nct;
+        int fo[2], fi[2];
+        bool fok[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int f = f0 + 16 * (ct + u) + li;
+            fok[u] = (u == 0 || two) && f < O * I;
+            const int fc = fok[u] ? f : 0;
+            fo[u] = I == 1 ? fc : magic ? int(__umulhi(unsigned(fc), magic)) : fc / I;
+            fi[u] = fc - fo[u] * I;
+        }
+        V acc[2] = {V{T(0), T(0), T(0), T(0)}, V{T(0), T(0), T(0), T(0)}};
+#pragma unroll
+        for (int c0 = 0; c0 < SEP_KMAX; c0 += 4) {
+            if (c0 < ns) {
+                T bv[2][4];
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const bool ok = pb[c0 + j] && fok[u];
+                        const int ad = ok ? bp[c0 + j] + fo[u] * nli[c0 + j] + fi[u] : 0;
+                        const T v = bsrc[ad];
+                        bv[u][j] = ok ? v : T(0);
+                    }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    acc[0] = SwMfma<T>::run(av[c0 + j], bv[0][j], acc[0]);
+                    acc[1] = SwMfma<T>::run(av[c0 + j], bv[1][j], acc[1]);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (fok[u] && rok[q]) out[rb[q] + fo[u] * rn[q] + fi[u]] = acc[u][q];
+    }
+    __syncthreads();  // the one barrier every wave of the workgroup takes (sep_tile's partial sums)
+}
+
 // A phase of separable ops only (every op UD_SEP, one column tile): the small tile
 // body alone, so the launch keeps the occupancy its registers allow.
 template <typename T>
@@ -565,7 +732,10 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_sep_batch(const UdBatchOp* __
     const int2 rv = recs[((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane];
     const UdBatchOp& o = ops[wg_op[blockIdx.x]];
     constexpr int sl = sizeof(T) == 8 ? 0 : 1;
-    sep_tile<T, 8>(static_cast<const T*>(o.mat[sl]), o.runs, static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
+    if (__builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
+        sep_tile_cols<T>(static_cast<const T*>(o.mat[sl]), o.runs, static_cast<T*>(o.arena[sl]), rv, lane, o.a);
+    else
+        sep_tile<T, 8>(static_cast<const T*>(o.mat[sl]), o.runs, static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
 }
 
 }  // namespace
@@ -604,6 +774,7 @@ struct Group {
 
 struct HostTile {
     int m0, f0, b, t, group, la, lb, nterm, sper, steps, gsize;
+    int nct = 1;  // separable: column tiles this tile's wave sweeps (packed panels)
 };
 
 }  // namespace
@@ -758,6 +929,17 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
                             sper = steps_of(la, lb, deg);
                         }
                         const int steps = nterm * sper;
+                        const int ctiles = (g.O * g.I + 15) / 16;
+                        if (sep && sper >= 1 && sper <= SEP_KMAX && ctiles > 1) {
+                            // a short panel: one wave sweeps up to batch / steps column tiles
+                            const int per = std::max(1, std::min(ctiles, batch / sper));
+                            for (int c0 = 0; c0 < ctiles; c0 += per) {
+                                HostTile h{m0, 16 * c0, int(b), t, gi, la, lb, nterm, sper, steps, 1};
+                                h.nct = std::min(per, ctiles - c0);
+                                ot.push_back(h);
+                            }
+                            continue;
+                        }
                         int gsize = 1;  // waves sharing the tile: at most `batch` steps each
                         while (gsize < UD_WAVES && (steps + gsize - 1) / gsize > batch) gsize *= 2;
                         for (int f0 = 0; f0 < g.O * g.I; f0 += 16)
@@ -765,7 +947,7 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
                     }
                 }
         std::stable_sort(ot.begin(), ot.end(), [](const HostTile& u, const HostTile& v) {
-            return u.gsize != v.gsize ? u.gsize > v.gsize : u.steps > v.steps;
+            return u.gsize != v.gsize ? u.gsize > v.gsize : u.steps * u.nct > v.steps * v.nct;
         });
         for (const HostTile& h : ot) {
             const int slot = int(recs.size() % UD_WAVES);
@@ -788,8 +970,8 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
                 r[R_GBASE] = slot;
                 r[R_MAGIC] = int(g.magic);
                 const int q0 = std::min(h.steps, w * qn), q1 = std::min(h.steps, (w + 1) * qn);
-                if (sep) {  // steps counted over the tile's runs
-                    r[R_TLO] = 0;
+                if (sep) {  // steps counted over the tile's runs; R_NCT column tiles
+                    r[R_NCT] = h.nct;
                     r[R_SLO] = q0;
                     r[R_SHI] = q1;
                     r[R_TN] = q1 > q0 ? 1 : 0;
@@ -917,6 +1099,11 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
                     if (r[R_SPER] != steps || r[R_TN] != (r[R_SHI] > r[R_SLO] ? 1 : 0) || r[R_SLO] < 0 ||
                         r[R_SHI] < r[R_SLO] || r[R_SHI] > steps)
                         throw std::logic_error("sweep separable record steps");
+                    const int nct = r[R_NCT];
+                    if (nct < 1 || r[R_F0] % 16 || r[R_F0] + 16 * (nct - 1) >= g.O * g.I ||
+                        (nct > 1 && (gsize != 1 || r[R_SLO] != 0 || r[R_SHI] != steps || steps < 1 ||
+                                     steps > SEP_KMAX || r[R_NRUN] > SEP_KMAX)))
+                        throw std::logic_error("sweep separable record column tiles");
                     continue;
                 }
                 const int sper = steps_of(r[R_LA], r[R_LB], deg);
@@ -1593,6 +1780,9 @@ struct kmx_sweep_batch {
     int* d_wgop = nullptr;  // op of every workgroup, phases in order
     int* d_recs = nullptr;  // UD_WAVES work records per workgroup
     bool has32 = true;
+    bool all_sep = true;   // every member separable (the fused RK3 step)
+    long long kdof = 0;    // values of the RK3 stage workspace (every member's state)
+    double* d_k = nullptr;  // RK3 stage workspace: k1 then k2, kdof each (first kmx_sweep_batch_rk3)
 };
 
 kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, int wave_batches, bool transpose) {
@@ -1690,7 +1880,8 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                     for (int w = 0; w < UD_WAVES; ++w) {
                         const size_t wg = size_t(P.wg0 + g);
                         if (rec_int(wg, w, R_GSIZE) == 0 || rec_int(wg, w, R_TN) == 0) continue;
-                        const long long st = rec_int(wg, w, R_SHI) - rec_int(wg, w, R_SLO);
+                        const long long st = (long long)(rec_int(wg, w, R_SHI) - rec_int(wg, w, R_SLO)) *
+                                             std::max(1, rec_int(wg, w, R_NCT));
                         most = std::max(most, st);
                     }
                     while (most > 0) {
@@ -1715,18 +1906,23 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
         }
         if (flat.size() != wgop.size() * UD_WAVES * UD_REC) throw std::logic_error("sweep batch records");
         std::vector<BoxJob> jobs, wgjobs;
-        long long blk = 0;
+        long long blk = 0, koff = 0;
         for (size_t q = 0; q < members.size(); ++q) {
             const kmx_sweep* sw = members[q]->sw;
-            if (!sw->separable) continue;
-            jobs.push_back(BoxJob{sw->d_xmap, sw->d_ymap, {sw->d_arena[0], sw->d_arena[1]}, sw->nb,
+            if (!sw->separable) {
+                sb->all_sep = false;
+                continue;
+            }
+            jobs.push_back(BoxJob{sw->d_xmap, sw->d_ymap, {sw->d_arena[0], sw->d_arena[1]}, sw->nb, koff,
                                   int(sw->desc.size()), int(q), int(blk)});
+            koff += (long long)members[q]->N * members[q]->B;
             blk += (sw->nb + 255) / 256;
             if (blk >= (1LL << 31)) throw std::runtime_error("sweep batch too large");
             wgjobs.resize(size_t(blk), jobs.back());
         }
         sb->njobs = int(jobs.size());
         sb->box_wgs = int(blk);
+        sb->kdof = koff;
         auto up = [](const auto* h, size_t n, auto*& d) {
             if (hipMalloc(&d, std::max<size_t>(1, n) * sizeof(*h)) != hipSuccess) throw std::bad_alloc();
             if (n && hipMemcpy(d, h, n * sizeof(*h), hipMemcpyHostToDevice) != hipSuccess)
@@ -1787,8 +1983,73 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
     return KMX_OK;
 }
 
+int kmx_sweep_batch_rk3(kmx_sweep_batch* sb, int slot, double* const* fs, const double* dts, int nsteps,
+                        hipStream_t s) {
+    if (!sb->all_sep || sb->njobs != int(sb->members.size())) {
+        set_error("kmx_batch_rk3: every plan of the batch must run a separable sweep in this precision");
+        return KMX_E_UNSUPPORTED;
+    }
+    if (slot == 1 && !sb->has32) {
+        set_error("kmx_batch_rk3: a sweep member has no FP32 blocks");
+        return KMX_E_ARG;
+    }
+    for (const kmx_plan* p : sb->members)
+        if (p->sw->stale || (slot == 0 && !p->sw->fp64)) {
+            set_error("kmx_batch_rk3: a member's sweep cannot serve this precision (replaced identity factor, or "
+                      "near-identity factors under FP64)");
+            return KMX_E_UNSUPPORTED;
+        }
+    if (!sb->d_k && sb->kdof > 0) {
+        if (hipMalloc(&sb->d_k, size_t(2 * sb->kdof) * sizeof(double)) != hipSuccess) {
+            set_error("kmx_batch_rk3: allocating the stage workspace failed");
+            return KMX_E_NOMEM;
+        }
+    }
+    Rk3Args a;
+    std::memset(&a, 0, sizeof(a));
+    for (size_t q = 0; q < sb->members.size(); ++q) {
+        a.f[q] = fs[q];
+        a.dt[q] = dts[q];
+    }
+    a.k1 = sb->d_k;
+    a.k2 = sb->d_k + sb->kdof;
+    const dim3 bg(unsigned(sb->box_wgs)), bb(256);
+    auto ops = [&]() {
+        for (const auto& P : sb->phases) {
+            if (P.nwg == 0) continue;
+            const int* wgop = sb->d_wgop + P.wg0;
+            const int2* rc = reinterpret_cast<const int2*>(sb->d_recs) + size_t(P.wg0) * UD_WAVES * (UD_REC / 2);
+            const dim3 grid(unsigned(P.nwg)), block(UD_WAVES * 64);
+            if (slot == 0) hipLaunchKernelGGL(k_sep_batch<double>, grid, block, 0, s, sb->d_ops, wgop, rc);
+            else hipLaunchKernelGGL(k_sep_batch<float>, grid, block, 0, s, sb->d_ops, wgop, rc);
+        }
+    };
+    auto stage = [&](auto kern, int next) { hipLaunchKernelGGL(kern, bg, bb, 0, s, sb->d_wgjobs, a, next); };
+    if (slot == 0) stage(k_box_rk3<double, 0>, 0);
+    else stage(k_box_rk3<float, 0>, 0);
+    for (int st = 0; st < nsteps; ++st) {
+        const int next = st + 1 < nsteps ? 1 : 0;
+        ops();
+        if (slot == 0) stage(k_box_rk3<double, 1>, 0);
+        else stage(k_box_rk3<float, 1>, 0);
+        ops();
+        if (slot == 0) stage(k_box_rk3<double, 2>, 0);
+        else stage(k_box_rk3<float, 2>, 0);
+        ops();
+        if (slot == 0) stage(k_box_rk3<double, 3>, next);
+        else stage(k_box_rk3<float, 3>, next);
+    }
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        set_error(std::string("kmx_batch_rk3 launch: ") + hipGetErrorString(err));
+        return KMX_E_HIP;
+    }
+    return KMX_OK;
+}
+
 void kmx_sweep_batch_free(kmx_sweep_batch* sb) {
     if (!sb) return;
+    if (sb->d_k) hipFree(sb->d_k);
     if (sb->d_ops) hipFree(sb->d_ops);
     if (sb->d_wgop) hipFree(sb->d_wgop);
     if (sb->d_recs) hipFree(sb->d_recs);

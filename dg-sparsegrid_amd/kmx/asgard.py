@@ -15,6 +15,7 @@ kmx.grid.global_matrix; connectivity, when used, is pde.connectivity =
 (row_ptr, cols) from kmx.grid.connect_nd.  Vectors may be numpy arrays (a
 host round trip, like the MEX path) or float64 CUDA tensors (resident).
 """
+import hashlib
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -36,7 +37,7 @@ class OPTS:
     use_oldhash: bool = False
     use_connectivity: bool = False
     max_lev: int = 8
-    max_lev_coeffs: bool = True
+    max_lev_coeffs: bool = False  # OPTS.m:25 defaults it on, OPTS.m:168 turns it off without adaptivity
     num_steps: int = 5
     quiet: bool = True
     output_grid: str = "quadrature"  # OPTS.m:30 (matrix_plot_D restates this grid only)
@@ -44,21 +45,44 @@ class OPTS:
     extra: dict = field(default_factory=dict)
 
 
+def _table_key(A_data):
+    """A stable key of the element table (its 1D indices), not an object id."""
+    idx = np.ascontiguousarray(np.asarray(A_data["element_local_index_D"], dtype=np.int32))
+    return idx.shape, hashlib.sha1(idx.tobytes()).hexdigest()
+
+
 def _plan_for(pde, opts, A_data, deg, imex_flag):
-    """Build (and cache on the pde object) the resident plan for this operator."""
-    key = (id(A_data), deg, imex_flag, opts.use_connectivity)
-    cache = pde.__dict__.setdefault("_kmx_plans", {})
-    if key in cache:
-        return cache[key]
+    """The resident plan for this operator, cached on the pde object per (element
+    table, deg, imex flag, connectivity).  The cache entry holds the 1D matrices
+    it was built from (so their identities stay valid): when pde.mats entries are
+    replaced -- the time-dependent refresh of get_coeff_mats -- the changed
+    matrices go to the plan through kmx_plan_update_term, or the plan is rebuilt
+    if the shapes or the connectivity changed."""
     flags = getattr(pde, "imex", None) or ["N"] * len(pde.mats)
     term_vec = [t for t in range(len(pde.mats)) if flags[t] == imex_flag]  # apply_A.m:23-30
-    mats = [pde.mats[t] for t in term_vec]
+    mats = [list(pde.mats[t]) for t in term_vec]
+    conn = pde.connectivity if opts.use_connectivity else None
+    key = (_table_key(A_data), deg, imex_flag, bool(opts.use_connectivity))
+    cache = pde.__dict__.setdefault("_kmx_plans", {})
+    ent = cache.get(key)
+    if ent is not None:
+        plan, held, held_conn = ent
+        if plan is None and not mats:
+            return None
+        if plan is not None and held_conn is conn and len(held) == len(mats) and all(
+                len(a) == len(b) and all(np.shape(x) == np.shape(y) for x, y in zip(a, b)) for a, b in zip(held, mats)):
+            for t, (old, new) in enumerate(zip(held, mats)):
+                for d, (x, y) in enumerate(zip(old, new)):
+                    if x is not y:
+                        plan.update_term(t, d, y)
+            cache[key] = (plan, mats, conn)
+            return plan
     idx = np.asarray(A_data["element_local_index_D"])
     rp = cols = None
     if opts.use_connectivity:
-        rp, cols = pde.connectivity
+        rp, cols = conn
     plan = Plan(mats, idx, deg, rp, cols, index_base=1, fp32=True, adjoint=True) if mats else None
-    cache[key] = plan
+    cache[key] = (plan, mats, conn)
     return plan
 
 
@@ -78,18 +102,22 @@ def apply_A(pde, opts, A_data, f, deg, Vmax=None, Emax=None, imex_flag="N"):
 
 
 def time_advance(pde, opts, A_data, f, t, dt, deg, hash_table=None, Vmax=None, Emax=None):
-    """time_advance.m:1-34 dispatch for 'RK3' (default) and 'BE'."""
+    """time_advance.m:1-34 dispatch for 'RK3' (default) and 'BE', with the
+    PDE's sources and inhomogeneous Dirichlet terms at each stage's time."""
     import torch
 
     host = isinstance(f, np.ndarray)
     x = torch.from_numpy(np.ascontiguousarray(f, dtype=np.float64)).cuda() if host else f
-    src = pde.__dict__.get("_kmx_source")
-    if src is None and pde.sources:
-        src = TS.SeparableSource(pde, deg, np.asarray(A_data["element_local_index_D"]))
-        pde.__dict__["_kmx_source"] = src
+    # sources and inhomogeneous boundary terms (source_vector.m + boundary_condition_vector.m)
+    srcs = pde.__dict__.setdefault("_kmx_sources", {})
+    skey = (_table_key(A_data), deg)
+    if skey not in srcs:
+        s0 = TS.SeparableSource(pde, deg, np.asarray(A_data["element_local_index_D"]))
+        srcs[skey] = s0 if len(s0) else None
+    src = srcs[skey]
     if opts.timestep_method == "BE":
         plan = _plan_for(pde, opts, A_data, deg, "N")
-        rhs = src(t + dt) if src is not None else None  # backward_euler: s(t+dt), time_advance.m:309-327
+        rhs = src(t + dt) if src is not None else None  # backward_euler: s(t+dt) + bc(t+dt), time_advance.m:270-271
         y, _, _ = TS.backward_euler(plan, x, dt, rhs, tol=opts.extra.get("be_tol", 1e-12))
     else:
         y = TS.rk3_step(lambda v: apply_A(pde, opts, A_data, v, deg), x, t, dt, src)

@@ -104,6 +104,13 @@ class BatchedSystem:
         self.batch.apply(self.views(x), self.views(y), prec=prec)
         return y
 
+    def rk3(self, nsteps=1, prec="fp32", f=None):
+        """nsteps fused RK3 steps of every instance (kmx_batch_rk3) on the
+        concatenated state f (default: the system's own), in place."""
+        f = self.f if f is None else f
+        self.batch.rk3(self.views(f), [inst["dt"] for inst in self.instances], nsteps, prec=prec)
+        return f
+
 
 class BatchRK3Graph:
     """One RK3 step (time_advance.m:151-204, no sources) of every instance,
@@ -161,3 +168,30 @@ class BatchRK3Graph:
         y = torch.empty_like(f)
         self._stages(f, k1, k2, k3, y)
         return f.clone().addcmul_(self.b1, k1).addcmul_(self.b2, k2).addcmul_(self.b3, k3)
+
+
+class BatchRK3Fused:
+    """steps_per_replay fused RK3 steps of every instance (kmx_batch_rk3: per
+    stage one merged-sweep launch and one stage launch) captured in a HIP graph
+    and replayed on the system's state.  Needs every instance on the merged
+    separable sweep for prec."""
+
+    def __init__(self, system, prec="fp32", steps_per_replay=1):
+        import torch
+
+        self.s, self.prec, self.n = system, prec, int(steps_per_replay)
+        scratch = system.f.clone()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            system.rk3(1, prec, f=scratch)  # allocates the stage workspace outside the capture
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            system.rk3(self.n, prec)
+
+    def step(self):
+        self.graph.replay()
+        return self.s.f
+

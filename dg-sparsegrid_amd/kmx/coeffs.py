@@ -377,6 +377,106 @@ def forward_wavelet_transform(deg, lev, lmin, lmax, foo, dV, params, t):
     return f * (np.abs(f) > 1e-12)
 
 
+def _at(fn, x, params, t):
+    """fn(x, params, t) at one point, as a float."""
+    return float(np.ravel(np.asarray(fn(np.array([x], dtype=np.float64), params, t), dtype=np.float64))[0])
+
+
+def compute_boundary_condition(g, dV, t, lev, deg, xmin, xmax, bc_func, side, params=None):
+    """compute_boundary_condition.m:1-55: the trace term (g bc v)|_boundary of one
+    side, deg*2^lev values, nonzero in the boundary cell only: -g bc dV p_L at
+    xMin ('L'), +g bc dV p_R at xMax ('R'), with g and dV taken h*1e-7 inside
+    the domain when g is not finite there (the reference's 'R' branch reads an
+    undefined name in that case; here it uses the same trace value as the finite
+    branch)."""
+    n = 2 ** lev
+    h = (xmax - xmin) / n
+    bc = np.zeros(deg * n)
+    if side == "L":
+        iv = lin_legendre(-1.0, deg)[0] / np.sqrt(h) * _at(bc_func, xmin, params, t)
+        x = xmin if np.isfinite(_at(g, xmin, params, t)) else xmin + h * 1e-7
+        bc[:deg] = -_at(g, x, params, t) * iv * _at(dV, x, params, t)
+    else:
+        iv = lin_legendre(1.0, deg)[0] / np.sqrt(h) * _at(bc_func, xmax, params, t)
+        x = xmax if np.isfinite(_at(g, xmax, params, t)) else xmax - h * 1e-7
+        bc[deg * (n - 1):] = _at(g, x, params, t) * iv * _at(dV, x, params, t)
+    return bc
+
+
+def boundary_condition_terms(pde, deg, idx1d, t=0.0, max_lev=None):
+    """boundary_condition_vector.m:1-197 as a list of (time factor, vector): one
+    pair per DIV / GRAD pterm side with Dirichlet data, the vector being that
+    side's terms combined over the elements with time factor 1, its spatial parts
+    evaluated at t.  Per side: in its own dimension the trace term
+    (compute_boundary_condition), rotated to wavelet space, moved to the RHS with
+    the pterm's LHS mass matrix and multiplied by the matrices of the pterms
+    before it in the chain; in every other dimension the projection of the side's
+    boundary function times that dimension's p-th pterm g, through that pterm's
+    mass solve and the same dimension's preceding pterms.  The 1D matrices are
+    those get_coeff_mats builds (at max_lev when given, cut to each dimension's
+    level).  A side without boundary functions (the DIV / GRAD default: zero
+    functions) has no term."""
+    nd = pde.ndim
+    params = pde.params
+    idx1d = np.asarray(idx1d)
+    out = []
+
+    def pmats(d, pterm):
+        dim = pde.dims[d]
+        lev = dim.lev if max_lev is None else max_lev
+        return pterm_coeff_matrix(deg, 0.0, dim, pterm, params, lev)  # (mat, unrotated, LHS mass)
+
+    for tm in pde.terms:
+        for d1, dim in enumerate(pde.dims):
+            chain = tm[d1]
+            dof = deg * 2 ** dim.lev
+            for p, pterm in enumerate(chain):
+                if pterm.type not in ("grad", "div"):
+                    continue
+                for side, kind, flist in (("L", pterm.BCL, pterm.BCL_fList), ("R", pterm.BCR, pterm.BCR_fList)):
+                    if kind != "D" or flist is None:
+                        continue
+                    parts = []
+                    for d2 in range(nd):
+                        if d2 == d1:
+                            tmp = compute_boundary_condition(pterm.g, pterm.dV, t, dim.lev, deg, dim.min, dim.max,
+                                                             flist[d1], side, params)
+                            tmp = fmwt(deg, dim.lev) @ tmp
+                            M = pmats(d1, pterm)[2][:dof, :dof]
+                            tmp = np.linalg.solve(M, tmp)
+                            pre = np.eye(dof)
+                            for q in range(p):
+                                pre = pre @ pmats(d1, chain[q])[0][:dof, :dof]
+                            parts.append(pre @ tmp)
+                            continue
+                        dim2 = pde.dims[d2]
+                        other = tm[d2]
+                        mp = other[p]
+                        fn = flist[d2]
+                        v = forward_wavelet_transform(
+                            deg, dim2.lev, dim2.min, dim2.max,
+                            lambda x, pp, tt, fn=fn, mp=mp: np.asarray(fn(x, pp, tt)) * np.asarray(mp.g(x, pp, tt)),
+                            lambda x, mp=mp: mp.dV(x, params, t), params, t)
+                        N = v.size
+                        v = np.linalg.solve(pmats(d2, mp)[2][:N, :N], v)
+                        for q in range(p):
+                            v = pmats(d2, other[q])[0][:N, :N] @ v
+                        parts.append(v)
+                    out.append((lambda tt, fn=flist[nd]: float(fn(tt, params)),
+                                combine_dimensions_D(deg, parts, 1.0, idx1d)))
+    return out
+
+
+def boundary_condition_vector(pde, deg, idx1d, t, max_lev=None):
+    """boundary_condition_vector.m:1-197: the inhomogeneous Dirichlet terms of
+    every DIV / GRAD pterm with a 'D' side at time t, summed over terms,
+    dimensions, pterms and sides (each side's vector times its time factor)."""
+    out = np.zeros(np.asarray(idx1d).shape[0] * deg ** pde.ndim)
+    for fac, vec in boundary_condition_terms(pde, deg, idx1d, t, max_lev):
+        out += vec * fac(t)
+    return out
+
+
 def combine_dimensions_D(deg, fList, time_multiplier, idx1d):
     """combine_dimensions_D.m:1-57: per element, krond of the per-dim blocks."""
     N, nd = idx1d.shape
@@ -464,15 +564,17 @@ def continuity3(lev=3):
 
 def diffusion_nd(nd=2, lev=3):
     """pdes/diffusion2.m generalised to nd dims: per dim the LDG chain
-    DIV(g=1, LF=+1, N/N) * GRAD(g=1, LF=-1, D/D), MASS*MASS in the other dims,
-    on [0,1]^nd; solution prod cos(pi x_i) exp(-nd pi^2 t)."""
+    DIV(g=1, LF=+1, N/N) * GRAD(g=1, LF=-1, D/D) with the solution's Dirichlet
+    data, MASS*MASS in the other dims, on [0,1]^nd; solution prod cos(pi x_i)
+    exp(-nd pi^2 t)."""
     levs = np.broadcast_to(np.asarray(lev), (nd,))
     dims = [Dimension(0, 1, int(levs[d]), name=f"x{d + 1}") for d in range(nd)]
+    cosf = lambda x, p=None, t=None: np.cos(np.pi * np.asarray(x, dtype=np.float64))  # noqa: E731
+    flist = [cosf] * nd + [lambda t, p=None: np.exp(-nd * np.pi ** 2 * t)]  # the solution's boundary values
     terms = []
     for k in range(nd):
-        terms.append([[DIV(_const(1), +1, "N", "N"), GRAD(_const(1), -1, "D", "D")] if d == k
-                      else [MASS(_const(1)), MASS(_const(1))] for d in range(nd)])
-    cosf = lambda x, p, t: np.cos(np.pi * x)  # noqa: E731
+        terms.append([[DIV(_const(1), +1, "N", "N"), GRAD(_const(1), -1, "D", "D", BCL_fList=flist, BCR_fList=flist)]
+                      if d == k else [MASS(_const(1)), MASS(_const(1))] for d in range(nd)])
     sol = [[cosf] * nd + [lambda t: np.exp(-nd * np.pi ** 2 * t)]]
     set_dt = lambda pde, cfl: cfl * (1.0 / 2 ** pde.dims[0].lev) ** 2  # noqa: E731
     return PDE(dims, terms, [], sol, sol, {"parameter1": 0, "parameter2": 1}, set_dt, f"diffusion{nd}")
@@ -507,6 +609,54 @@ def advection_diffusion_varcoeff(nd=3, lev=3, nu=0.02):
     ic = [[bump] * nd + [lambda t: 1.0]]
     set_dt = lambda pde, cfl: (pde.dims[0].max - pde.dims[0].min) / 2 ** pde.dims[0].lev * cfl  # noqa: E731
     return PDE(dims, terms, [], ic, [], {}, set_dt, f"advdiff{nd}_varcoeff")
+
+
+def diffusion1(lev=3, nu=np.pi / 2, lf_div=-1.0, lf_grad=+1.0):
+    """pdes/diffusion1.m: df/dt = d2f/dx2 + s on [0, 1] by LDG, DIV(g=1, LF=-1,
+    N/N) GRAD(g=1, LF=+1, D/D) with inhomogeneous Dirichlet data cos(nu x)
+    exp(-2 nu^2 t), nu = pi/2, plus the two zero-coefficient penalty terms;
+    source -nu^2 cos(nu x) exp(-2 nu^2 t).  The reference's gold boundary-vector
+    and RK3 fixtures for diffusion1 were written by a revision with nu = 0.01 and
+    diffusion2's fluxes, DIV LF = +1, GRAD LF = -1 (`diffusion1_gold`)."""
+    soln_x = lambda x, p=None, t=None: np.cos(nu * np.asarray(x, dtype=np.float64))  # noqa: E731
+    soln_t = lambda t, p=None: np.exp(-2 * nu ** 2 * t)  # noqa: E731
+    flist = [soln_x, soln_t]
+    dims = [Dimension(0.0, 1.0, int(np.ravel(lev)[0]), name="x")]
+    terms = [
+        [[DIV(_const(1.0), lf_div, "N", "N"), GRAD(_const(1.0), lf_grad, "D", "D", BCL_fList=flist, BCR_fList=flist)]],
+        [[DIV(_const(0.0), -1, "D", "D")]],
+        [[DIV(_const(-0.0), 0, "D", "D")]],
+    ]
+    sources = [[lambda x, p, t: -nu ** 2 * np.cos(nu * np.asarray(x)), lambda t: np.exp(-2 * nu ** 2 * t)]]
+    sol = [[soln_x, lambda t: np.exp(-2 * nu ** 2 * t)]]
+    set_dt = lambda pde, cfl: cfl * (1.0 / 2 ** pde.dims[0].lev) ** 2  # noqa: E731
+    return PDE(dims, terms, sources, sol, sol, {"parameter1": 0, "parameter2": 1}, set_dt, "diffusion1")
+
+
+def diffusion1_gold(lev=3):
+    """diffusion1 as the reference's gold fixtures were written (nu = 0.01, DIV
+    LF = +1, GRAD LF = -1): reproduces gold boundary_conditions/vector_diffusion1_*
+    and time_advance/diffusion1_sg_* (operators built at the dimension level)."""
+    return diffusion1(lev, nu=0.01, lf_div=+1.0, lf_grad=-1.0)
+
+
+def diffusion2(lev=3):
+    """pdes/diffusion2.m: df/dt = d2f/dx2 + d2f/dy2 on [0, 1]^2 by LDG, per
+    dimension DIV(g=1, LF=+1, N/N) GRAD(g=1, LF=-1, D/D) with inhomogeneous
+    Dirichlet data cos(pi x) cos(pi y) exp(-2 pi^2 t), MASS MASS in the other
+    dimension; no source."""
+    levs = np.broadcast_to(np.asarray(lev), (2,))
+    cx = lambda x, p=None, t=None: np.cos(np.pi * np.asarray(x, dtype=np.float64))  # noqa: E731
+    st = lambda t, p=None: np.exp(-2 * np.pi ** 2 * t)  # noqa: E731
+    flist = [cx, cx, st]
+    dims = [Dimension(0.0, 1.0, int(levs[0]), name="x"), Dimension(0.0, 1.0, int(levs[1]), name="y")]
+    one = _const(1.0)
+    ldg = [DIV(one, +1, "N", "N"), GRAD(one, -1, "D", "D", BCL_fList=flist, BCR_fList=flist)]
+    mm = [MASS(one), MASS(one)]
+    terms = [[ldg, mm], [mm, ldg]]
+    sol = [[cx, cx, lambda t: np.exp(-2 * np.pi ** 2 * t)]]
+    set_dt = lambda pde, cfl: cfl * (1.0 / 2 ** pde.dims[0].lev) ** 2  # noqa: E731
+    return PDE(dims, terms, [], sol, sol, {"parameter1": 0, "parameter2": 1}, set_dt, "diffusion2")
 
 
 PDES = {"continuity1": continuity1, "continuity2": continuity2, "continuity3": continuity3}

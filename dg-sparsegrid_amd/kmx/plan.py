@@ -245,6 +245,14 @@ class Batch:
         check(lib().kmx_batch_apply(self._h, px, py, _prec(prec), _stream(stream)))
         return ys
 
+    def rk3(self, fs, dts, nsteps=1, prec="fp32", stream=None):
+        """nsteps fused RK3 steps of every plan (kmx_batch_rk3): fs advanced in place."""
+        n = len(self.plans)
+        pf = (C.c_void_p * n)(*[_ptr(f).value for f in fs])
+        pd = (C.c_double * n)(*[float(d) for d in dts])
+        check(lib().kmx_batch_rk3(self._h, pf, pd, int(nsteps), _prec(prec), _stream(stream)))
+        return fs
+
     def destroy(self):
         if getattr(self, "_h", None) is not None:
             lib().kmx_batch_destroy(self._h)

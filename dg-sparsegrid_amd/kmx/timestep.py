@@ -27,9 +27,15 @@ def _torch():
 
 
 class SeparableSource:
-    """sum_s g_s(t) * S_s with S_s = combine_dimensions_D(forward transforms)."""
+    """sum_s g_s(t) * S_s: the md_eval_function.m sources, S_s =
+    combine_dimensions_D(forward transforms), and (boundary=True) the
+    inhomogeneous Dirichlet terms of boundary_condition_vector.m, one pair
+    (time factor, vector) per pterm side with Dirichlet data.  The boundary
+    terms' spatial parts are taken at t = 0 and checked at a later time against
+    the vector the reference would build there (time-separable boundary data,
+    as every PDE here has); other data raise ValueError."""
 
-    def __init__(self, pde, deg, idx1d, funcs=None, device="cuda"):
+    def __init__(self, pde, deg, idx1d, funcs=None, device="cuda", boundary=True, max_lev=None):
         torch = _torch()
         funcs = pde.sources if funcs is None else funcs
         nd = pde.ndim
@@ -40,6 +46,19 @@ class SeparableSource:
                                                     lambda x, d=d: pde.dims[d].moment_dV(x), pde.params, 0.0)
                   for d in range(nd)]
             spatial.append(_coeffs.combine_dimensions_D(deg, fl, 1.0, np.asarray(idx1d)))
+        self.num_boundary = 0
+        if boundary:
+            terms = _coeffs.boundary_condition_terms(pde, deg, idx1d, 0.0, max_lev)
+            if terms:
+                t1 = 0.37
+                sep = sum(f(t1) * v for f, v in terms)
+                direct = _coeffs.boundary_condition_vector(pde, deg, idx1d, t1, max_lev)
+                if np.linalg.norm(sep - direct) > 1e-13 * max(np.linalg.norm(direct), 1e-300):
+                    raise ValueError("boundary data are not time-separable")
+                for f, v in terms:
+                    self.g.append(lambda t, f=f: f(t))
+                    spatial.append(v)
+                self.num_boundary = len(terms)
         self.S_host = np.stack(spatial) if spatial else np.zeros((0, np.asarray(idx1d).shape[0] * deg ** nd))
         self.S = torch.from_numpy(self.S_host).to(device)
         self.dof = self.S_host.shape[1]

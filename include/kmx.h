@@ -217,6 +217,19 @@ int kmx_batch_destroy(kmx_batch* batch);
 int kmx_batch_set_sweep(kmx_batch* batch, int prec, int on);
 int kmx_batch_get_sweep(const kmx_batch* batch, int prec, int* members, int* active, int* phases);
 
+/* nsteps explicit RK3 steps of every plan of the batch, y' = A y (the Kutta RK3
+ * of time_advance.m:151-204 -- a21 = 1/2, a31 = -1, a32 = 2, b = 1/6, 2/3, 1/6 --
+ * without source or boundary terms), with the stage arithmetic fused into the
+ * merged separable sweeps: per stage one launch of the dimension ops and one that
+ * sums them into the stage derivative and writes the next stage's input (plus one
+ * launch per call that reads the states in).  fs[i]: plan i's DEVICE state, N*deg^d
+ * FP64, advanced in place; dts[i]: its step (host array).  Needs the batch's merged
+ * sweep on for prec (kmx_batch_set_sweep) with every plan a separable member (else
+ * KMX_E_UNSUPPORTED).  The first call allocates the stage workspace (2 x the
+ * batch's values, FP64); later calls allocate nothing.  Asynchronous on `stream`.
+ * No counterpart in the reference (its time steps call apply_A three times). */
+int kmx_batch_rk3(kmx_batch* batch, double* const* fs, const double* dts, int nsteps, int prec, void* stream);
+
 /* Y = kron(A1,...,Ak) X for k = 1..6 (kron_multd.m:30-42, Kronmult/kronmult{k}.m).
  * A[i]: DEVICE pointer, column-major nrow[i] x ncol[i] (rectangular allowed);
  * X: DEVICE, prod(ncol) x nvec column-major; Y: DEVICE, prod(nrow) x nvec.

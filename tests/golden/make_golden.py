@@ -41,6 +41,9 @@ PATTERNS = [
     r"time_advance/continuity2_implicit_l2_d2_t\d$",
     r"time_advance/diffusion2_sg_l2_d2_t\d$",
     r"time_advance/diffusion2_implicit_sg_l3_d3_t\d$",
+    r"time_advance/diffusion1_sg_l(2_d2|3_d3)_t\d$",
+    r"time_advance/diffusion2_sg_l3_d3_t\d$",
+    r"boundary_conditions/(vector|compute)_diffusion1_.*$",
     r"quadrature/lgwt_(roots|weights)_\d+_\w+$",
 ]
 

@@ -208,3 +208,33 @@ def test_identity_dimension_without_self_connections_is_not_separable(kmx_gpu):
     y = plan.apply(torch.from_numpy(x).cuda(), prec="fp64").cpu().numpy()
     assert rel(y, yo) <= TOL64
     plan.destroy()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fused_rk3_matches_applies(kmx_gpu, prec):
+    """kmx_batch_rk3 (stage arithmetic fused into the merged sweeps, graph-replayed)
+    against the same RK3 steps built from batched applies and FP64 vector updates
+    (time_advance.m:151-204): FP64 within 1e-13 of the FP64 pair path, FP32
+    within 1e-6 of it; every replay checked against the eager steps."""
+    import torch
+
+    from kmx import batch as BT
+
+    inst = BT.make_instances(count=16, seed=8, lev2d=(4, 7), lev3d=(4, 6))
+    ref = BT.BatchedSystem(inst)
+    fused = BT.BatchedSystem(inst, sweep=(prec,))
+    assert fused.sweep[prec]["members"] == len(inst)
+    g = BT.BatchRK3Fused(fused, prec=prec, steps_per_replay=2)
+    gref = BT.BatchRK3Graph(ref, prec="fp64")
+    eager = fused.f.clone()
+    tol = 1e-13 if prec == "fp64" else 1e-6
+    for r in range(4):
+        g.step()
+        fused.rk3(2, prec, f=eager)  # the same fused steps, eager
+        for _ in range(2):
+            gref.step()
+        torch.cuda.synchronize()
+        assert torch.equal(fused.f, eager), r
+        e = rel(fused.f.cpu().numpy(), ref.f.cpu().numpy())
+        print(f"fused rk3 {prec} after {2 * (r + 1)} steps: {e:.2e} vs the fp64 pair path", flush=True)
+        assert e <= tol, (r, e)
