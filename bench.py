@@ -403,7 +403,7 @@ def single_instance_leg(plan, tuned, op, xd, yd, steps, warmup):
     operator through the dimension sweep of a level-block plan."""
     import kmx
 
-    runs = {"fp64": "fp64", "fp32": "fp32", "fp32_grouped": "fp32_grouped"}
+    runs = {"fp64": "fp64", "fp32": "fp32"}
     res, err, flops, maxabs = {}, {}, {}, {}
     for name, prec in runs.items():
         res[name] = time_apply(plan, xd, yd, prec, warmup, steps)

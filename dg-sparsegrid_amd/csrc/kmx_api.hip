@@ -57,9 +57,6 @@ void free_plan(kmx_plan* p) {
     dfree(p->d_row_order);
     dfree(p->d_blk64);
     dfree(p->d_blk32);
-    dfree(p->d_gcols);
-    dfree(p->d_grp_ptr);
-    dfree(p->d_grp_row_ptr);
     dfree(p->d_col_ptr);
     dfree(p->d_rows);
     dfree(p->d_col_order);
@@ -229,67 +226,6 @@ int kmx_plan_create(const kmx_plan_desc* desc, kmx_plan** out) {
             p->d_cols = upload(cols.data(), cols.size());
         }
 
-        // prefix groups for the grouped FP32 kernel: per row, pairs sorted by
-        // (J_1..J_{d-1}) then J_d, groups ordered by size (largest first).  Not built
-        // for lists above 2^25 pairs (host sort time and a copy of the list); such
-        // plans run KMX_PREC_FP32_GROUPED as KMX_PREC_FP32.
-        if (want32 && D >= 2 && p->P <= (1LL << 25)) {
-            int chunk = 4;
-            if (const char* e = getenv("KMX_GROUP_CHUNK")) chunk = std::max(1, atoi(e));
-            std::vector<int> gcols;
-            gcols.reserve(size_t(p->P));
-            std::vector<int> gptr;
-            std::vector<long long> grp_row(size_t(N) + 1, 0);
-            std::vector<int> tmp, key_order;
-            for (int i = 0; i < N; ++i) {
-                tmp.clear();
-                for (long long q = rp[i]; q < rp[i + 1]; ++q) tmp.push_back(p->all_pairs ? int(q - rp[i]) : cols[q]);
-                auto lessJ = [&](int a, int b) {
-                    for (int k = 0; k < D; ++k)
-                        if (idx[size_t(a) * D + k] != idx[size_t(b) * D + k])
-                            return idx[size_t(a) * D + k] < idx[size_t(b) * D + k];
-                    return a < b;
-                };
-                std::sort(tmp.begin(), tmp.end(), lessJ);
-                // split into groups with equal prefix
-                std::vector<std::pair<int, int>> groups;  // [start, end) in tmp
-                size_t s = 0;
-                for (size_t q = 1; q <= tmp.size(); ++q) {
-                    bool brk = (q == tmp.size());
-                    if (!brk)
-                        for (int k = 0; k < D - 1; ++k)
-                            if (idx[size_t(tmp[q]) * D + k] != idx[size_t(tmp[s]) * D + k]) {
-                                brk = true;
-                                break;
-                            }
-                    if (brk) {
-                        if (q > s) groups.push_back({int(s), int(q)});
-                        s = q;
-                    }
-                }
-                // chunks of at most `chunk` pairs bound the serial work of one lane
-                std::vector<std::pair<int, int>> chunks;
-                for (auto& g : groups)
-                    for (int q = g.first; q < g.second; q += chunk) chunks.push_back({q, std::min(q + chunk, g.second)});
-                groups.swap(chunks);
-                std::stable_sort(groups.begin(), groups.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
-                    return (a.second - a.first) > (b.second - b.first);
-                });
-                grp_row[i] = (long long)gptr.size();
-                for (auto& g : groups) {
-                    gptr.push_back(int(gcols.size()));
-                    for (int q = g.first; q < g.second; ++q) gcols.push_back(tmp[q]);
-                }
-                gptr.push_back(int(gcols.size()));
-            }
-            grp_row[N] = (long long)gptr.size();
-            if (gcols.size() > size_t(0x7fffffff)) throw ArgError("pair list too long for the grouped layout");
-            p->G = (long long)gptr.size() - N;
-            p->d_gcols = upload(gcols.data(), gcols.size());
-            p->d_grp_ptr = upload(gptr.data(), gptr.size());
-            p->d_grp_row_ptr = upload(grp_row.data(), grp_row.size());
-        }
-
         // adjoint: transposed pair list (CSC) and transposed blocks
         if (desc->flags & KMX_PLAN_ADJOINT) {
             std::vector<long long> cp(size_t(N) + 1, 0);
@@ -436,7 +372,6 @@ int kmx_apply_transpose(kmx_plan* p, const double* x, double* y, int prec, void*
         set_error("kmx_apply_transpose: x and y must not alias");
         return KMX_E_ARG;
     }
-    if (prec == KMX_PREC_FP32_GROUPED) prec = KMX_PREC_FP32;
     return kmx_launch_apply(p, true, x, y, prec, reinterpret_cast<hipStream_t>(stream));
 }
 

@@ -117,18 +117,6 @@ static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* 
                 err = launch_generic_f32(a, D, deg, p->N, smem, s);
             }
         }
-    } else if (prec == KMX_PREC_FP32_GROUPED) {
-        KArgs<float> a;
-        fill_args(a, p, false, x, y);
-        a.cols = p->d_gcols;
-        a.grp_ptr = p->d_grp_ptr;
-        a.grp_row_ptr = p->d_grp_row_ptr;
-        const int nt = (p->N && p->G / p->N >= 384) ? 512 : 256;  // about one chunk per lane
-        LaunchFn f = pick_grouped(D, deg, p->T, nt);
-        if (!f) f = pick_grouped(D, deg, p->T, 256);
-        size_t smem = std::max(size_t(a.strip_elems) * sizeof(float), size_t(nt / 64) * B * sizeof(float));
-        if (!f || smem > lds_limit()) return launch_variant(p, transpose, x, y, KMX_PREC_FP32, v, s);
-        err = f(&a, p->N, smem, s);
     } else {
         set_error("kmx_apply: unknown precision");
         return KMX_E_ARG;
@@ -142,7 +130,6 @@ static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* 
 
 int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, int prec, hipStream_t s) {
     if (p->N == 0) return KMX_OK;
-    if (prec == KMX_PREC_FP32_GROUPED && (transpose || !p->d_gcols)) prec = KMX_PREC_FP32;  // same sum pair by pair
     if (prec == KMX_PREC_BF16) {
         if (transpose || !p->bs || !p->bs->ok || !p->d_blk32) {
             set_error("kmx_apply: BF16 runs on the level-block path only (forward apply of a "
@@ -160,17 +147,17 @@ int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, in
         }
         return kmx_blocks_apply(p, 2, x, y, s);
     }
-    if (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32 && prec != KMX_PREC_FP32_GROUPED) {
+    if (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32) {
         set_error("kmx_apply: unknown precision");
         return KMX_E_ARG;
     }
-    if (prec != KMX_PREC_FP32_GROUPED && p->sw && p->var[prec_slot(prec)].sweep) {
+    if (p->sw && p->var[prec_slot(prec)].sweep) {
         if (!transpose) return kmx_sweep_apply(p, prec_slot(prec), x, y, s);
         if (kmx_sweep_has_transpose(p)) return kmx_sweep_apply_transpose(p, prec_slot(prec), x, y, s);
     }
-    if (!transpose && prec != KMX_PREC_FP32_GROUPED && p->bs && p->bs->ok && p->bs->split[prec_slot(prec)].active)
+    if (!transpose && p->bs && p->bs->ok && p->bs->split[prec_slot(prec)].active)
         return kmx_blocks_apply(p, prec_slot(prec), x, y, s);
-    kmx_variant v = prec == KMX_PREC_FP32_GROUPED ? p->var[1] : p->var[prec_slot(prec)];
+    kmx_variant v = p->var[prec_slot(prec)];
     return launch_variant(p, transpose, x, y, prec, v, s);
 }
 
@@ -632,7 +619,6 @@ extern "C" int kmx_batch_apply(kmx_batch* b, const double* const* xs, double* co
         return KMX_E_ARG;
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    if (prec == KMX_PREC_FP32_GROUPED) prec = KMX_PREC_FP32;
     if (prec == KMX_PREC_FP32 && !b->has32) {
         set_error("kmx_batch_apply: a plan has no FP32 blocks");
         return KMX_E_ARG;

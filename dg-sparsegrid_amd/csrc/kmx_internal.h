@@ -38,9 +38,6 @@ struct KArgs {
     int N;
     int nterms;
     int strip_elems;       // sum over (t,k) of nb_k*deg^2 (LDS strip size in T elements)
-    // grouped variant only
-    const int* grp_ptr;    // per row: [row_ptr..] groups (CSR over pairs sorted by prefix)
-    const long long* grp_row_ptr;  // N+1 offsets into grp_ptr
 };
 
 // x / y device pointers of the instances of a batch, by slot (a kernel argument).
@@ -143,11 +140,6 @@ struct kmx_plan {
     int* d_row_order = nullptr;
     double* d_blk64 = nullptr;
     float* d_blk32 = nullptr;
-    // grouped (prefix) layout of the pair list, FP32 grouped variant
-    int* d_gcols = nullptr;           // pairs re-sorted by (J_1..J_{d-1}, J_d) per row
-    int* d_grp_ptr = nullptr;         // group start offsets (absolute into d_gcols), per row + sentinel
-    long long* d_grp_row_ptr = nullptr;  // N+1 offsets into d_grp_ptr
-    long long G = 0;                  // total groups
 
     // adjoint
     long long* d_col_ptr = nullptr;

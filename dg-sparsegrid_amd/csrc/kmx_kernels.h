@@ -692,7 +692,6 @@ LaunchFn pick_f64(int D, int deg, int nt, int lds);
 LaunchFn pick_f32(int D, int deg, int nt, int lds);
 LaunchFn pick_split_f64(int D, int deg, int nt, bool lds);
 LaunchFn pick_split_f32(int D, int deg, int nt, bool lds);
-LaunchFn pick_grouped(int D, int deg, int T, int nt);
 hipError_t launch_generic_f64(const KArgs<double>& a, int D, int deg, int nrows, size_t smem, hipStream_t s);
 hipError_t launch_generic_f32(const KArgs<float>& a, int D, int deg, int nrows, size_t smem, hipStream_t s);
 hipError_t launch_batch_f64(const KArgs<double>* args, const int2* items, int nitems, const int* shape,

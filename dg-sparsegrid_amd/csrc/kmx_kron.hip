@@ -195,38 +195,32 @@ extern "C" int kmx_kronmult_batch_plan(int k, const int* nrow, const int* ncol, 
     return KMX_OK;
 }
 
-extern "C" int kmx_kronmult(int k, const int* nrow, const int* ncol, const double* const* A, const double* X,
-                            int nvec, double* Y, void* stream) {
-    if (k < 1 || k > 6 || !nrow || !ncol || !A || !X || !Y || nvec < 0) {
-        set_error("kmx_kronmult: bad arguments");
-        return KMX_E_ARG;
-    }
-    if (nvec == 0) return KMX_OK;
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    // extents: ext[i] = ncol_i until dim i is contracted, then nrow_i
-    std::vector<long long> ext(ncol, ncol + k);
+namespace {
+
+// Values of the largest intermediate of the method-1 chain (ext[i] = ncol_i until
+// dim i is contracted, then nrow_i; nvec vectors).
+long long chain_workspace(int k, const int* nrow, const int* ncol, int nvec) {
+    std::vector<long long> e2(ncol, ncol + k);
     long long maxsz = 0;
-    {
-        std::vector<long long> e2 = ext;
-        for (int d = k - 1; d >= 0; --d) {
-            e2[d] = nrow[d];
-            long long sz = nvec;
-            for (int i = 0; i < k; ++i) sz *= e2[i];
-            maxsz = std::max(maxsz, sz);
-        }
+    for (int d = k - 1; d >= 0; --d) {
+        e2[d] = nrow[d];
+        long long sz = nvec;
+        for (int i = 0; i < k; ++i) sz *= e2[i];
+        maxsz = std::max(maxsz, sz);
     }
-    double* buf[2] = {nullptr, nullptr};
-    if (k > 1) {
-        if (hipMalloc(&buf[0], sizeof(double) * maxsz) != hipSuccess ||
-            (k > 2 && hipMalloc(&buf[1], sizeof(double) * maxsz) != hipSuccess)) {
-            hipFree(buf[0]);
-            set_error("kmx_kronmult: workspace allocation failed");
-            return KMX_E_NOMEM;
-        }
-    }
+    return maxsz;
+}
+
+// The mode products of kron(A1..Ak) X, last factor first (method 1), through the
+// ping-pong buffers buf[0] / buf[1] (buf[1] used only for k > 2).  For k > 6 this
+// is also the order of kron_multd.m's recursion (kron_multd.m:64-86: Z = A_k X,
+// then kron(A_1..A_{k-1}) on Z's transpose).
+void launch_chain(int k, const int* nrow, const int* ncol, const double* const* A, const double* X, int nvec,
+                  double* Y, double* const* buf, hipStream_t s) {
+    std::vector<long long> ext(ncol, ncol + k);
     const double* cur = X;
     int ping = 0;
-    for (int d = k - 1; d >= 0; --d) {  // method 1 order: last factor first
+    for (int d = k - 1; d >= 0; --d) {
         ModeArgs m;
         m.in = cur;
         m.out = (d == 0) ? Y : buf[ping];
@@ -237,13 +231,44 @@ extern "C" int kmx_kronmult(int k, const int* nrow, const int* ncol, const doubl
         for (int i = d + 1; i < k; ++i) m.inner *= ext[i];
         m.outer = nvec;
         for (int i = 0; i < d; ++i) m.outer *= ext[i];
-        long long total = m.inner * m.nrow * m.outer;
-        int grid = int(std::min<long long>((total + 255) / 256, 65536));
+        const long long total = m.inner * m.nrow * m.outer;
+        const int grid = int(std::min<long long>((total + 255) / 256, 65536));
         if (grid > 0) hipLaunchKernelGGL(k_mode_product, dim3(grid), dim3(256), 0, s, m);
         ext[d] = nrow[d];
         cur = m.out;
         ping ^= 1;
     }
+}
+
+bool kron_args_ok(int k, const int* nrow, const int* ncol, const double* const* A, const double* X, int nvec,
+                  const double* Y) {
+    if (k < 1 || k > KMX_KRON_MAX_FACTORS || !nrow || !ncol || !A || !X || !Y || nvec < 0) return false;
+    for (int i = 0; i < k; ++i)
+        if (nrow[i] < 1 || ncol[i] < 1 || !A[i]) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" int kmx_kronmult(int k, const int* nrow, const int* ncol, const double* const* A, const double* X,
+                            int nvec, double* Y, void* stream) {
+    if (!kron_args_ok(k, nrow, ncol, A, X, nvec, Y)) {
+        set_error("kmx_kronmult: bad arguments");
+        return KMX_E_ARG;
+    }
+    if (nvec == 0) return KMX_OK;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const long long maxsz = chain_workspace(k, nrow, ncol, nvec);
+    double* buf[2] = {nullptr, nullptr};
+    if (k > 1) {
+        if (hipMalloc(&buf[0], sizeof(double) * maxsz) != hipSuccess ||
+            (k > 2 && hipMalloc(&buf[1], sizeof(double) * maxsz) != hipSuccess)) {
+            hipFree(buf[0]);
+            set_error("kmx_kronmult: workspace allocation failed");
+            return KMX_E_NOMEM;
+        }
+    }
+    launch_chain(k, nrow, ncol, A, X, nvec, Y, buf, s);
     hipError_t e = hipGetLastError();
     if (k > 1) {
         hipStreamSynchronize(s);

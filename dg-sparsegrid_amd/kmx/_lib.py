@@ -18,7 +18,6 @@ KMX_E_UNSUPPORTED = -5
 
 PREC_FP64 = 0
 PREC_FP32 = 1
-PREC_FP32_GROUPED = 2
 PREC_BF16 = 3
 
 PLAN_FP32 = 1
@@ -74,6 +73,21 @@ class SweepInfo(C.Structure):
         ("total_blocks", C.c_int64),
         ("operator_values", C.c_int64),
         ("box_values", C.c_int64),
+    ]
+
+
+PTERM_MASS, PTERM_DIV, PTERM_GRAD = 0, 1, 2
+
+
+class PtermDesc(C.Structure):
+    _fields_ = [
+        ("type", C.c_int),
+        ("LF", C.c_double),
+        ("ibcl", C.c_char),
+        ("ibcr", C.c_char),
+        ("g_quad", C.POINTER(C.c_double)),
+        ("g_face", C.POINTER(C.c_double)),
+        ("lhs_quad", C.POINTER(C.c_double)),
     ]
 
 
@@ -139,6 +153,10 @@ PROTOTYPES = {
                                     C.c_void_p]),
     "kmx_kronmult_batch_plan": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
                                           C.POINTER(GemmDesc), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
+    "kmx_fmwt_matrix": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kmx_quad_points": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int), C.c_void_p]),
+    "kmx_sd_term_matrix": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_int,
+                                     C.POINTER(PtermDesc), C.c_void_p, C.c_void_p]),
     "kmx_last_error": (C.c_char_p, []),
     "kmx_sync": (C.c_int, [C.c_void_p]),
     "kmx_version": (C.c_char_p, []),

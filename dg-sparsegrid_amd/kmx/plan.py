@@ -14,7 +14,6 @@ from ._lib import KmxError, check, lib  # noqa: F401
 PREC = {
     "fp64": _lib.PREC_FP64,
     "fp32": _lib.PREC_FP32,
-    "fp32_grouped": _lib.PREC_FP32_GROUPED,
     "bf16": _lib.PREC_BF16,
 }
 
@@ -312,8 +311,8 @@ class RealSpace:
             pass
 
 
-def kronmult(mats, X, stream=None):
-    """Y = kron(A1..Ak) X on the device (kron_multd.m semantics, k = 1..6).
+def kronmult(mats, X, stream=None, Y=None):
+    """Y = kron(A1..Ak) X on the device (kron_multd.m semantics, k = 1..16).
 
     mats: 2-D float64 CUDA tensors (nrow_i, ncol_i); X: (prod ncol, nvec)
     float64 CUDA tensor (or 1-D for nvec = 1).  Returns Y (prod nrow, nvec).
@@ -327,9 +326,10 @@ def kronmult(mats, X, stream=None):
     X2 = X.reshape(ncx, -1)
     nvec = X2.shape[1]
     xs = X2.t().contiguous()  # column-major storage of X
-    y = torch.empty(int(np.prod(nr)) * nvec, dtype=torch.float64, device=X.device)
-    check(lib().kmx_kronmult(k, (C.c_int * k)(*nr), (C.c_int * k)(*nc), (C.c_void_p * k)(*[m.data_ptr() for m in cm]),
-                             C.c_void_p(xs.data_ptr()), int(nvec), C.c_void_p(y.data_ptr()), _stream(stream)))
+    y = torch.empty(int(np.prod(nr)) * nvec, dtype=torch.float64, device=X.device) if Y is None else Y
+    fn = lib().kmx_kronmult
+    check(fn(k, (C.c_int * k)(*nr), (C.c_int * k)(*nc), (C.c_void_p * k)(*[m.data_ptr() for m in cm]),
+             C.c_void_p(xs.data_ptr()), int(nvec), C.c_void_p(y.data_ptr()), _stream(stream)))
     return y.view(nvec, -1).t()
 
 

@@ -10,8 +10,9 @@
   replayed (inputs: the state and the stage source coefficients).
 * `backward_euler`   -- time_advance.m:267-350 solves (I - dt A) f1 = f0 +
   dt (s + bc) with a direct solve; here matrix-free restarted GMRES in FP64
-  around mixed-precision inner solves, refined until the FP64 residual
-  ||(I - dt A) f1 - b|| / ||b|| reaches the requested tolerance.
+  (`DeviceGMRES`: graph-captured Arnoldi steps) around mixed-precision inner
+  solves, refined until the FP64 residual ||(I - dt A) f1 - b|| / ||b||
+  reaches the requested tolerance.
 """
 import math
 
@@ -178,57 +179,126 @@ class RK3Graph:
 
 
 # ------------------------------------------------------------ implicit (BE)
-def gmres(matvec, b, x0=None, tol=1e-6, restart=40, maxiter=400, dot=None):
-    """Restarted GMRES (modified Gram-Schmidt, Givens rotations), FP64 on the
-    device.  Returns (x, relative residual, iterations)."""
-    torch = _torch()
-    n = b.numel()
-    x = torch.zeros_like(b) if x0 is None else x0.clone()
-    bnorm = float(torch.linalg.vector_norm(b))
-    if bnorm == 0.0:
-        return x, 0.0, 0
-    it = 0
-    rel = 1.0
-    while it < maxiter:
-        r = b - matvec(x)
-        beta = float(torch.linalg.vector_norm(r))
-        rel = beta / bnorm
-        if rel <= tol:
-            break
-        m = min(restart, maxiter - it)
-        V = torch.empty((m + 1, n), dtype=b.dtype, device=b.device)
-        H = np.zeros((m + 1, m))
-        cs, sn = np.zeros(m), np.zeros(m)
-        g = np.zeros(m + 1)
-        g[0] = beta
-        V[0] = r / beta
-        k_used = 0
-        for k in range(m):
-            w = matvec(V[k])
-            for j in range(k + 1):
-                H[j, k] = float(torch.dot(w, V[j]))
-                w = w - H[j, k] * V[j]
-            H[k + 1, k] = float(torch.linalg.vector_norm(w))
-            if H[k + 1, k] > 0:
-                V[k + 1] = w / H[k + 1, k]
-            for j in range(k):
-                tmp = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
-                H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
-                H[j, k] = tmp
-            den = math.hypot(H[k, k], H[k + 1, k])
-            cs[k], sn[k] = (1.0, 0.0) if den == 0 else (H[k, k] / den, H[k + 1, k] / den)
-            H[k, k] = cs[k] * H[k, k] + sn[k] * H[k + 1, k]
-            H[k + 1, k] = 0.0
-            g[k + 1] = -sn[k] * g[k]
-            g[k] = cs[k] * g[k]
-            it += 1
-            k_used = k + 1
-            if abs(g[k + 1]) / bnorm <= tol or it >= maxiter:
+class DeviceGMRES:
+    """Restarted GMRES with the Arnoldi step on the device, FP64 vectors.
+
+    apply_into(x, y) writes y = M x (device, capturable).  One Arnoldi step --
+    the product with basis vector k, classical Gram-Schmidt against the whole
+    (restart + 1)-row basis twice (rows past k are zero, so the shapes never
+    change), the norm and the next basis vector -- is captured once in a HIP
+    graph with k held on the device; each replay copies the step's Hessenberg
+    column to pinned host memory, where the Givens rotations and the residual
+    estimate are updated (one small copy and one synchronisation per
+    iteration, no per-dot round trips)."""
+
+    def __init__(self, apply_into, n, restart=40, device="cuda"):
+        torch = _torch()
+        self.apply_into, self.n, self.m = apply_into, int(n), int(restart)
+        m = self.m
+        self.V = torch.zeros((m + 1, self.n), dtype=torch.float64, device=device)
+        self.w = torch.empty(self.n, dtype=torch.float64, device=device)
+        self.xin = torch.empty(self.n, dtype=torch.float64, device=device)
+        self.k = torch.zeros(1, dtype=torch.int64, device=device)
+        self.hdev = torch.zeros(m + 2, dtype=torch.float64, device=device)
+        self.hhost = torch.zeros(m + 2, dtype=torch.float64).pin_memory()
+        self.graph = None
+
+    def _step(self):
+        torch = _torch()
+        V, w = self.V, self.w
+        self.xin.copy_(V.index_select(0, self.k).view(-1))
+        self.apply_into(self.xin, w)
+        h = torch.mv(V, w)
+        w.sub_(torch.mv(V.t(), h))
+        h2 = torch.mv(V, w)
+        w.sub_(torch.mv(V.t(), h2))
+        h.add_(h2)
+        nrm = torch.sqrt(torch.mv(w.view(1, -1), w))  # a one-row product: no reduction workspace
+        V.index_copy_(0, self.k + 1, (w / nrm).view(1, -1))
+        self.hdev[: self.m + 1].copy_(h)
+        self.hdev[self.m + 1:].copy_(nrm)
+        self.hhost.copy_(self.hdev, non_blocking=True)
+        self.k.add_(1)
+
+    def _capture(self):
+        torch = _torch()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        saved = self.V.clone(), self.k.clone()
+        with torch.cuda.stream(side):
+            self._step()  # warm the kernels outside the capture
+        torch.cuda.current_stream().wait_stream(side)
+        self.V.copy_(saved[0])
+        self.k.copy_(saved[1])
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._step()
+        torch.cuda.synchronize()
+
+    def solve(self, b, tol=1e-6, maxiter=400):
+        """x with ||b - M x|| <= tol ||b|| (estimate), from x = 0.  Returns
+        (x, relative residual estimate, iterations)."""
+        torch = _torch()
+        m = self.m
+        x = torch.zeros_like(b)
+        bnorm = float(torch.linalg.vector_norm(b))
+        if bnorm == 0.0:
+            return x, 0.0, 0
+        if self.graph is None:
+            self._capture()
+        it, rel, r = 0, 1.0, b.clone()
+        while it < maxiter:
+            beta = float(torch.linalg.vector_norm(r))
+            rel = beta / bnorm
+            if rel <= tol:
                 break
-        yk = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used]) if k_used else np.zeros(0)
-        x = x + torch.from_numpy(yk).to(b.device) @ V[:k_used]
-    r = b - matvec(x)
-    return x, float(torch.linalg.vector_norm(r)) / bnorm, it
+            self.V.zero_()
+            self.V[0].copy_(r / beta)
+            self.k.zero_()
+            H = np.zeros((m + 1, m))
+            cs, sn = np.zeros(m), np.zeros(m)
+            g = np.zeros(m + 1)
+            g[0] = beta
+            k_used = 0
+            for k in range(min(m, maxiter - it)):
+                self.graph.replay()
+                torch.cuda.current_stream().synchronize()
+                hv = self.hhost.numpy()
+                H[: k + 1, k] = hv[: k + 1]
+                H[k + 1, k] = hv[m + 1]
+                for j in range(k):
+                    tmp = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
+                    H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
+                    H[j, k] = tmp
+                den = math.hypot(H[k, k], H[k + 1, k])
+                cs[k], sn[k] = (1.0, 0.0) if den == 0 else (H[k, k] / den, H[k + 1, k] / den)
+                H[k, k] = cs[k] * H[k, k] + sn[k] * H[k + 1, k]
+                H[k + 1, k] = 0.0
+                g[k + 1] = -sn[k] * g[k]
+                g[k] = cs[k] * g[k]
+                it += 1
+                k_used = k + 1
+                if abs(g[k + 1]) / bnorm <= tol or hv[m + 1] == 0.0:
+                    break
+            yk = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used])
+            x += torch.from_numpy(yk).to(b.device) @ self.V[:k_used]
+            self.apply_into(x, self.w)
+            r = b - self.w
+        return x, rel, it
+
+
+def gmres(matvec, b, x0=None, tol=1e-6, restart=40, maxiter=400):
+    """Restarted GMRES for y = matvec(v) (allocating form) on the device: x0 is
+    added to a DeviceGMRES solution of the residual system.  Returns (x, relative
+    residual, iterations)."""
+    torch = _torch()
+    x0 = torch.zeros_like(b) if x0 is None else x0
+    r0 = b - matvec(x0)
+    solver = DeviceGMRES(lambda x, y: y.copy_(matvec(x)), b.numel(), restart, b.device)
+    d, _, it = solver.solve(r0, tol=tol * float(torch.linalg.vector_norm(b)) / max(
+        float(torch.linalg.vector_norm(r0)), 1e-300), maxiter=maxiter)
+    x = x0 + d
+    return x, float(torch.linalg.vector_norm(b - matvec(x))) / max(float(torch.linalg.vector_norm(b)), 1e-300), it
 
 
 def backward_euler(plan, f0, dt, rhs=None, tol=1e-10, inner_prec="fp32", inner_tol=1e-5, restart=40,
@@ -242,21 +312,27 @@ def backward_euler(plan, f0, dt, rhs=None, tol=1e-10, inner_prec="fp32", inner_t
     b = f0.clone() if rhs is None else f0 + dt * rhs
     bnorm = float(_torch().linalg.vector_norm(b))
 
+    torch = _torch()
+
     def op64(v):
         return v - dt * plan.apply(v, prec="fp64")
 
-    def op_in(v):
-        return v - dt * plan.apply(v.contiguous(), prec=inner_prec)
+    tmp = torch.empty_like(f0)
 
+    def op_in_into(x, y):  # y = (I - dt A) x with inner_prec applies, capturable
+        plan.apply(x, tmp, prec=inner_prec)
+        torch.add(x, tmp, alpha=-dt, out=y)
+
+    solver = DeviceGMRES(op_in_into, f0.numel(), restart, f0.device)
     f = f0.clone()  # f0 is a good first guess for small dt
     hist, inner = [], 0
     for _ in range(max_refine):
         r = b - op64(f)
-        rel = float(_torch().linalg.vector_norm(r)) / bnorm if bnorm else 0.0
+        rel = float(torch.linalg.vector_norm(r)) / bnorm if bnorm else 0.0
         hist.append(rel)
         if rel <= tol:
             break
-        d, _, its = gmres(op_in, r, tol=inner_tol, restart=restart, maxiter=max_inner)
+        d, _, its = solver.solve(r, tol=inner_tol, maxiter=max_inner)
         inner += its
         f = f + d
     return f, hist, inner

@@ -40,8 +40,6 @@ extern "C" {
 /* ---- precision of the contractions (x and y are always FP64 in memory) */
 #define KMX_PREC_FP64 0       /* plain FP64, the reference's arithmetic (parity <= 1e-13 rel.) */
 #define KMX_PREC_FP32 1       /* FP32 operands and accumulation, same pair-by-pair algorithm */
-#define KMX_PREC_FP32_GROUPED 2 /* FP32, pairs sharing all but the last 1D index are summed
-                                   before the outer contractions (same sum, fewer flops) */
 #define KMX_PREC_BF16 3       /* BF16 operands on the matrix cores, FP32 accumulation: the
                                  level-block mode products of a KMX_PLAN_LEVEL_BLOCKS |
                                  KMX_PLAN_FP32 plan whose whole pair list factorizes (else
@@ -230,10 +228,12 @@ int kmx_batch_get_sweep(const kmx_batch* batch, int prec, int* members, int* act
  * No counterpart in the reference (its time steps call apply_A three times). */
 int kmx_batch_rk3(kmx_batch* batch, double* const* fs, const double* dts, int nsteps, int prec, void* stream);
 
-/* Y = kron(A1,...,Ak) X for k = 1..6 (kron_multd.m:30-42, Kronmult/kronmult{k}.m).
+/* Y = kron(A1,...,Ak) X for k = 1..KMX_KRON_MAX_FACTORS (kron_multd.m:30-42,
+ * Kronmult/kronmult{k}.m; k > 6 as kron_multd.m:43-86's recursion).
  * A[i]: DEVICE pointer, column-major nrow[i] x ncol[i] (rectangular allowed);
  * X: DEVICE, prod(ncol) x nvec column-major; Y: DEVICE, prod(nrow) x nvec.
  * nrow/ncol/A are host arrays of length k. */
+#define KMX_KRON_MAX_FACTORS 16
 int kmx_kronmult(int k, const int* nrow, const int* ncol, const double* const* A, const double* X, int nvec,
                  double* Y, void* stream);
 
@@ -285,6 +285,41 @@ int kmx_realspace_create(int ndim, int deg, int num_elem, const int* elem_idx1d,
                          const int* ncols, const double* const* meval, kmx_realspace** out);
 int kmx_realspace_apply(kmx_realspace* r, const double* f_w, double* f_r, void* stream);
 int kmx_realspace_destroy(kmx_realspace* r);
+
+/* ---- 1D operators on the device (get_coeff_mats.m:29-53 for one term and dimension).
+ * The forward multiwavelet transform (OperatorTwoScale_wavelet2.m:1-144) as a dense
+ * column-major deg*2^lev square on the HOST, from the two-scale filters H0, G0
+ * (deg x deg, row-major: two_scale_rel_<deg>.mat). */
+int kmx_fmwt_matrix(int deg, int lev, const double* H0, const double* G0, double* F);
+
+/* The Gauss-Legendre points (lgwt.m, default_quad_number.m: nq = max(deg + 1, 10)
+ * per cell) of the 2^lev cells of [xmin, xmax], cell by cell, where the pterm
+ * functions are to be sampled: *nq always; x (HOST, 2^lev * nq values) if not NULL. */
+int kmx_quad_points(int deg, int lev, double xmin, double xmax, int* nq, double* x);
+
+#define KMX_PTERM_MASS 0
+#define KMX_PTERM_DIV 1
+#define KMX_PTERM_GRAD 2
+/* One pterm (MASS.m / DIV.m / GRAD.m), its functions sampled by the caller.  All
+ * arrays are HOST arrays.  LF and the internal boundary types are as the pterm
+ * object holds them (GRAD.m already swaps D <-> N and negates LF). */
+typedef struct {
+    int type;                /* KMX_PTERM_*                                          */
+    double LF;               /* flux coefficient (DIV / GRAD)                        */
+    char ibcl, ibcr;         /* 'D', 'N' or 'P' (DIV / GRAD)                         */
+    const double* g_quad;    /* [2^lev * nq] g(x) * dV(x) at the quadrature points    */
+    const double* g_face;    /* [2^lev + 1] g(x) * dV(x) at the cell faces (DIV / GRAD) */
+    const double* lhs_quad;  /* [2^lev * nq] LHS_mass_g(x) * moment_dV(x) at the points */
+} kmx_pterm_desc;
+
+/* The wavelet-space matrix of one dimension of one term: the product, in chain
+ * order, of its pterms' M \ L (sd_term_coeff_matrix.m:1-27, pterm_coeff_matrix.m,
+ * coeff_matrix.m:105-323), deg*2^lev square, column-major, into DEVICE memory mat.
+ * Built in the cell basis (block-diagonal M, block-banded L, the chain's product)
+ * and rotated once by the transform (H0, G0 as for kmx_fmwt_matrix).  Allocates
+ * its work space; synchronous on `stream`. */
+int kmx_sd_term_matrix(int deg, int lev, double xmin, double xmax, const double* H0, const double* G0, int npterms,
+                       const kmx_pterm_desc* pterms, double* mat, void* stream);
 
 const char* kmx_last_error(void);
 int kmx_sync(void* stream);

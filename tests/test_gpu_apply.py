@@ -79,8 +79,6 @@ def test_shapes_fp64_fp32(kmx_gpu, nd, lev, deg, T, conn):
     xd = torch.from_numpy(x).cuda()
     y32 = plan.apply(xd, prec="fp32").cpu().numpy()
     assert rel(y32, yo) <= TOL32
-    yg = plan.apply(xd, prec="fp32_grouped").cpu().numpy()
-    assert rel(yg, yo) <= TOL32
 
 
 def test_3d_lev7_full_size_fp64(kmx_gpu):
@@ -97,9 +95,8 @@ def test_3d_lev7_full_size_fp64(kmx_gpu):
     print(f"3D lev 7 fp64 rel err {r64:.3e}")
     assert r64 <= TOL64
     y32 = plan.apply(xd, prec="fp32").cpu().numpy()
-    yg = plan.apply(xd, prec="fp32_grouped").cpu().numpy()
-    print(f"3D lev 7 fp32 rel err {rel(y32, yo):.3e} grouped {rel(yg, yo):.3e}")
-    assert rel(y32, yo) <= TOL32 and rel(yg, yo) <= TOL32
+    print(f"3D lev 7 fp32 rel err {rel(y32, yo):.3e}")
+    assert rel(y32, yo) <= TOL32
     # adjoint identity <A x, w> = <x, A^T w> at full size
     w = np.random.default_rng(5).standard_normal(x.size)
     wd = torch.from_numpy(w).cuda()
@@ -233,13 +230,14 @@ def test_errors(kmx_gpu):
     assert e.value.code == KMX_E_SHAPE
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_kronmult_gpu(kmx_gpu, k):
-    """kron_multd.m / kronmult{k}.m on the device, rectangular factors (Kronmult/test1.m)."""
+    """kron_multd.m / kronmult{k}.m on the device, rectangular factors (Kronmult/test1.m);
+    k = 7, 8 against the oracle's restatement of kron_multd.m's recursion."""
     import torch
 
     rng = np.random.default_rng(20 + k)
-    shapes = [(2, 3), (3, 4), (4, 5), (5, 6), (6, 7), (3, 2)][:k]
+    shapes = [(2, 3), (3, 4), (4, 5), (5, 6), (6, 7), (3, 2), (2, 2), (3, 2)][:k]
     A = [rng.random(s) for s in shapes]
     X = rng.random((int(np.prod([s[1] for s in shapes])), 2))
     Y = kmx_gpu.kronmult([torch.from_numpy(a).cuda() for a in A], torch.from_numpy(X).cuda()).cpu().numpy()
