@@ -5,6 +5,8 @@
 #include "kmx_internal.h"
 
 #include <algorithm>
+#include <cstdio>
+#include <mutex>
 #include <numeric>
 
 namespace kmx {
@@ -277,6 +279,63 @@ extern "C" int kmx_kronmult(int k, const int* nrow, const int* ncol, const doubl
     }
     if (e != hipSuccess) {
         set_error(std::string("kmx_kronmult: ") + hipGetErrorString(e));
+        return KMX_E_HIP;
+    }
+    return KMX_OK;
+}
+
+// ---- kmx_kronmult without allocation: the chain's two intermediates live in one
+// file-scope device array (each GPU holds its own copy for the life of the
+// process).  Its device address is looked up once per device, with that device
+// current, and only that address reaches the kernels.
+static __device__ __align__(256) double scratch[KMX_KRON_SCRATCH_ELEMS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+double* scratch_for_current_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+        g_scratch[dev] = static_cast<double*>(ptr);
+    }
+    return g_scratch[dev];
+}
+}  // namespace
+
+extern "C" int kmx_kronmult_noalloc(int k, const int* nrow, const int* ncol, const double* const* A,
+                                    const double* X, int nvec, double* Y, void* stream) {
+    if (!kron_args_ok(k, nrow, ncol, A, X, nvec, Y)) {
+        set_error("kmx_kronmult_noalloc: bad arguments");
+        return KMX_E_ARG;
+    }
+    if (nvec == 0) return KMX_OK;
+    const long long maxsz = chain_workspace(k, nrow, ncol, nvec);
+    const long long half = KMX_KRON_SCRATCH_ELEMS / 2;
+    if (k > 1 && maxsz > half) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "kmx_kronmult_noalloc: an intermediate of %lld values exceeds the scratch limit of %lld "
+                      "(KMX_KRON_SCRATCH_ELEMS / 2); use kmx_kronmult",
+                      maxsz, half);
+        set_error(msg);
+        return KMX_E_SHAPE;
+    }
+    double* base = scratch_for_current_device();
+    if (!base) {
+        set_error("kmx_kronmult_noalloc: hipGetSymbolAddress of the scratch array failed");
+        return KMX_E_HIP;
+    }
+    double* buf[2] = {base, base + half};
+    launch_chain(k, nrow, ncol, A, X, nvec, Y, buf, reinterpret_cast<hipStream_t>(stream));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("kmx_kronmult_noalloc: ") + hipGetErrorString(e));
         return KMX_E_HIP;
     }
     return KMX_OK;

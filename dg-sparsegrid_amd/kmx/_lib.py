@@ -20,6 +20,8 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16 = 3
 
+KRON_SCRATCH_ELEMS = 1 << 20
+
 PLAN_FP32 = 1
 PLAN_ADJOINT = 2
 PLAN_LEVEL_BLOCKS = 4
@@ -147,6 +149,8 @@ PROTOTYPES = {
                                 C.c_void_p]),
     "kmx_kronmult": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.c_void_p,
                                C.c_int, C.c_void_p, C.c_void_p]),
+    "kmx_kronmult_noalloc": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_void_p),
+                                       C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "kmx_gemm_vbatched": (C.c_int, [C.c_char, C.c_char, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                     C.c_double, C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.POINTER(C.c_void_p),
                                     C.POINTER(C.c_int), C.c_double, C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.c_int,

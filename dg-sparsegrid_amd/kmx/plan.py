@@ -311,12 +311,14 @@ class RealSpace:
             pass
 
 
-def kronmult(mats, X, stream=None, Y=None):
+def kronmult(mats, X, stream=None, noalloc=False, Y=None):
     """Y = kron(A1..Ak) X on the device (kron_multd.m semantics, k = 1..16).
 
     mats: 2-D float64 CUDA tensors (nrow_i, ncol_i); X: (prod ncol, nvec)
     float64 CUDA tensor (or 1-D for nvec = 1).  Returns Y (prod nrow, nvec).
-    Column-major copies are made for the C ABI, which uses MATLAB layout."""
+    Column-major copies are made for the C ABI, which uses MATLAB layout.
+    noalloc: call kmx_kronmult_noalloc (the library allocates nothing; its
+    intermediates use the library's fixed scratch array)."""
     torch = _torch()
     k = len(mats)
     cm = [m.t().contiguous() for m in mats]  # storage = column-major of m
@@ -327,7 +329,7 @@ def kronmult(mats, X, stream=None, Y=None):
     nvec = X2.shape[1]
     xs = X2.t().contiguous()  # column-major storage of X
     y = torch.empty(int(np.prod(nr)) * nvec, dtype=torch.float64, device=X.device) if Y is None else Y
-    fn = lib().kmx_kronmult
+    fn = lib().kmx_kronmult_noalloc if noalloc else lib().kmx_kronmult
     check(fn(k, (C.c_int * k)(*nr), (C.c_int * k)(*nc), (C.c_void_p * k)(*[m.data_ptr() for m in cm]),
              C.c_void_p(xs.data_ptr()), int(nvec), C.c_void_p(y.data_ptr()), _stream(stream)))
     return y.view(nvec, -1).t()

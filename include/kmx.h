@@ -237,6 +237,18 @@ int kmx_batch_rk3(kmx_batch* batch, double* const* fs, const double* dts, int ns
 int kmx_kronmult(int k, const int* nrow, const int* ncol, const double* const* A, const double* X, int nvec,
                  double* Y, void* stream);
 
+/* kmx_kronmult without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): no counterpart in
+ * the reference.  Same arguments and result as kmx_kronmult; the chain's
+ * intermediates live in a library-owned device array of KMX_KRON_SCRATCH_ELEMS
+ * doubles (8 MiB, held per GPU until the process ends), so each intermediate
+ * may hold at most KMX_KRON_SCRATCH_ELEMS / 2 values (else KMX_E_SHAPE).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (concurrent
+ * streams or threads) race.  Asynchronous on `stream`; no synchronisation. */
+#define KMX_KRON_SCRATCH_ELEMS (1 << 20)
+int kmx_kronmult_noalloc(int k, const int* nrow, const int* ncol, const double* const* A, const double* X, int nvec,
+                         double* Y, void* stream);
+
 /* Variable-size batched GEMM (Kronmult/gemm_vbatched.m:1-38, magmablas-style):
  * C_i = alpha*op(A_i)*op(B_i) + beta*C_i with op from transA/transB ('N'/'T'),
  * m_i x k_i times k_i x n_i.  k is taken from k[] (the reference reads nlist at
