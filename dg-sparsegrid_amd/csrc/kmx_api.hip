@@ -258,11 +258,9 @@ int kmx_plan_create(const kmx_plan_desc* desc, kmx_plan** out) {
         }
         if (desc->flags & KMX_PLAN_LEVEL_BLOCKS) {
             std::vector<const double*> mats(desc->mats, desc->mats + size_t(T) * D);
-            kmx_blocks_build(p, idx, std::move(rp), std::move(cols), mats.data());
+            kmx_blocks_build(p, idx, rp, cols, mats.data());
         }
         kmx_plan_apply_cached_variants(p);
-        const int rc = kmx_plan_init_splits(p);
-        if (rc != KMX_OK) throw HipError(kmx_last_error());
         *out = p;
         return KMX_OK;
     } catch (const std::exception& e) {

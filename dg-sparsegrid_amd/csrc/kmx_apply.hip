@@ -24,8 +24,7 @@ static size_t lds_limit() {
 }
 
 template <typename T>
-static void fill_args(KArgs<T>& a, const kmx_plan* p, bool transpose, const double* x, double* y,
-                      const kmx_pairs* pl = nullptr) {
+static void fill_args(KArgs<T>& a, const kmx_plan* p, bool transpose, const double* x, double* y) {
     std::memset(&a, 0, sizeof(a));
     if (std::is_same<T, double>::value)
         a.blk = reinterpret_cast<const T*>(transpose ? p->d_blkT64 : p->d_blk64);
@@ -38,11 +37,6 @@ static void fill_args(KArgs<T>& a, const kmx_plan* p, bool transpose, const doub
     a.row_ptr = transpose ? p->d_col_ptr : p->d_row_ptr;
     a.cols = transpose ? p->d_rows : p->d_cols;
     a.row_order = transpose ? p->d_col_order : p->d_row_order;
-    if (pl) {
-        a.row_ptr = pl->rp;
-        a.cols = pl->cols;
-        a.row_order = pl->order;
-    }
     a.x = x;
     a.y = y;
     a.N = p->N;
@@ -68,7 +62,7 @@ static bool strip_fits(const kmx_plan* p, size_t elem, int mode = 1) {
 
 // One launch of the pair-by-pair kernel family with an explicit variant.
 static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* y, int prec, kmx_variant v,
-                          hipStream_t s, const kmx_pairs* pl = nullptr) {
+                          hipStream_t s) {
     const int D = p->D, deg = p->deg, B = p->B;
     hipError_t err = hipSuccess;
     if (prec == KMX_PREC_FP64 || prec == KMX_PREC_FP32) {
@@ -94,11 +88,11 @@ static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* 
             const size_t smem = std::max(lds ? strip : size_t(0), size_t(nt / 64) * B * elem);
             if (f64) {
                 KArgs<double> a;
-                fill_args(a, p, transpose, x, y, pl);
+                fill_args(a, p, transpose, x, y);
                 err = f(&a, p->N, smem, s);
             } else {
                 KArgs<float> a;
-                fill_args(a, p, transpose, x, y, pl);
+                fill_args(a, p, transpose, x, y);
                 err = f(&a, p->N, smem, s);
             }
         } else {
@@ -109,11 +103,11 @@ static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* 
             }
             if (f64) {
                 KArgs<double> a;
-                fill_args(a, p, transpose, x, y, pl);
+                fill_args(a, p, transpose, x, y);
                 err = launch_generic_f64(a, D, deg, p->N, smem, s);
             } else {
                 KArgs<float> a;
-                fill_args(a, p, transpose, x, y, pl);
+                fill_args(a, p, transpose, x, y);
                 err = launch_generic_f32(a, D, deg, p->N, smem, s);
             }
         }
@@ -130,22 +124,13 @@ static int launch_variant(kmx_plan* p, bool transpose, const double* x, double* 
 
 int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, int prec, hipStream_t s) {
     if (p->N == 0) return KMX_OK;
-    if (prec == KMX_PREC_BF16) {
-        if (transpose || !p->bs || !p->bs->ok || !p->d_blk32) {
-            set_error("kmx_apply: BF16 runs on the level-block path only (forward apply of a "
-                      "KMX_PLAN_LEVEL_BLOCKS | KMX_PLAN_FP32 plan with level-block structure)");
+    if (prec == KMX_PREC_BF16) {  // the separable sweep with BF16 operands, whatever kmx_plan_set_sweep says
+        if (!kmx_sweep_bf16_ok(p) || (transpose && !kmx_sweep_has_transpose(p))) {
+            set_error("kmx_apply: BF16 runs on the separable dimension sweep only (a KMX_PLAN_LEVEL_BLOCKS | "
+                      "KMX_PLAN_FP32 plan whose terms each act in one dimension)");
             return KMX_E_UNSUPPORTED;
         }
-        kmx_split& sp = p->bs->split[2];
-        if (!sp.active) {
-            const int rc = kmx_blocks_split(p, 2, -1.0);
-            if (rc != KMX_OK) return rc;
-            if (!sp.active) {
-                set_error("kmx_apply: no block pair of this plan fits the BF16 level-block kernel");
-                return KMX_E_UNSUPPORTED;
-            }
-        }
-        return kmx_blocks_apply(p, 2, x, y, s);
+        return transpose ? kmx_sweep_apply_transpose(p, 2, x, y, s) : kmx_sweep_apply(p, 2, x, y, s);
     }
     if (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32) {
         set_error("kmx_apply: unknown precision");
@@ -155,16 +140,8 @@ int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, in
         if (!transpose) return kmx_sweep_apply(p, prec_slot(prec), x, y, s);
         if (kmx_sweep_has_transpose(p)) return kmx_sweep_apply_transpose(p, prec_slot(prec), x, y, s);
     }
-    if (!transpose && p->bs && p->bs->ok && p->bs->split[prec_slot(prec)].active)
-        return kmx_blocks_apply(p, prec_slot(prec), x, y, s);
     kmx_variant v = p->var[prec_slot(prec)];
     return launch_variant(p, transpose, x, y, prec, v, s);
-}
-
-// The pair-by-pair kernels over an explicit pair list (the level-block path's
-// remainder), with the plan's tuned variant.
-int kmx_launch_pairs(kmx_plan* p, const kmx_pairs* pl, const double* x, double* y, int prec, hipStream_t s) {
-    return launch_variant(p, false, x, y, prec, p->var[prec_slot(prec)], s, pl);
 }
 
 // ---------------------------------------------------------------- autotuner
@@ -194,33 +171,16 @@ TuneKey key_of(const kmx_plan* p, int prec) {
 }
 }  // namespace
 
-// Level-block split of one precision with threshold beta (-2: library default).
-static int set_split(kmx_plan* p, int prec, double beta) {
-    if (!p->bs || !p->bs->ok) return KMX_OK;
-    if (prec == KMX_PREC_FP32 && !p->d_blk32) return KMX_OK;
-    if (beta == -2.0) beta = kmx_blocks_default_beta();
-    kmx_split& sp = p->bs->split[prec_slot(prec)];
-    if (sp.beta == beta && (sp.active || (std::isinf(beta) && beta > 0))) return KMX_OK;
-    return kmx_blocks_split(p, prec_slot(prec), beta);
-}
-
 void kmx_plan_apply_cached_variants(kmx_plan* p) {
     std::lock_guard<std::mutex> g(tune_mutex());
     for (int prec : {KMX_PREC_FP64, KMX_PREC_FP32}) {
         auto it = tune_cache().find(key_of(p, prec));
         if (it != tune_cache().end()) {
-            p->var[prec_slot(prec)] = it->second;
-            set_split(p, prec, it->second.beta);
+            kmx_variant v = it->second;
+            v.sweep = v.sweep && p->sw && (prec == KMX_PREC_FP32 || kmx_sweep_fp64_ok(p));
+            p->var[prec_slot(prec)] = v;
         }
     }
-}
-
-int kmx_plan_init_splits(kmx_plan* p) {
-    for (int prec : {KMX_PREC_FP64, KMX_PREC_FP32}) {
-        const int rc = set_split(p, prec, p->var[prec_slot(prec)].beta);
-        if (rc != KMX_OK) return rc;
-    }
-    return KMX_OK;
 }
 
 namespace {
@@ -273,16 +233,17 @@ extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
         const bool has_reg = f64 ? pick_f64(p->D, p->deg, 256, 0) : pick_f32(p->D, p->deg, 256, 0);
         const bool has_split = f64 ? pick_split_f64(p->D, p->deg, 256, false) : pick_split_f32(p->D, p->deg, 256, false);
         kmx_variant bestv = p->var[prec_slot(prec)];
+        bestv.sweep = 0;
+        float best = 1e30f;
         // 1. pair-by-pair kernel variant, on the whole pair list
         if (has_reg || has_split) {
-            float best = 1e30f;
             for (int kind : {0, 1})
                 for (int nt : {128, 256, 512})
                     for (int lds : {1, 2, 0}) {
                         if (lds && !strip_fits(p, elem, lds)) continue;
                         if (lds == 2 && (kind == 1 || p->D < 2)) continue;
                         if ((kind == 0 && !has_reg) || (kind == 1 && (!has_split || nt == 128))) continue;
-                        kmx_variant v{nt, lds, kind, 0, bestv.beta};
+                        const kmx_variant v{nt, lds, kind, 0};
                         float ms;
                         rc = time_calls([&] { return launch_variant(p, false, x, y, prec, v, s); }, s, e0, e1, ms);
                         if (rc != KMX_OK) break;
@@ -291,38 +252,19 @@ extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
                             bestv = v;
                         }
                     }
-            if (rc != KMX_OK) break;
-            p->var[prec_slot(prec)] = bestv;
+        } else {
+            rc = time_calls([&] { return launch_variant(p, false, x, y, prec, bestv, s); }, s, e0, e1, best);
         }
-        // 2. level-block split threshold, with that variant for the remainder
-        if (p->bs && p->bs->ok) {
-            float best = 1e30f;
-            double bestb = bestv.beta;
-            for (double beta : {HUGE_VAL, 0.0, 1000.0, 4000.0, 16000.0, 64000.0, 256000.0}) {
-                rc = set_split(p, prec, beta);
-                if (rc != KMX_OK) break;
-                float ms;
-                rc = time_calls([&] { return kmx_launch_apply(p, false, x, y, prec, s); }, s, e0, e1, ms);
-                if (rc != KMX_OK) break;
-                if (ms < best) {
-                    best = ms;
-                    bestb = beta;
-                }
-            }
+        if (rc != KMX_OK) break;
+        p->var[prec_slot(prec)] = bestv;
+        // 2. the dimension-sweep path against it
+        if (p->sw && (prec == KMX_PREC_FP32 || kmx_sweep_fp64_ok(p))) {
+            p->var[prec_slot(prec)].sweep = 1;
+            float ms;
+            rc = time_calls([&] { return kmx_launch_apply(p, false, x, y, prec, s); }, s, e0, e1, ms);
             if (rc != KMX_OK) break;
-            bestv.beta = bestb;
+            bestv.sweep = ms < best ? 1 : 0;
             p->var[prec_slot(prec)] = bestv;
-            rc = set_split(p, prec, bestb);
-            if (rc != KMX_OK) break;
-            // 3. the dimension-sweep path against the best of the above
-            if (p->sw && (prec == KMX_PREC_FP32 || kmx_sweep_fp64_ok(p))) {
-                p->var[prec_slot(prec)].sweep = 1;
-                float ms;
-                rc = time_calls([&] { return kmx_launch_apply(p, false, x, y, prec, s); }, s, e0, e1, ms);
-                if (rc != KMX_OK) break;
-                bestv.sweep = ms < best ? 1 : 0;
-                p->var[prec_slot(prec)] = bestv;
-            }
         }
         std::lock_guard<std::mutex> g(tune_mutex());
         tune_cache()[key_of(p, prec)] = bestv;
@@ -334,23 +276,13 @@ extern "C" int kmx_plan_autotune(kmx_plan* p, void* stream) {
     return rc;
 }
 
-extern "C" int kmx_plan_set_split(kmx_plan* p, int prec, double beta) {
-    if (!p || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32) || std::isnan(beta)) {
-        set_error("kmx_plan_set_split: bad arguments");
-        return KMX_E_ARG;
-    }
-    if (beta < 0) beta = -1.0;
-    p->var[prec_slot(prec)].beta = beta;
-    if (!p->bs || !p->bs->ok) return KMX_OK;
-    if (prec == KMX_PREC_FP32 && !p->d_blk32) {
-        set_error("kmx_plan_set_split: plan has no FP32 blocks");
-        return KMX_E_ARG;
-    }
-    p->bs->split[prec_slot(prec)].beta = std::numeric_limits<double>::quiet_NaN();  // force a rebuild
-    return set_split(p, prec, beta);
-}
-
 extern "C" int kmx_plan_set_sweep(kmx_plan* p, int prec, int on) {
+    if (p && prec == KMX_PREC_BF16) {  // BF16 has no other path: on is all it takes
+        if (on && kmx_sweep_bf16_ok(p)) return KMX_OK;
+        set_error(on ? "kmx_plan_set_sweep: this plan has no separable sweep with BF16 values"
+                     : "kmx_plan_set_sweep: BF16 applies run on the separable sweep only; it cannot be switched off");
+        return KMX_E_UNSUPPORTED;
+    }
     if (!p || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
         set_error("kmx_plan_set_sweep: bad arguments");
         return KMX_E_ARG;
@@ -378,35 +310,20 @@ extern "C" int kmx_plan_sweep_info(const kmx_plan* p, kmx_sweep_info* info) {
 }
 
 extern "C" int kmx_plan_get_sweep(const kmx_plan* p, int prec, int* available, int* active, double* flops) {
-    if (!p || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
+    if (!p || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32 && prec != KMX_PREC_BF16)) {
         set_error("kmx_plan_get_sweep: bad arguments");
         return KMX_E_ARG;
+    }
+    if (prec == KMX_PREC_BF16) {
+        const int ok = kmx_sweep_bf16_ok(p) ? 1 : 0;
+        if (available) *available = ok;
+        if (active) *active = ok;
+        if (flops) *flops = kmx_sweep_flops(p);
+        return KMX_OK;
     }
     if (available) *available = (p->sw && (prec == KMX_PREC_FP32 || kmx_sweep_fp64_ok(p))) ? 1 : 0;
     if (active) *active = (p->sw && p->var[prec_slot(prec)].sweep) ? 1 : 0;
     if (flops) *flops = kmx_sweep_flops(p);
-    return KMX_OK;
-}
-
-extern "C" int kmx_plan_get_split(const kmx_plan* p, int prec, int* structured, double* beta, int64_t* block_pairs,
-                                  int64_t* pairs_in_blocks, int64_t* remaining_pairs, double* flops) {
-    if (!p || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
-        set_error("kmx_plan_get_split: bad arguments");
-        return KMX_E_ARG;
-    }
-    const bool ok = p->bs && p->bs->ok;
-    const kmx_split* sp = ok ? &p->bs->split[prec_slot(prec)] : nullptr;
-    const bool act = sp && sp->active;
-    if (structured) *structured = ok ? 1 : 0;
-    if (beta) *beta = ok ? sp->beta : HUGE_VAL;
-    if (block_pairs) *block_pairs = act ? sp->block_pairs : 0;
-    if (pairs_in_blocks) *pairs_in_blocks = act ? sp->pairs_in_blocks : 0;
-    if (remaining_pairs) *remaining_pairs = act ? sp->sub_pairs : p->P;
-    if (flops) {
-        double chain = 2.0 * p->D;
-        for (int k = 0; k <= p->D; ++k) chain *= p->deg;
-        *flops = double(act ? sp->sub_pairs : p->P) * p->T * chain + (act ? sp->fact_flops : 0.0);
-    }
     return KMX_OK;
 }
 
@@ -619,11 +536,11 @@ extern "C" int kmx_batch_apply(kmx_batch* b, const double* const* xs, double* co
         return KMX_E_ARG;
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    if (prec == KMX_PREC_FP32 && !b->has32) {
+    if ((prec == KMX_PREC_FP32 || prec == KMX_PREC_BF16) && !b->has32) {
         set_error("kmx_batch_apply: a plan has no FP32 blocks");
         return KMX_E_ARG;
     }
-    if (prec != KMX_PREC_FP32 && prec != KMX_PREC_FP64) {
+    if (prec != KMX_PREC_FP32 && prec != KMX_PREC_FP64 && prec != KMX_PREC_BF16) {
         set_error("kmx_batch_apply: unknown precision");
         return KMX_E_ARG;
     }
@@ -636,6 +553,21 @@ extern "C" int kmx_batch_apply(kmx_batch* b, const double* const* xs, double* co
     {
         const int rc = refresh_batch(b);
         if (rc != KMX_OK) return rc;
+    }
+    if (prec == KMX_PREC_BF16) {  // the FP32 merged sweep with BF16 values; every plan must be in it
+        if (!b->sweep_on[1] || b->members[1].size() != b->plans.size()) {
+            set_error("kmx_batch_apply: BF16 runs on the FP32 merged sweep (kmx_batch_set_sweep) with every plan "
+                      "a separable member");
+            return KMX_E_UNSUPPORTED;
+        }
+        BatchIO io;
+        std::memset(&io, 0, sizeof(io));
+        const std::vector<int>& mem = b->members[1];
+        for (size_t q = 0; q < mem.size(); ++q) {
+            io.x[q] = xs[mem[q]];
+            io.y[q] = ys[mem[q]];
+        }
+        return kmx_sweep_batch_apply(b->sweep[1], 2, io, s);
     }
     if (b->sweep_on[slot]) {
         BatchIO io;
@@ -684,11 +616,11 @@ extern "C" int kmx_batch_destroy(kmx_batch* b) {
 }
 
 extern "C" int kmx_batch_set_sweep(kmx_batch* b, int prec, int on) {
-    if (!b || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
+    if (!b || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32 && prec != KMX_PREC_BF16)) {
         set_error("kmx_batch_set_sweep: bad arguments");
         return KMX_E_ARG;
     }
-    const int slot = prec_slot(prec);
+    const int slot = prec_slot(prec);  // BF16: the FP32 merged sweep, whose tables it shares
     if (bool(on) == b->sweep_on[slot]) return KMX_OK;
     if (on) {
         const int rc = build_batch_sweep(b, slot);
@@ -709,12 +641,13 @@ extern "C" int kmx_batch_set_sweep(kmx_batch* b, int prec, int on) {
 }
 
 extern "C" int kmx_batch_rk3(kmx_batch* b, double* const* fs, const double* dts, int nsteps, int prec, void* stream) {
-    if (!b || !fs || !dts || nsteps < 0 || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
+    if (!b || !fs || !dts || nsteps < 0 ||
+        (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32 && prec != KMX_PREC_BF16)) {
         set_error("kmx_batch_rk3: bad arguments");
         return KMX_E_ARG;
     }
     if (nsteps == 0 || b->plans.empty()) return KMX_OK;
-    const int slot = prec_slot(prec);
+    const int slot = prec_slot(prec);  // BF16: the FP32 merged sweep's tables
     {
         const int rc = refresh_batch(b);
         if (rc != KMX_OK) return rc;
@@ -735,16 +668,19 @@ extern "C" int kmx_batch_rk3(kmx_batch* b, double* const* fs, const double* dts,
             return KMX_E_ARG;
         }
     }
-    return kmx_sweep_batch_rk3(b->sweep[slot], slot, f.data(), dt.data(), nsteps, reinterpret_cast<hipStream_t>(stream));
+    return kmx_sweep_batch_rk3(b->sweep[slot], prec == KMX_PREC_BF16 ? 2 : slot, f.data(), dt.data(), nsteps,
+                               reinterpret_cast<hipStream_t>(stream));
 }
 
 extern "C" int kmx_batch_get_sweep(const kmx_batch* b, int prec, int* members, int* active, int* phases) {
-    if (!b || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32)) {
+    if (!b || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32 && prec != KMX_PREC_BF16)) {
         set_error("kmx_batch_get_sweep: bad arguments");
         return KMX_E_ARG;
     }
     const int slot = prec_slot(prec);
-    const bool act = b->sweep_on[slot];
+    const bool act = b->sweep_on[slot] &&
+                     (prec != KMX_PREC_BF16 ||
+                      (b->members[1].size() == b->plans.size() && kmx_sweep_batch_has16(b->sweep[1])));
     if (members) *members = act ? int(b->members[slot].size()) : 0;
     if (active) *active = act ? 1 : 0;
     if (phases) *phases = act ? kmx_sweep_batch_phases(b->sweep[slot]) : 0;

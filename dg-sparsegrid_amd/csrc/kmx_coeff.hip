@@ -10,8 +10,11 @@
 // its neighbours, wrapped for periodic boundaries), so M \ L is one small solve
 // per cell, the chain is a product of block-banded matrices, and the rotation
 // W = F R F^T runs once, through the transform's row supports (each row of F is
-// nonzero on one contiguous range of columns).  Mathematically this is the
-// reference's M \ L in the wavelet basis (F is orthogonal); the roundoff differs.
+// nonzero on one contiguous range of columns).  The reference rotates first and
+// solves, (F M F^T) \ (F L F^T) = F^-T (M \ L) F^T, which equals F (M \ L) F^T for
+// an orthogonal F; the transform built from the two-scale filter data is
+// orthogonal to ||F F^T - I|| (about 2e-14 up to deg 3, 6e-14 at deg 4, 2.3e-13 at
+// deg 5 and 6), and the two results differ by about that much besides roundoff.
 #include "kmx_internal.h"
 
 #include <algorithm>

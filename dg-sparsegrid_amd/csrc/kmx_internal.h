@@ -56,37 +56,13 @@ struct kmx_variant {
     int lds = -1;
     int kind = 0;          // 0: one lane per pair, 1: pair split over deg lanes
     int sweep = 0;         // 1: the dimension-sweep path (when the plan has one)
-    double beta = -2.0;    // level-block split threshold (-2: library default)
-};
-
-// One precision's division of the pair list between the level-block kernel
-// (block pairs listed in items/tasks) and the pair-by-pair kernels (the rest,
-// as a CSR sub-list).  beta is the per-stage overhead, in flops, charged to a
-// factorized block pair when the two costs are compared.
-struct kmx_split {
-    bool active = false;
-    double beta = 0;
-    long long sub_pairs = 0;        // pairs left to the pair-by-pair kernel
-    long long block_pairs = 0;      // level-block pairs run factorized
-    long long pairs_in_blocks = 0;  // element pairs they cover
-    double fact_flops = 0;          // flops of the factorized block pairs per apply (all terms)
-    int ntasks = 0;
-    int maxX = 0, maxI = 0, maxY = 0;  // LDS buffer sizes (elements)
-    long long* d_rp = nullptr;
-    int* d_cols = nullptr;
-    int* d_order = nullptr;
-    int4* d_tasks = nullptr;   // (row block, first item, end item, partial offset)
-    int2* d_items = nullptr;   // (column block, contraction order)
-    int* d_btask = nullptr;    // per row block: first task (nblk + 1)
-    void* d_part = nullptr;    // partial row-block results
-    long long part_elems = 0;
 };
 
 // Level-block structure of a plan: elements grouped by their level
 // multi-index (one dense cell tensor per group), per-dimension 1D connection
 // masks, and the masked 1D matrices.  ok is set only when every connected
 // pair of blocks holds exactly the tensor product of the per-dimension masks,
-// so the factorized sum equals the pair-by-pair sum term for term.
+// so a factorized (dimension-sweep) sum equals the pair-by-pair sum term for term.
 struct kmx_blockset {
     bool ok = false;
     std::string why;
@@ -97,25 +73,8 @@ struct kmx_blockset {
     std::vector<int> blev;    // per block: D levels
     bool full_product = false;  // also every unconnected block pair has an empty mask product
     std::vector<std::vector<unsigned char>> mask;  // per dim: nb x nb, [I * nb + J]
-    struct BP {
-        int bi, bj;
-        long long pairs;
-        double fflops;  // one term, factorized, best order
-        int order;      // 3 bits per stage: dim contracted at stage s
-        int maxI;       // largest intermediate of that order
-        bool fits;
-    };
-    std::vector<BP> bps;
-    std::vector<int> bpid;    // nblk x nblk -> index into bps or -1
-    std::vector<long long> rp;  // host copy of the pair list (0-based)
-    std::vector<int> cols;
-    int* d_binfo = nullptr;
-    int* d_elems = nullptr;
-    int* d_eblk = nullptr;
-    int* d_ecell = nullptr;
     double* d_mat64 = nullptr;  // masked 1D matrices, column-major, offsets off[t*D+k]
     float* d_mat32 = nullptr;
-    kmx_split split[3];  // FP64, FP32, BF16 (every block pair factorized)
 };
 
 struct kmx_sweep;  // dimension-sweep path (kmx_sweep.hip)
@@ -160,33 +119,24 @@ struct kmx_plan {
     // bumped by every change of a term matrix or scale (batches refresh their copies)
     unsigned long long version = 0;
 
-    kmx_blockset* bs = nullptr;  // level-block path (null when disabled)
+    kmx_blockset* bs = nullptr;  // level-block structure (null unless KMX_PLAN_LEVEL_BLOCKS)
     kmx_sweep* sw = nullptr;     // dimension-sweep path (null when the structure does not allow it)
-};
-
-// Pair list handed to the pair-by-pair kernels (null members: the plan's own).
-struct kmx_pairs {
-    const long long* rp = nullptr;
-    const int* cols = nullptr;
-    const int* order = nullptr;
 };
 
 // Launchers (kmx_apply.hip)
 int kmx_launch_apply(kmx_plan* p, bool transpose, const double* x, double* y, int prec, hipStream_t s);
-int kmx_launch_pairs(kmx_plan* p, const kmx_pairs* pl, const double* x, double* y, int prec, hipStream_t s);
 void kmx_plan_apply_cached_variants(kmx_plan* p);
-int kmx_plan_init_splits(kmx_plan* p);
 
-// Level-block path (kmx_blocks.hip)
-void kmx_blocks_build(kmx_plan* p, const std::vector<int>& idx, std::vector<long long>&& rp,
-                      std::vector<int>&& cols, const double* const* mats);
-int kmx_blocks_split(kmx_plan* p, int slot, double beta);
-int kmx_blocks_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s);
+// Level-block structure (kmx_blocks.hip); rp / cols: the 0-based CSR pair list
+// (ignored for all-pairs plans)
+void kmx_blocks_build(kmx_plan* p, const std::vector<int>& idx, const std::vector<long long>& rp,
+                      const std::vector<int>& cols, const double* const* mats);
 void kmx_blocks_update_term(kmx_plan* p, int term, int dim, const double* mat, hipStream_t s);
 void kmx_blocks_free(kmx_plan* p);
-double kmx_blocks_default_beta();
 
-// Dimension-sweep path (kmx_sweep.hip)
+// Dimension-sweep path (kmx_sweep.hip).  slot: 0 FP64, 1 FP32, 2 BF16 (separable
+// sweeps only: BF16 values of the combined operators on the FP32 arena, BF16
+// matrix-core products with FP32 accumulation).
 void kmx_sweep_build(kmx_plan* p);
 int kmx_sweep_apply(kmx_plan* p, int slot, const double* x, double* y, hipStream_t s);
 void kmx_sweep_free(kmx_plan* p);
@@ -194,6 +144,8 @@ double kmx_sweep_flops(const kmx_plan* p);
 // whether FP64 applies may use the plan's sweep (a separable sweep whose undeclared
 // identity factors are exact to 4 ulps, or a general sweep)
 bool kmx_sweep_fp64_ok(const kmx_plan* p);
+// a separable sweep with BF16 values (BF16 applies, forward and, with its tables, adjoint)
+bool kmx_sweep_bf16_ok(const kmx_plan* p);
 void kmx_sweep_get_info(const kmx_plan* p, kmx_sweep_info* out);
 // after kmx_plan_update_term(term, dim) (the FP64 block copy already updated)
 void kmx_sweep_term_updated(kmx_plan* p, int term, int dim);
@@ -222,3 +174,4 @@ int kmx_sweep_batch_rk3(kmx_sweep_batch* sb, int slot, double* const* fs, const 
                         hipStream_t s);
 void kmx_sweep_batch_free(kmx_sweep_batch* sb);
 int kmx_sweep_batch_phases(const kmx_sweep_batch* sb);
+bool kmx_sweep_batch_has16(const kmx_sweep_batch* sb);  // every op separable with BF16 values

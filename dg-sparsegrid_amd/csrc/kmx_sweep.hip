@@ -72,6 +72,48 @@ struct SwMfma<float> {
     static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// BF16 values are kept as their bit patterns (the upper half of the FP32 pattern).
+typedef unsigned short bf16_bits;
+typedef short sw_s4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (NaN stays a quiet NaN).
+__device__ __forceinline__ short sw_to_bf16(float f) {
+    unsigned u = __builtin_bit_cast(unsigned, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return short((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return short(u >> 16);
+}
+
+// Four consecutive steps of a separable tile: lane (li, lk) holds A(row li,
+// position lk of step j) in av[j] and B(position lk of step j, column li) in bv[j],
+// both already zero where masked.  FP64 / FP32: one 16x16x4 product per step.
+// BF16 storage (TA = bf16_bits, T = float): the four steps are one 16x16x16 BF16
+// product with FP32 accumulation -- lane (li, lk) feeds k = 4 lk + j of row li and
+// of column li, so A and B pair up as in the per-step form and the result rows
+// are the same (D[4 (lane / 16) + r][lane % 16]).
+template <typename T, typename TA>
+struct SepMma {
+    using V = typename SwMfma<T>::V;
+    static __device__ __forceinline__ V run4(const TA* av, const T* bv, V acc) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc = SwMfma<T>::run(av[j], bv[j], acc);
+        return acc;
+    }
+};
+template <>
+struct SepMma<float, bf16_bits> {
+    using V = sw_f4;
+    static __device__ __forceinline__ V run4(const bf16_bits* av, const float* bv, V acc) {
+        sw_s4 a, b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            a[j] = short(av[j]);
+            b[j] = sw_to_bf16(bv[j]);
+        }
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+};
+
 // positions of level l along a dimension: deg * cells(l)
 __device__ __forceinline__ int deg_cells(int deg, int l) { return deg * (l ? 1 << (l - 1) : 1); }
 
@@ -434,7 +476,7 @@ int ud_kind_nbr(int kind) {
 // in the batch's x / y pointer arrays.
 struct UdBatchOp {
     UdOp a;
-    const void* mat[2];  // FP64, FP32 blocks of the plan
+    const void* mat[3];  // FP64, FP32 blocks of the plan; BF16 values (separable ops)
     void* arena[2];
     const int* xmap;
     const int* ymap;
@@ -443,12 +485,12 @@ struct UdBatchOp {
     int kind;
 };
 
-template <typename T, int NB>
-__device__ __forceinline__ void sep_tile(const T* __restrict__ mat, const int4* __restrict__ runs,
+template <typename T, typename TA, int NB>
+__device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int4* __restrict__ runs,
                                          T* __restrict__ arena, const int2 rv, const int w, const int lane,
                                          const UdOp& op, T* __restrict__ part);
-template <typename T>
-__device__ __forceinline__ void sep_tile_cols(const T* __restrict__ mat, const int4* __restrict__ runs,
+template <typename T, typename TA>
+__device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const int4* __restrict__ runs,
                                               T* __restrict__ arena, const int2 rv, const int lane, const UdOp& op);
 
 // One phase of a batch: ops of many instances, each on whole workgroups
@@ -481,9 +523,9 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
         case UK_UP_FINAL: ud_tile<T, UD_UP, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
         case UK_SEP:
             if (__builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
-                sep_tile_cols<T>(mat, o.runs, arena, rv, lane, o.a);
+                sep_tile_cols<T, T>(mat, o.runs, arena, rv, lane, o.a);
             else
-                sep_tile<T, 8>(mat, o.runs, arena, rv, w, lane, o.a, part);
+                sep_tile<T, T, 8>(mat, o.runs, arena, rv, w, lane, o.a, part);
             break;
         default:
             if constexpr (MAXNBR == 4) {
@@ -514,10 +556,13 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
 // read the panel's other rows (finite; never stored), rows past the matrix read
 // the storage's zero padding; positions past a run's end (its last step when
 // the run's positions are not a multiple of 4) read zero A and B is masked.
-template <typename T, int NB>
-__device__ __forceinline__ void sep_tile(const T* __restrict__ mat, const int4* __restrict__ runs,
+// TA: the storage type of A (T, or BF16 bit patterns with T = float; NB is a
+// multiple of 4, the steps of one BF16 product).
+template <typename T, typename TA, int NB>
+__device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int4* __restrict__ runs,
                                          T* __restrict__ arena, const int2 rv, const int w, const int lane,
                                          const UdOp& op, T* __restrict__ part) {
+    static_assert(NB % 4 == 0, "steps come in groups of four");
     using V = typename SwMfma<T>::V;
     const int gsize = __builtin_amdgcn_readlane(rv.x, R_GSIZE);
     const int gbase = __builtin_amdgcn_readlane(rv.x, R_GBASE);
@@ -550,7 +595,7 @@ __device__ __forceinline__ void sep_tile(const T* __restrict__ mat, const int4* 
             const int run0 = __builtin_amdgcn_readlane(rv.x, R_RUN0);
             const int rend = run0 + __builtin_amdgcn_readlane(rv.x, R_NRUN);
             const int slo = __builtin_amdgcn_readlane(rv.x, R_SLO), shi = __builtin_amdgcn_readlane(rv.x, R_SHI);
-            const T* amat = mat + op.a_off[0] + lane;
+            const TA* amat = mat + op.a_off[0] + lane;
             const T* bsrc = arena + op.in_base;
             // the tile's runs, one per lane (64 at a time)
             int c0 = run0;
@@ -596,13 +641,19 @@ __device__ __forceinline__ void sep_tile(const T* __restrict__ mat, const int4* 
                         }
                     }
                 }
-                T av[NB], bv[NB];
+                TA av[NB];
+                T bv[NB];
 #pragma unroll
                 for (int c = 0; c < NB; ++c) av[c] = amat[ao[c]];
 #pragma unroll
                 for (int c = 0; c < NB; ++c) bv[c] = bsrc[bo[c]];
 #pragma unroll
-                for (int c = 0; c < NB; ++c) acc = SwMfma<T>::run(pa[c] ? av[c] : T(0), pb[c] ? bv[c] : T(0), acc);
+                for (int c = 0; c < NB; ++c) {
+                    av[c] = pa[c] ? av[c] : TA(0);
+                    bv[c] = pb[c] ? bv[c] : T(0);
+                }
+#pragma unroll
+                for (int c = 0; c < NB; c += 4) acc = SepMma<T, TA>::run4(av + c, bv + c, acc);
             }
         }
         if (gsize > 1 && !leader)
@@ -626,9 +677,10 @@ __device__ __forceinline__ void sep_tile(const T* __restrict__ mat, const int4* 
 // offsets without the column term stay in registers, and each column tile then
 // costs its B loads, its matrix operations and its stores.  Two column tiles are
 // in flight at a time.  Holds one workgroup barrier, like sep_tile.
-template <typename T>
-__device__ __forceinline__ void sep_tile_cols(const T* __restrict__ mat, const int4* __restrict__ runs,
+template <typename T, typename TA>
+__device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const int4* __restrict__ runs,
                                               T* __restrict__ arena, const int2 rv, const int lane, const UdOp& op) {
+    static_assert(SEP_KMAX % 4 == 0, "steps come in groups of four");
     using V = typename SwMfma<T>::V;
     const int f0 = __builtin_amdgcn_readlane(rv.x, R_F0), nct = __builtin_amdgcn_readlane(rv.x, R_NCT);
     const int O = __builtin_amdgcn_readlane(rv.x, R_O), I = __builtin_amdgcn_readlane(rv.x, R_I);
@@ -645,10 +697,10 @@ __device__ __forceinline__ void sep_tile_cols(const T* __restrict__ mat, const i
     }
     const int ns = __builtin_amdgcn_readlane(rv.x, R_SHI);  // the panel's steps (from step 0)
     const int run0 = __builtin_amdgcn_readlane(rv.x, R_RUN0), nrun = __builtin_amdgcn_readlane(rv.x, R_NRUN);
-    const T* amat = mat + op.a_off[0] + lane;
+    const TA* amat = mat + op.a_off[0] + lane;
     const T* bsrc = arena + op.in_base;
     const int4 myr = runs[run0 + min(lane, nrun - 1)];  // nrun <= SEP_KMAX
-    T av[SEP_KMAX];
+    TA av[SEP_KMAX];
     int bp[SEP_KMAX], nli[SEP_KMAX];
     bool pb[SEP_KMAX];
     {
@@ -667,7 +719,7 @@ __device__ __forceinline__ void sep_tile_cols(const T* __restrict__ mat, const i
         for (int c = 0; c < SEP_KMAX; ++c) {
             const bool on = c < ns;
             pb[c] = on && 4 * sin + lk < npos;
-            av[c] = on ? amat[a + 64 * sin] : T(0);
+            av[c] = on ? amat[a + 64 * sin] : TA(0);
             bp[c] = pb[c] ? base + 4 * sin * I : 0;
             nli[c] = nl * I;
             if (on && ++sin == nst && c + 1 < ns) {
@@ -704,11 +756,8 @@ __device__ __forceinline__ void sep_tile_cols(const T* __restrict__ mat, const i
                         const T v = bsrc[ad];
                         bv[u][j] = ok ? v : T(0);
                     }
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    acc[0] = SwMfma<T>::run(av[c0 + j], bv[0][j], acc[0]);
-                    acc[1] = SwMfma<T>::run(av[c0 + j], bv[1][j], acc[1]);
-                }
+                acc[0] = SepMma<T, TA>::run4(av + c0, bv[0], acc[0]);
+                acc[1] = SepMma<T, TA>::run4(av + c0, bv[1], acc[1]);
             }
         }
 #pragma unroll
@@ -720,9 +769,10 @@ __device__ __forceinline__ void sep_tile_cols(const T* __restrict__ mat, const i
     __syncthreads();  // the one barrier every wave of the workgroup takes (sep_tile's partial sums)
 }
 
-// A phase of separable ops only (every op UD_SEP, one column tile): the small tile
-// body alone, so the launch keeps the occupancy its registers allow.
-template <typename T>
+// A phase of separable ops only (every op UD_SEP): the separable tile bodies
+// alone, so the launch keeps the occupancy their registers allow.  TA = T, or
+// BF16 values on the FP32 arena (T = float).
+template <typename T, typename TA>
 __global__ void __launch_bounds__(UD_WAVES * 64) k_sep_batch(const UdBatchOp* __restrict__ ops,
                                                               const int* __restrict__ wg_op,
                                                               const int2* __restrict__ recs) {
@@ -731,11 +781,13 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_sep_batch(const UdBatchOp* __
     const int lane = threadIdx.x & 63;
     const int2 rv = recs[((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane];
     const UdBatchOp& o = ops[wg_op[blockIdx.x]];
-    constexpr int sl = sizeof(T) == 8 ? 0 : 1;
+    constexpr int sl = sizeof(T) == 8 ? 0 : 1;            // arena
+    constexpr int ml = sizeof(TA) == 2 ? 2 : sl;          // values
+    const TA* mat = static_cast<const TA*>(o.mat[ml]);
     if (__builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
-        sep_tile_cols<T>(static_cast<const T*>(o.mat[sl]), o.runs, static_cast<T*>(o.arena[sl]), rv, lane, o.a);
+        sep_tile_cols<T, TA>(mat, o.runs, static_cast<T*>(o.arena[sl]), rv, lane, o.a);
     else
-        sep_tile<T, 8>(static_cast<const T*>(o.mat[sl]), o.runs, static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
+        sep_tile<T, TA, 8>(mat, o.runs, static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
 }
 
 }  // namespace
@@ -795,6 +847,7 @@ struct SepSide {
     int4* d_runs = nullptr;
     double* d_a64 = nullptr;
     float* d_a32 = nullptr;
+    unsigned short* d_a16 = nullptr;  // BF16 bit patterns of the same values (with the FP32 copy)
 };
 
 struct kmx_sweep {
@@ -1250,6 +1303,20 @@ void sep_runs(const kmx_plan* p, kmx_sweep& sw, int s) {
 // every op) of orientation `s` into its compact storage, FP64 and FP32, and the run
 // table; allocates the device buffers on first use.  Waits for the device first
 // (queued applies may read them).
+// FP64 -> BF16 bit pattern, rounded once to nearest even at 8 significant bits
+// (no detour through FP32, whose own rounding could move a tie).
+unsigned short host_bf16(double v) {
+    if (std::isnan(v)) return 0x7fc0;
+    if (v == 0.0 || std::isinf(v)) return (unsigned short)(__builtin_bit_cast(unsigned, float(v)) >> 16);
+    int e = 0;
+    const double m = std::frexp(v, &e);                     // |m| in [0.5, 1)
+    const double r = std::nearbyint(std::ldexp(m, 8));      // 8 significant bits, ties to even
+    const float f = float(std::ldexp(r, e - 8));            // exact unless out of range
+    const unsigned u = __builtin_bit_cast(unsigned, f);
+    if (std::fabs(f) < 1.1754943508222875e-38f) return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    return (unsigned short)(u >> 16);
+}
+
 void sep_values(const kmx_plan* p, kmx_sweep& sw, int s, int only = -1) {
     SepSide& sd = sw.side[s];
     const int deg = p->deg;
@@ -1259,7 +1326,8 @@ void sep_values(const kmx_plan* p, kmx_sweep& sw, int s, int only = -1) {
     if (!sd.d_runs) {
         if (hipMalloc(&sd.d_runs, std::max<size_t>(1, sd.runs.size()) * sizeof(int4)) != hipSuccess ||
             hipMalloc(&sd.d_a64, size_t(std::max(1LL, sd.a_elems)) * sizeof(double)) != hipSuccess ||
-            (want32 && hipMalloc(&sd.d_a32, size_t(std::max(1LL, sd.a_elems)) * sizeof(float)) != hipSuccess))
+            (want32 && hipMalloc(&sd.d_a32, size_t(std::max(1LL, sd.a_elems)) * sizeof(float)) != hipSuccess) ||
+            (want32 && hipMalloc(&sd.d_a16, size_t(std::max(1LL, sd.a_elems)) * sizeof(unsigned short)) != hipSuccess))
             throw std::bad_alloc();
         if (!sd.runs.empty() &&
             hipMemcpy(sd.d_runs, sd.runs.data(), sd.runs.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
@@ -1292,7 +1360,11 @@ void sep_values(const kmx_plan* p, kmx_sweep& sw, int s, int only = -1) {
             throw std::runtime_error("hipMemcpy failed");
         if (want32 && !v64.empty()) {
             std::vector<float> v32(v64.begin(), v64.end());
-            if (hipMemcpy(sd.d_a32 + lo, v32.data(), v32.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            std::vector<unsigned short> v16(v64.size());
+            for (size_t i = 0; i < v64.size(); ++i) v16[i] = host_bf16(v64[i]);
+            if (hipMemcpy(sd.d_a32 + lo, v32.data(), v32.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(sd.d_a16 + lo, v16.data(), v16.size() * sizeof(unsigned short), hipMemcpyHostToDevice) !=
+                    hipSuccess)
                 throw std::runtime_error("hipMemcpy failed");
         }
     }
@@ -1302,9 +1374,11 @@ void sep_side_free(SepSide& sd) {
     if (sd.d_runs) hipFree(sd.d_runs);
     if (sd.d_a64) hipFree(sd.d_a64);
     if (sd.d_a32) hipFree(sd.d_a32);
+    if (sd.d_a16) hipFree(sd.d_a16);
     sd.d_runs = nullptr;
     sd.d_a64 = nullptr;
     sd.d_a32 = nullptr;
+    sd.d_a16 = nullptr;
 }
 
 // Patterns, runs and values of both orientations, the flop counts, and the plan's
@@ -1780,6 +1854,7 @@ struct kmx_sweep_batch {
     int* d_wgop = nullptr;  // op of every workgroup, phases in order
     int* d_recs = nullptr;  // UD_WAVES work records per workgroup
     bool has32 = true;
+    bool has16 = true;     // every op separable with BF16 values (BF16 applies on the FP32 arenas)
     bool all_sep = true;   // every member separable (the fused RK3 step)
     long long kdof = 0;    // values of the RK3 stage workspace (every member's state)
     double* d_k = nullptr;  // RK3 stage workspace: k1 then k2, kdof each (first kmx_sweep_batch_rk3)
@@ -1836,8 +1911,10 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                     const bool sep = op.part == UD_SEP;
                     o.mat[0] = sep ? static_cast<const void*>(sw->side[side].d_a64) : p->bs->d_mat64;
                     o.mat[1] = sep ? static_cast<const void*>(sw->side[side].d_a32) : p->bs->d_mat32;
+                    o.mat[2] = sep ? static_cast<const void*>(sw->side[side].d_a16) : nullptr;
                     o.runs = sep ? sw->side[side].d_runs : nullptr;
                     if (sep && !sw->side[side].d_a32) sb->has32 = false;
+                    if (!sep || !sw->side[side].d_a16 || !sw->d_arena[1]) sb->has16 = false;
                     o.arena[0] = sw->d_arena[0];
                     o.arena[1] = sw->d_arena[1];
                     if (!sw->d_arena[1] || !p->bs->d_mat32) sb->has32 = false;
@@ -1950,6 +2027,10 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
         set_error("kmx_batch_apply: a sweep member has no FP32 blocks");
         return KMX_E_ARG;
     }
+    if (slot == 2 && !sb->has16) {
+        set_error("kmx_apply: BF16 runs on separable sweeps of plans with FP32 blocks only");
+        return KMX_E_UNSUPPORTED;
+    }
     const dim3 bg(unsigned(sb->box_wgs)), bb(256);
     if (sb->njobs > 0) {
         if (slot == 0) hipLaunchKernelGGL(k_box_gather_batch<double>, bg, bb, 0, s, sb->d_wgjobs, io);
@@ -1961,8 +2042,9 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
         const int2* rc = reinterpret_cast<const int2*>(sb->d_recs) + size_t(P.wg0) * UD_WAVES * (UD_REC / 2);
         const dim3 grid(unsigned(P.nwg)), block(UD_WAVES * 64);
         if (P.sep_only) {
-            if (slot == 0) hipLaunchKernelGGL(k_sep_batch<double>, grid, block, 0, s, sb->d_ops, wgop, rc);
-            else hipLaunchKernelGGL(k_sep_batch<float>, grid, block, 0, s, sb->d_ops, wgop, rc);
+            if (slot == 0) hipLaunchKernelGGL((k_sep_batch<double, double>), grid, block, 0, s, sb->d_ops, wgop, rc);
+            else if (slot == 1) hipLaunchKernelGGL((k_sep_batch<float, float>), grid, block, 0, s, sb->d_ops, wgop, rc);
+            else hipLaunchKernelGGL((k_sep_batch<float, bf16_bits>), grid, block, 0, s, sb->d_ops, wgop, rc);
         } else if (slot == 0) {
             if (P.maxnbr <= 2) hipLaunchKernelGGL((k_ud_batch<double, 2>), grid, block, 0, s, sb->d_ops, wgop, rc, io);
             else hipLaunchKernelGGL((k_ud_batch<double, 4>), grid, block, 0, s, sb->d_ops, wgop, rc, io);
@@ -1993,6 +2075,10 @@ int kmx_sweep_batch_rk3(kmx_sweep_batch* sb, int slot, double* const* fs, const 
         set_error("kmx_batch_rk3: a sweep member has no FP32 blocks");
         return KMX_E_ARG;
     }
+    if (slot == 2 && !sb->has16) {
+        set_error("kmx_batch_rk3: BF16 runs on separable sweeps of plans with FP32 blocks only");
+        return KMX_E_UNSUPPORTED;
+    }
     for (const kmx_plan* p : sb->members)
         if (p->sw->stale || (slot == 0 && !p->sw->fp64)) {
             set_error("kmx_batch_rk3: a member's sweep cannot serve this precision (replaced identity factor, or "
@@ -2020,8 +2106,9 @@ int kmx_sweep_batch_rk3(kmx_sweep_batch* sb, int slot, double* const* fs, const 
             const int* wgop = sb->d_wgop + P.wg0;
             const int2* rc = reinterpret_cast<const int2*>(sb->d_recs) + size_t(P.wg0) * UD_WAVES * (UD_REC / 2);
             const dim3 grid(unsigned(P.nwg)), block(UD_WAVES * 64);
-            if (slot == 0) hipLaunchKernelGGL(k_sep_batch<double>, grid, block, 0, s, sb->d_ops, wgop, rc);
-            else hipLaunchKernelGGL(k_sep_batch<float>, grid, block, 0, s, sb->d_ops, wgop, rc);
+            if (slot == 0) hipLaunchKernelGGL((k_sep_batch<double, double>), grid, block, 0, s, sb->d_ops, wgop, rc);
+            else if (slot == 1) hipLaunchKernelGGL((k_sep_batch<float, float>), grid, block, 0, s, sb->d_ops, wgop, rc);
+            else hipLaunchKernelGGL((k_sep_batch<float, bf16_bits>), grid, block, 0, s, sb->d_ops, wgop, rc);
         }
     };
     auto stage = [&](auto kern, int next) { hipLaunchKernelGGL(kern, bg, bb, 0, s, sb->d_wgjobs, a, next); };
@@ -2058,6 +2145,14 @@ void kmx_sweep_batch_free(kmx_sweep_batch* sb) {
 }
 
 int kmx_sweep_batch_phases(const kmx_sweep_batch* sb) { return sb ? int(sb->phases.size()) : 0; }
+
+bool kmx_sweep_batch_has16(const kmx_sweep_batch* sb) { return sb && sb->has16; }
+
+bool kmx_sweep_bf16_ok(const kmx_plan* p) {
+    const kmx_sweep* sw = p ? p->sw : nullptr;
+    return sw && sw->separable && sw->self && !sw->stale && sw->d_arena[1] && sw->side[0].d_a16 &&
+           sw->self->has16;
+}
 
 void kmx_sweep_term_updated(kmx_plan* p, int term, int dim) {
     kmx_sweep* sw = p->sw;

@@ -55,7 +55,7 @@ int main(void) {
                             }
                 ref[i * B + a1 * DEG + a2] = s;
             }
-    kmx_plan_desc desc = {D, DEG, N, T, rows, (const double* const*)mats, idx, NULL, NULL, 1, KMX_PLAN_FP32};
+    kmx_plan_desc desc = {D, DEG, N, T, rows, (const double* const*)mats, idx, NULL, NULL, 1, KMX_PLAN_FP32, NULL};
     kmx_plan* plan = NULL;
     if (kmx_plan_create(&desc, &plan) != KMX_OK) {
         fprintf(stderr, "kmx_plan_create: %s\n", kmx_last_error());

@@ -123,7 +123,6 @@ PROTOTYPES = {
     "kmx_plan_destroy": (C.c_int, [C.c_void_p]),
     "kmx_plan_autotune": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kmx_plan_get_variant": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
-    "kmx_plan_set_split": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "kmx_plan_set_sweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "kmx_plan_get_sweep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_double)]),
@@ -132,9 +131,6 @@ PROTOTYPES = {
                                        C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "kmx_realspace_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kmx_realspace_destroy": (C.c_int, [C.c_void_p]),
-    "kmx_plan_get_split": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
-                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
-                                     C.POINTER(C.c_double)]),
     "kmx_plan_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
     "kmx_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "kmx_apply_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),

@@ -57,7 +57,8 @@ class BatchedSystem:
     sweep: None (default) applies every instance pair by pair (the persistent
     launch); a tuple of precisions, e.g. ("fp32",), builds level-block plans and
     runs every instance that has a dimension-sweep path as one merged sweep
-    (one launch per phase for all of them) in those precisions."""
+    (one launch per phase for all of them) in those precisions ("bf16" runs on
+    the FP32 merged sweep's tables, with every instance separable)."""
 
     def __init__(self, instances, device="cuda", sweep=None):
         import torch
@@ -80,7 +81,7 @@ class BatchedSystem:
         self.batch = Batch(self.plans)
         for prec in sweep:
             self.batch.set_sweep(prec, True)
-        self.sweep = {prec: self.batch.sweep(prec) for prec in ("fp64", "fp32")}
+        self.sweep = {prec: self.batch.sweep(prec) for prec in ("fp64", "fp32", "bf16")}
         sizes = [p.dof for p in self.plans]
         self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
         self.dof = int(self.offsets[-1])
@@ -91,7 +92,7 @@ class BatchedSystem:
         self.flops = sum(p.P * p.T * 2 * p.ndim * p.deg ** (p.ndim + 1) for p in self.plans)
         # flops one apply performs per precision (merged sweeps + the pair path of the rest)
         self.flops_by_prec = {}
-        for prec in ("fp64", "fp32"):
+        for prec in ("fp64", "fp32", "bf16"):
             on = self.sweep[prec]["active"]
             self.flops_by_prec[prec] = sum(
                 p.sweep()["flops"] if on and p.sweep()["available"] else p.P * p.T * 2 * p.ndim * p.deg ** (p.ndim + 1)

@@ -61,10 +61,9 @@ class Plan:
     index_base 1 if idx1d / cols are 1-based (MATLAB), 0 otherwise
     fp32      keep FP32 block copies (KMX_PLAN_FP32)
     adjoint   keep the transposed pair list (KMX_PLAN_ADJOINT)
-    level_blocks  also build the level-block structure (KMX_PLAN_LEVEL_BLOCKS):
-              block pairs of the pair list may then run as mode products
-              (set_split / autotune) and the dimension sweep may be built;
-              off, every apply runs pair by pair
+    level_blocks  also detect the level-block structure (KMX_PLAN_LEVEL_BLOCKS)
+              and build the dimension sweep on it (set_sweep / autotune; BF16
+              applies need its separable form); off, every apply runs pair by pair
     identity  None, or T x d booleans declaring factors the identity (MASS
               pterms with g = 1): the separable sweep takes them as exactly I
               (kmx_plan_desc.identity)
@@ -160,17 +159,12 @@ class Plan:
     def autotune(self, stream=None):
         """Time the kernel variants on this plan and keep the fastest (kmx_plan_autotune)."""
         check(lib().kmx_plan_autotune(self.handle, _stream(stream)))
-        return {p: dict(self.variant(p), **self.split(p), sweep=self.sweep(p)) for p in ("fp64", "fp32")}
+        return {p: dict(self.variant(p), sweep=self.sweep(p)) for p in ("fp64", "fp32")}
 
     def variant(self, prec="fp64"):
         nt, lds = C.c_int(), C.c_int()
         check(lib().kmx_plan_get_variant(self.handle, _prec(prec), C.byref(nt), C.byref(lds)))
         return {"threads": nt.value, "lds_staged": bool(lds.value), "lds_mode": lds.value}
-
-    def set_split(self, prec, beta):
-        """Level-block split threshold (kmx_plan_set_split): beta < 0 runs every
-        block pair that fits factorized, beta = inf none (pair by pair)."""
-        check(lib().kmx_plan_set_split(self.handle, _prec(prec), float(beta)))
 
     def set_sweep(self, prec, on=True):
         """Use (or stop using) the dimension-sweep path for prec (kmx_plan_set_sweep)."""
@@ -189,14 +183,6 @@ class Plan:
         for f in ("available", "separable", "fp64"):
             out[f] = bool(out[f])
         return out
-
-    def split(self, prec="fp64"):
-        st, beta = C.c_int(), C.c_double()
-        bp, pb, rem, fl = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
-        check(lib().kmx_plan_get_split(self.handle, _prec(prec), C.byref(st), C.byref(beta), C.byref(bp),
-                                       C.byref(pb), C.byref(rem), C.byref(fl)))
-        return {"structured": bool(st.value), "beta": beta.value, "block_pairs": bp.value,
-                "pairs_in_blocks": pb.value, "remaining_pairs": rem.value, "flops": fl.value}
 
     def set_term_scale(self, t, alpha):
         check(lib().kmx_plan_set_term_scale(self.handle, int(t), float(alpha)))

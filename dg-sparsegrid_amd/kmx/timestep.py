@@ -15,6 +15,7 @@
   reaches the requested tolerance.
 """
 import math
+import weakref
 
 import numpy as np
 
@@ -180,30 +181,34 @@ class RK3Graph:
 
 # ------------------------------------------------------------ implicit (BE)
 class DeviceGMRES:
-    """Restarted GMRES with the Arnoldi step on the device, FP64 vectors.
+    """Restarted GMRES with the Arnoldi steps on the device, FP64 vectors.
 
     apply_into(x, y) writes y = M x (device, capturable).  One Arnoldi step --
     the product with basis vector k, classical Gram-Schmidt against the whole
     (restart + 1)-row basis twice (rows past k are zero, so the shapes never
-    change), the norm and the next basis vector -- is captured once in a HIP
-    graph with k held on the device; each replay copies the step's Hessenberg
-    column to pinned host memory, where the Givens rotations and the residual
-    estimate are updated (one small copy and one synchronisation per
-    iteration, no per-dot round trips)."""
+    change), the norm and the next basis vector -- is captured `chunk` times in
+    a row in one HIP graph, with k held on the device; each replay copies the
+    chunk's Hessenberg columns to pinned host memory, where the Givens rotations
+    and the residual estimate are updated (one synchronisation per chunk, no
+    per-dot round trips).  A chunk may run up to chunk - 1 steps past
+    convergence; their basis vectors are not used.  restart must be a multiple
+    of chunk."""
 
-    def __init__(self, apply_into, n, restart=40, device="cuda"):
+    def __init__(self, apply_into, n, restart=40, device="cuda", chunk=4):
         torch = _torch()
-        self.apply_into, self.n, self.m = apply_into, int(n), int(restart)
+        if restart % chunk:
+            raise ValueError("restart must be a multiple of chunk")
+        self.apply_into, self.n, self.m, self.chunk = apply_into, int(n), int(restart), int(chunk)
         m = self.m
         self.V = torch.zeros((m + 1, self.n), dtype=torch.float64, device=device)
         self.w = torch.empty(self.n, dtype=torch.float64, device=device)
         self.xin = torch.empty(self.n, dtype=torch.float64, device=device)
         self.k = torch.zeros(1, dtype=torch.int64, device=device)
-        self.hdev = torch.zeros(m + 2, dtype=torch.float64, device=device)
-        self.hhost = torch.zeros(m + 2, dtype=torch.float64).pin_memory()
+        self.hdev = torch.zeros((self.chunk, m + 2), dtype=torch.float64, device=device)
+        self.hhost = torch.zeros((self.chunk, m + 2), dtype=torch.float64).pin_memory()
         self.graph = None
 
-    def _step(self):
+    def _step(self, c):
         torch = _torch()
         V, w = self.V, self.w
         self.xin.copy_(V.index_select(0, self.k).view(-1))
@@ -215,10 +220,14 @@ class DeviceGMRES:
         h.add_(h2)
         nrm = torch.sqrt(torch.mv(w.view(1, -1), w))  # a one-row product: no reduction workspace
         V.index_copy_(0, self.k + 1, (w / nrm).view(1, -1))
-        self.hdev[: self.m + 1].copy_(h)
-        self.hdev[self.m + 1:].copy_(nrm)
-        self.hhost.copy_(self.hdev, non_blocking=True)
+        self.hdev[c, : self.m + 1].copy_(h)
+        self.hdev[c, self.m + 1:].copy_(nrm)
         self.k.add_(1)
+
+    def _chunk(self):
+        for c in range(self.chunk):
+            self._step(c)
+        self.hhost.copy_(self.hdev, non_blocking=True)
 
     def _capture(self):
         torch = _torch()
@@ -226,18 +235,18 @@ class DeviceGMRES:
         side.wait_stream(torch.cuda.current_stream())
         saved = self.V.clone(), self.k.clone()
         with torch.cuda.stream(side):
-            self._step()  # warm the kernels outside the capture
+            self._chunk()  # warm the kernels outside the capture
         torch.cuda.current_stream().wait_stream(side)
         self.V.copy_(saved[0])
         self.k.copy_(saved[1])
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self._step()
+            self._chunk()
         torch.cuda.synchronize()
 
     def solve(self, b, tol=1e-6, maxiter=400):
         """x with ||b - M x|| <= tol ||b|| (estimate), from x = 0.  Returns
-        (x, relative residual estimate, iterations)."""
+        (x, relative residual estimate, iterations used)."""
         torch = _torch()
         m = self.m
         x = torch.zeros_like(b)
@@ -259,27 +268,33 @@ class DeviceGMRES:
             cs, sn = np.zeros(m), np.zeros(m)
             g = np.zeros(m + 1)
             g[0] = beta
-            k_used = 0
-            for k in range(min(m, maxiter - it)):
+            k_used, done = 0, False
+            for k0 in range(0, m, self.chunk):
+                if done or it >= maxiter:
+                    break
                 self.graph.replay()
                 torch.cuda.current_stream().synchronize()
-                hv = self.hhost.numpy()
-                H[: k + 1, k] = hv[: k + 1]
-                H[k + 1, k] = hv[m + 1]
-                for j in range(k):
-                    tmp = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
-                    H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
-                    H[j, k] = tmp
-                den = math.hypot(H[k, k], H[k + 1, k])
-                cs[k], sn[k] = (1.0, 0.0) if den == 0 else (H[k, k] / den, H[k + 1, k] / den)
-                H[k, k] = cs[k] * H[k, k] + sn[k] * H[k + 1, k]
-                H[k + 1, k] = 0.0
-                g[k + 1] = -sn[k] * g[k]
-                g[k] = cs[k] * g[k]
-                it += 1
-                k_used = k + 1
-                if abs(g[k + 1]) / bnorm <= tol or hv[m + 1] == 0.0:
-                    break
+                hv_all = self.hhost.numpy()
+                for c in range(self.chunk):
+                    k = k0 + c
+                    hv = hv_all[c]
+                    H[: k + 1, k] = hv[: k + 1]
+                    H[k + 1, k] = hv[m + 1]
+                    for j in range(k):
+                        tmp = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
+                        H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
+                        H[j, k] = tmp
+                    den = math.hypot(H[k, k], H[k + 1, k])
+                    cs[k], sn[k] = (1.0, 0.0) if den == 0 else (H[k, k] / den, H[k + 1, k] / den)
+                    H[k, k] = cs[k] * H[k, k] + sn[k] * H[k + 1, k]
+                    H[k + 1, k] = 0.0
+                    g[k + 1] = -sn[k] * g[k]
+                    g[k] = cs[k] * g[k]
+                    it += 1
+                    k_used = k + 1
+                    if abs(g[k + 1]) / bnorm <= tol or hv[m + 1] == 0.0 or it >= maxiter:
+                        done = True
+                        break
             yk = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used])
             x += torch.from_numpy(yk).to(b.device) @ self.V[:k_used]
             self.apply_into(x, self.w)
@@ -299,6 +314,9 @@ def gmres(matvec, b, x0=None, tol=1e-6, restart=40, maxiter=400):
         float(torch.linalg.vector_norm(r0)), 1e-300), maxiter=maxiter)
     x = x0 + d
     return x, float(torch.linalg.vector_norm(b - matvec(x))) / max(float(torch.linalg.vector_norm(b)), 1e-300), it
+
+
+_SOLVERS = weakref.WeakKeyDictionary()  # plan -> {(dt, inner precision, restart, n): (DeviceGMRES, its buffer)}
 
 
 def backward_euler(plan, f0, dt, rhs=None, tol=1e-10, inner_prec="fp32", inner_tol=1e-5, restart=40,
@@ -323,7 +341,13 @@ def backward_euler(plan, f0, dt, rhs=None, tol=1e-10, inner_prec="fp32", inner_t
         plan.apply(x, tmp, prec=inner_prec)
         torch.add(x, tmp, alpha=-dt, out=y)
 
-    solver = DeviceGMRES(op_in_into, f0.numel(), restart, f0.device)
+    key = (float(dt), inner_prec, int(restart), f0.numel())
+    cache = _SOLVERS.setdefault(plan, {})
+    if key not in cache:  # the captured Arnoldi graph is reused by later steps with the same dt
+        cache[key] = (DeviceGMRES(op_in_into, f0.numel(), restart, f0.device), tmp)
+    solver, tmp_cached = cache[key]
+    if tmp_cached is not tmp:
+        tmp = tmp_cached  # the captured graph writes the cached buffer
     f = f0.clone()  # f0 is a good first guess for small dt
     hist, inner = [], 0
     for _ in range(max_refine):

@@ -40,17 +40,19 @@ extern "C" {
 /* ---- precision of the contractions (x and y are always FP64 in memory) */
 #define KMX_PREC_FP64 0       /* plain FP64, the reference's arithmetic (parity <= 1e-13 rel.) */
 #define KMX_PREC_FP32 1       /* FP32 operands and accumulation, same pair-by-pair algorithm */
-#define KMX_PREC_BF16 3       /* BF16 operands on the matrix cores, FP32 accumulation: the
-                                 level-block mode products of a KMX_PLAN_LEVEL_BLOCKS |
-                                 KMX_PLAN_FP32 plan whose whole pair list factorizes (else
-                                 KMX_E_UNSUPPORTED).  A precision study: ~1e-3 relative */
+#define KMX_PREC_BF16 3       /* BF16 operands on the matrix cores (16x16x16 BF16 products),
+                                 FP32 accumulation: the separable dimension sweep of a
+                                 KMX_PLAN_LEVEL_BLOCKS | KMX_PLAN_FP32 plan, its combined
+                                 operators rounded once to BF16, x rounded as it is read
+                                 (else KMX_E_UNSUPPORTED).  ~2e-3 relative: a precision study */
 
 /* ---- plan flags */
 #define KMX_PLAN_FP32 1u    /* also keep FP32 copies of the coefficient blocks */
 #define KMX_PLAN_ADJOINT 2u /* also keep the transposed pair list / blocks (kmx_apply_transpose) */
-#define KMX_PLAN_LEVEL_BLOCKS 4u /* also build the level-block structure (kmx_plan_set_split);
-                                    without it every apply runs pair by pair, in the reference's
-                                    order of operations (apply_A.m:100-237) */
+#define KMX_PLAN_LEVEL_BLOCKS 4u /* also detect the level-block structure and build the
+                                    dimension sweep on it (kmx_plan_set_sweep); without it every
+                                    apply runs pair by pair, in the reference's order of
+                                    operations (apply_A.m:100-237) */
 
 typedef struct kmx_plan kmx_plan;
 
@@ -111,36 +113,18 @@ int kmx_plan_autotune(kmx_plan* plan, void* stream);
  * coefficient strips (0 none, 1 every dim, 2 every dim but dim 1). */
 int kmx_plan_get_variant(const kmx_plan* plan, int prec, int* threads, int* lds_staged);
 
-/* Level-block path.  Elements sharing a level multi-index form a dense cell
- * tensor; when the pairs between two such blocks are the tensor product of
- * per-dimension 1D connections (connect_nD.m:63-106 builds them that way, an
- * all-pairs list trivially), the plan may apply that block pair as d mode
- * products of masked 1D sub-matrices instead of one Kronecker chain per
- * element pair -- the same terms, summed in another order.  A block pair runs
- * factorized when its flops plus beta per contraction stage are below its
- * pair-by-pair flops.  beta < 0: every block pair that fits; beta = +inf:
- * none (pure pair-by-pair, the default).  Needs KMX_PLAN_LEVEL_BLOCKS;
- * kmx_plan_autotune then also times a few thresholds.  No counterpart in the
- * reference. */
-int kmx_plan_set_split(kmx_plan* plan, int prec, double beta);
-
-/* structured = 1 when the pair list has the level-block structure; the
- * current beta, the block pairs run factorized, the element pairs they cover,
- * the element pairs left to the pair-by-pair kernel, and the flops one apply
- * performs (factorized mode products + 2 d deg^(d+1) per remaining pair and
- * term), for prec.  Any output pointer may be null. */
-int kmx_plan_get_split(const kmx_plan* plan, int prec, int* structured, double* beta, int64_t* block_pairs,
-                       int64_t* pairs_in_blocks, int64_t* remaining_pairs, double* flops);
-
-/* Dimension-sweep path of a KMX_PLAN_LEVEL_BLOCKS plan: when every pair of
- * level groups is the tensor product of the 1D connections (connected or not)
- * and the level tuples form a downward-closed set (sparse and full grids),
- * the apply runs as d sweeps, one GEMM per (source prefix, target suffix) of
- * level groups per dimension, on the matrix cores -- the same terms in
- * another order.  set: on = 1 uses it for prec (KMX_E_UNSUPPORTED when the
- * plan has none), 0 goes back; kmx_plan_autotune also times it.  get:
- * whether the plan has it, whether prec uses it, and its flops per apply.
- * No counterpart in the reference. */
+/* Dimension-sweep path of a KMX_PLAN_LEVEL_BLOCKS plan.  Elements sharing a
+ * level multi-index form a dense cell tensor; when every pair of such level
+ * groups is the tensor product of the 1D connections (connect_nD.m:63-106
+ * builds them that way, an all-pairs list trivially; connected or not) and the
+ * level tuples form a downward-closed set (sparse and full grids), the apply
+ * runs as d sweeps, one GEMM per (source prefix, target suffix) of level
+ * groups per dimension, on the matrix cores -- the same terms in another
+ * order.  set: on = 1 uses it for prec (KMX_E_UNSUPPORTED when the plan has
+ * none), 0 goes back; kmx_plan_autotune also times it.  BF16 runs on the
+ * separable form only and always (on = 1 is accepted when it is available,
+ * on = 0 refused).  get: whether the plan has it for prec, whether prec uses
+ * it, and its flops per apply.  No counterpart in the reference. */
 int kmx_plan_set_sweep(kmx_plan* plan, int prec, int on);
 int kmx_plan_get_sweep(const kmx_plan* plan, int prec, int* available, int* active, double* flops);
 
@@ -201,7 +185,9 @@ int kmx_batch_apply(kmx_batch* batch, const double* const* xs, double* const* ys
 int kmx_batch_destroy(kmx_batch* batch);
 
 /* Merged dimension sweeps for a batch.  on = 1: for prec (KMX_PREC_FP64 /
- * KMX_PREC_FP32), every plan of the batch that has a dimension-sweep path
+ * KMX_PREC_FP32; KMX_PREC_BF16 names the FP32 merged sweep, whose tables BF16
+ * applies use when every plan of the batch is a separable member of it), every
+ * plan of the batch that has a dimension-sweep path
  * (kmx_plan_get_sweep; plans created with KMX_PLAN_LEVEL_BLOCKS) runs it, and
  * the sweeps of all such plans share one launch per phase (a 3D plan has five
  * phases: two down, one whole operator, two up); the other plans keep the
@@ -217,7 +203,8 @@ int kmx_batch_get_sweep(const kmx_batch* batch, int prec, int* members, int* act
 
 /* nsteps explicit RK3 steps of every plan of the batch, y' = A y (the Kutta RK3
  * of time_advance.m:151-204 -- a21 = 1/2, a31 = -1, a32 = 2, b = 1/6, 2/3, 1/6 --
- * without source or boundary terms), with the stage arithmetic fused into the
+ * without source or boundary terms; contractions in prec, states and stage
+ * derivatives in FP64), with the stage arithmetic fused into the
  * merged separable sweeps: per stage one launch of the dimension ops and one that
  * sums them into the stage derivative and writes the next stage's input (plus one
  * launch per call that reads the states in).  fs[i]: plan i's DEVICE state, N*deg^d

@@ -24,9 +24,10 @@ Parity status per function (what pins it; tests/test_oracle_gold.py):
   gemm_vbatched
       per-entry closed form (numpy matmul)               -- pinned
   wavelet_to_realspace
-      numpy.kron closed form summed over elements        -- pinned;
-      gold/generated-inputs/transformations/wavelet_to_realspace_* need the
-      Meval of matrix_plot_D.m, which is not restated -- gold parity unpinned
+      numpy.kron closed form summed over elements, and
+      gold/generated-inputs/transformations/wavelet_to_realspace_continuity_{1,2,3}
+      with the evaluation matrices of kmx.coeffs.matrix_plot_D (itself pinned by
+      gold matrix_plot_D/*)                               -- pinned
 """
 import ctypes as C
 import os

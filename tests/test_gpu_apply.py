@@ -109,6 +109,35 @@ def test_3d_lev7_full_size_fp64(kmx_gpu):
     assert rel(z, 2 * y64 - 3 * yw) <= 1e-13
 
 
+def test_3d_lev7_continuity3_operators(kmx_gpu):
+    """The same size with continuity3's own 1D operators (upwind DIV, MASS;
+    old hash + connect_nD, deg 3): the FP64 drop-in within 1e-13 of the oracle,
+    FP32 within 5e-6, and the separable sweep (FP32 and FP64) likewise."""
+    import torch
+    from kmx import coeffs, grid
+
+    nd, lev, deg = 3, 7, 3
+    table = grid.hash_table_nd([lev] * nd, "SG")
+    rp, cols = grid.connect_nd(table, lev, lev, "SG")
+    mats, _ = coeffs.operator_mats("continuity3", nd, lev, deg)
+    x = np.random.default_rng(11).standard_normal(table.num_elem * deg ** nd)
+    plan = kmx_gpu.Plan(mats, table.idx1d, deg, rp, cols, fp32=True, level_blocks=True)
+    xd = torch.from_numpy(x).cuda()
+    yo = oracle_y(mats, table.idx1d, deg, rp, cols, x, threads=16)
+    r64 = rel(plan.apply(xd, prec="fp64").cpu().numpy(), yo)
+    r32 = rel(plan.apply(xd, prec="fp32").cpu().numpy(), yo)
+    print(f"continuity3 lev 7: fp64 {r64:.2e}, fp32 {r32:.2e}")
+    assert r64 <= TOL64 and r32 <= TOL32
+    assert plan.sweep_info()["separable"]
+    for prec, tol in (("fp32", TOL32), ("fp64", TOL64)):
+        assert plan.sweep(prec)["available"], prec
+        plan.set_sweep(prec, True)
+        rs = rel(plan.apply(xd, prec=prec).cpu().numpy(), yo)
+        print(f"  separable sweep {prec}: {rs:.2e}")
+        assert rs <= tol
+        plan.set_sweep(prec, False)
+
+
 def test_autotune_keeps_results(kmx_gpu):
     """Every kernel variant the autotuner may pick computes the same sum."""
     import torch

@@ -1,11 +1,13 @@
-"""GPU parity of the level-block (sum-factorized) apply against the oracle.
+"""GPU parity of the dimension-sweep applies built on the level-block structure
+(KMX_PLAN_LEVEL_BLOCKS) against the oracle.
 
-The level-block path (opt-in, KMX_PLAN_LEVEL_BLOCKS) sums exactly the terms
-apply_A.m:100-237 sums (the plan verifies that every connected pair of level blocks is the tensor
-product of per-dimension 1D connections before enabling it), so the FP64 bar
-is the plain one: 1e-13 relative (2-norm).  FP32: 5e-6.  Each case is run
-with every fitting block pair factorized (beta < 0), with a mixed split, and
-with none (beta = inf), all against the same oracle result.
+Elements sharing a level multi-index form dense cell tensors; when every
+connected pair of such blocks is the tensor product of per-dimension 1D
+connections (connect_nD.m:63-106 builds them that way, an all-pairs list
+trivially), the plan records the structure and builds a dimension sweep that
+sums exactly the terms apply_A.m:100-237 sums, in another order.  FP64 bar:
+1e-13 relative (2-norm); FP32: 5e-6.  Pair lists without that structure keep
+the pair-by-pair kernels.
 """
 import numpy as np
 import pytest
@@ -16,7 +18,6 @@ pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-13
 TOL32 = 5e-6
-SPLITS = (-1.0, 4000.0, float("inf"))
 
 
 def rel(a, b):
@@ -41,75 +42,26 @@ def oracle_y(mats, idx, deg, rp, cols, x):
     return O.apply_A(O.ApplyInputs(mats, idx, deg, rp, cols), x, nthreads=8)
 
 
-def check_all_splits(kmx, mats, idx, deg, rp, cols, x, want_structured=True, tol64=TOL64):
+def test_unstructured_pair_list_keeps_the_pair_kernels(kmx_gpu):
+    """A pair list that is not a per-block tensor product (random subset of
+    connect_nD): no sweep, and the pair-by-pair apply still matches."""
     import torch
 
-    yo = oracle_y(mats, idx, deg, rp, cols, x)
-    plan = kmx.Plan(mats, idx, deg, rp, cols, index_base=1, fp32=True, level_blocks=True)
-    xd = torch.from_numpy(x).cuda()
-    info = []
-    for beta in SPLITS:
-        for prec, tol in (("fp64", tol64), ("fp32", TOL32)):
-            plan.set_split(prec, beta)
-            y = plan.apply(xd, prec=prec).cpu().numpy()
-            sp = plan.split(prec)
-            assert sp["structured"] == want_structured
-            if not want_structured or beta == float("inf"):
-                assert sp["block_pairs"] == 0 and sp["remaining_pairs"] == plan.P
-            else:
-                assert sp["pairs_in_blocks"] + sp["remaining_pairs"] == plan.P
-            e = rel(y, yo)
-            info.append((beta, prec, sp["block_pairs"], e))
-            assert e <= tol, (beta, prec, sp, e)
-    plan.destroy()
-    return info
-
-
-@pytest.mark.parametrize(
-    "nd,lev,deg,T,conn",
-    [
-        (2, 3, 2, 2, False), (2, 3, 2, 2, True),   # 2D lev 3 deg 2: new hash all pairs / old hash connect_nD
-        (2, 5, 1, 2, True), (2, 5, 4, 2, False), (2, 4, 3, 3, True),
-        (3, 4, 2, 3, True), (3, 3, 3, 3, False), (3, 4, 4, 2, True),
-        (4, 3, 2, 4, True), (5, 2, 2, 2, True), (6, 3, 2, 6, True),   # 6D lev 3: the 6D lev 4 shape at a lower level
-    ],
-)
-def test_level_blocks_match_oracle(kmx_gpu, nd, lev, deg, T, conn):
-    mats, idx, rp, cols, x = case(nd, lev, deg, T, conn, seed=7 + nd * 100 + lev * 10 + deg)
-    check_all_splits(kmx_gpu, mats, idx, deg, rp, cols, x)
-
-
-def test_3d_lev7_level_blocks(kmx_gpu):
-    """3D lev 7 deg 3 with the maximal connect_nD list (P = 1,903,000)."""
-    mats, idx, rp, cols, x = case(3, 7, 3, 3, True, seed=2)
-    info = check_all_splits(kmx_gpu, mats, idx, 3, rp, cols, x)
-    assert any(b > 0 for _, _, b, _ in info)
-
-
-def test_adapted_grid_all_pairs(kmx_gpu):
-    """Incomplete level blocks (refined grid, all pairs): absent cells of a
-    block are zero columns and discarded rows."""
-    from kmx import grid as G
-
-    rng = np.random.default_rng(11)
-    base = G.hash_table_sparse_nd([4, 4, 4], 6, "SG")
-    table = G.refine_random(base, 6, 0.6, rng)
-    lev = int(table.lev.max())
-    n = 2 * 2 ** lev
-    mats = [[rng.standard_normal((n, n)) for _ in range(3)] for _ in range(2)]
-    x = rng.standard_normal(table.num_elem * 8)
-    check_all_splits(kmx_gpu, mats, table.idx1d, 2, None, None, x)
-
-
-def test_unstructured_pair_list_falls_back(kmx_gpu):
-    """A pair list that is not a per-block tensor product (random subset of
-    connect_nD) keeps the pair-by-pair path and still matches."""
     mats, idx, rp, cols, x = case(3, 4, 2, 2, True, seed=5)
     rng = np.random.default_rng(3)
     keep = rng.random(len(cols)) < 0.7
     nrp = np.zeros_like(rp)
     nrp[1:] = np.cumsum([keep[rp[i]:rp[i + 1]].sum() for i in range(len(rp) - 1)])
-    check_all_splits(kmx_gpu, mats, idx, 2, nrp, cols[keep], x, want_structured=False)
+    ncols = cols[keep]
+    plan = kmx_gpu.Plan(mats, idx, 2, nrp, ncols, index_base=1, fp32=True, level_blocks=True)
+    assert not plan.sweep_info()["available"]
+    with pytest.raises(kmx_gpu.KmxError):
+        plan.set_sweep("fp32", True)
+    yo = oracle_y(mats, idx, 2, nrp, ncols, x)
+    xd = torch.from_numpy(x).cuda()
+    for prec, tol in (("fp64", TOL64), ("fp32", TOL32)):
+        assert rel(plan.apply(xd, prec=prec).cpu().numpy(), yo) <= tol, prec
+    plan.destroy()
 
 
 def test_plans_are_pair_by_pair_by_default(kmx_gpu):
@@ -117,17 +69,18 @@ def test_plans_are_pair_by_pair_by_default(kmx_gpu):
 
     mats, idx, rp, cols, x = case(3, 4, 2, 2, True, seed=9)
     plan = kmx_gpu.Plan(mats, idx, 2, rp, cols, index_base=1, fp32=True)
-    sp = plan.split("fp64")
-    assert not sp["structured"] and sp["remaining_pairs"] == plan.P
+    assert not plan.sweep_info()["available"]
+    assert not plan.sweep("fp64")["active"]
     y = plan.apply(torch.from_numpy(x).cuda(), prec="fp64").cpu().numpy()
     assert rel(y, oracle_y(mats, idx, 2, rp, cols, x)) <= TOL64
 
 
-def test_update_term_and_scale_reach_level_blocks(kmx_gpu):
+def test_update_term_and_scale_reach_the_general_sweep(kmx_gpu):
     import torch
 
     mats, idx, rp, cols, x = case(3, 5, 2, 3, True, seed=13)
     plan = kmx_gpu.Plan(mats, idx, 2, rp, cols, index_base=1, fp32=True, level_blocks=True)
+    assert plan.sweep_info()["available"] and not plan.sweep_info()["separable"]
     xd = torch.from_numpy(x).cuda()
     rng = np.random.default_rng(1)
     new = rng.standard_normal(mats[1][2].shape)
@@ -137,17 +90,18 @@ def test_update_term_and_scale_reach_level_blocks(kmx_gpu):
     mats2[1][2] = new
     mats2[0] = [mats[0][0] * -0.5, mats[0][1], mats[0][2]]
     yo = oracle_y(mats2, idx, 2, rp, cols, x)
-    for beta in (-1.0, 4000.0):
-        for prec, tol in (("fp64", TOL64), ("fp32", TOL32)):
-            plan.set_split(prec, beta)
-            assert plan.split(prec)["block_pairs"] > 0
-            y = plan.apply(xd, prec=prec).cpu().numpy()
-            assert rel(y, yo) <= tol, (beta, prec)
+    for prec, tol in (("fp64", TOL64), ("fp32", TOL32)):
+        plan.set_sweep(prec, True)
+        assert rel(plan.apply(xd, prec=prec).cpu().numpy(), yo) <= tol, prec
+        plan.set_sweep(prec, False)
+        assert rel(plan.apply(xd, prec=prec).cpu().numpy(), yo) <= tol, prec
+    plan.destroy()
 
 
 SWEEP_CASES = [
-    (2, 3, 2, 2, False), (2, 3, 2, 2, True), (2, 5, 3, 3, True), (2, 4, 4, 2, False),
-    (3, 4, 2, 3, True), (3, 3, 3, 3, False), (3, 5, 3, 3, True), (4, 3, 2, 4, True), (6, 3, 2, 6, True),
+    (2, 3, 2, 2, False), (2, 3, 2, 2, True), (2, 5, 3, 3, True), (2, 4, 4, 2, False), (2, 5, 1, 2, True),
+    (2, 4, 3, 3, True), (3, 4, 2, 3, True), (3, 3, 3, 3, False), (3, 5, 3, 3, True), (3, 4, 4, 2, True),
+    (4, 3, 2, 4, True), (5, 2, 2, 2, True), (6, 3, 2, 6, True),
 ]
 
 
@@ -162,7 +116,7 @@ def test_dimension_sweep_matches_oracle(kmx_gpu, nd, lev, deg, T, conn):
     yo = oracle_y(mats, idx, deg, rp, cols, x)
     plan = kmx_gpu.Plan(mats, idx, deg, rp, cols, index_base=1, fp32=True, level_blocks=True)
     info = plan.sweep("fp64")
-    assert info["available"], plan.split("fp64")
+    assert info["available"], plan.sweep_info()
     xd = torch.from_numpy(x).cuda()
     for prec, tol in (("fp64", TOL64), ("fp32", TOL32)):
         plan.set_sweep(prec, True)

@@ -4,9 +4,15 @@ operator_two_scale fixtures and the CPU restatement, the quadrature points
 against lgwt.m's, and the device-built wavelet-space matrices of the PDEs here
 against kmx.coeffs (coeff_matrix.m / pterm_coeff_matrix.m /
 sd_term_coeff_matrix.m restated on the CPU and pinned by the gold coefficient,
-transform and trajectory fixtures), within 1e-13 of the matrix's largest entry.
+transform and trajectory fixtures).
+
 The device path solves M \\ L cell by cell in the real-space basis and rotates
-once; the CPU path rotates first and solves densely: same matrix, other roundoff."""
+once, F (M \\ L) F^T; the reference (and the CPU path) rotates first and solves,
+(F M F^T) \\ (F L F^T) = F^-T (M \\ L) F^T.  The two agree exactly for an
+orthogonal F; the transform built from the two-scale filter data is orthogonal
+only to ||F F^T - I|| (1e-14 at deg 2-3, 6e-14 at deg 4, 2.3e-13 at deg 5-6), so
+the bar is 1e-13 plus four times that defect, relative to the largest entry.
+MASS g with LHS mass g is F F^T on the device and I on the CPU: the defect itself."""
 import numpy as np
 import pytest
 
@@ -68,8 +74,11 @@ def test_device_sd_term_matches_restatement(kmx_gpu, i):
     want = K.sd_term_coeff_matrix(deg, 0.0, dim, chain, params, lev)
     got = OB.sd_term_matrix(deg, 0.0, dim, chain, params, lev).cpu().numpy()
     err = float(np.max(np.abs(got - want)) / np.max(np.abs(want)))
-    print(f"{name}: deg {deg} lev {lev}: max |device - CPU| / max |CPU| = {err:.2e}")
-    assert err <= 1e-13
+    F = K.fmwt(deg, lev)
+    defect = float(np.max(np.abs(F @ F.T - np.eye(F.shape[0]))))
+    print(f"{name}: deg {deg} lev {lev}: max |device - CPU| / max |CPU| = {err:.2e} "
+          f"(transform defect {defect:.1e})")
+    assert err <= 1e-13 + 4 * defect
 
 
 @pytest.mark.gpu
