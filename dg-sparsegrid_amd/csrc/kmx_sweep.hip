@@ -375,7 +375,7 @@ struct BoxJob {
     const int* ymap;
     void* arena[2];  // FP64, FP32 arena of the plan: slot 0 x's box copy, slots 1..nslot the ops' sums
     long long nb;
-    long long koff;  // the instance's first value in the batch's RK3 stage workspace
+    long long koff;  // the instance's first box value in the batch's RK3 stage workspace
     int nslot, inst, blk0;
 };
 template <typename T>
@@ -411,14 +411,16 @@ __global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict_
 // f + dt a21 k1, stage 2 k2 and the box copy of f + dt (a31 k1 + a32 k2), stage 3
 // the new state f + dt (b1 k1 + b2 k2 + b3 k3) and (next_box) its box copy for the
 // next step; stage 0 only copies f into the box.  Absent cells of the box copy are
-// never written (zero since the plan's set-up).
+// never written (zero since the plan's set-up).  k1 and k2 are kept in box order
+// (coalesced with the slots) and in T, the precision the ops computed them in;
+// the combinations are FP64.
 constexpr double RK_A21 = 0.5, RK_A31 = -1.0, RK_A32 = 2.0;
 constexpr double RK_B1 = 1.0 / 6.0, RK_B2 = 2.0 / 3.0, RK_B3 = 1.0 / 6.0;
 struct Rk3Args {
     double* f[KMX_BATCH_MAX];
     double dt[KMX_BATCH_MAX];
-    double* k1;  // stage workspace, instance values at koff
-    double* k2;
+    void* k1;  // stage workspace (T), the instance's box values at koff
+    void* k2;
 };
 template <typename T, int STAGE>
 __global__ void __launch_bounds__(256) k_box_rk3(const BoxJob* __restrict__ wg_jobs, Rk3Args a, int next_box) {
@@ -440,16 +442,16 @@ __global__ void __launch_bounds__(256) k_box_rk3(const BoxJob* __restrict__ wg_j
         return;
     }
     const double dt = a.dt[j.inst];
-    double* k1 = a.k1 + j.koff;
-    double* k2 = a.k2 + j.koff;
+    T* k1 = static_cast<T*>(a.k1) + j.koff;
+    T* k2 = static_cast<T*>(a.k2) + j.koff;
     if (STAGE == 1) {
-        k1[m] = k;
+        k1[i] = T(k);
         box[i] = T(fm + dt * RK_A21 * k);
     } else if (STAGE == 2) {
-        k2[m] = k;
-        box[i] = T(fm + dt * (RK_A31 * k1[m] + RK_A32 * k));
+        k2[i] = T(k);
+        box[i] = T(fm + dt * (RK_A31 * double(k1[i]) + RK_A32 * k));
     } else {
-        const double fn = fm + dt * (RK_B1 * k1[m] + RK_B2 * k2[m] + RK_B3 * k);
+        const double fn = fm + dt * (RK_B1 * double(k1[i]) + RK_B2 * double(k2[i]) + RK_B3 * k);
         f[m] = fn;
         if (next_box) box[i] = T(fn);
     }
@@ -1856,8 +1858,8 @@ struct kmx_sweep_batch {
     bool has32 = true;
     bool has16 = true;     // every op separable with BF16 values (BF16 applies on the FP32 arenas)
     bool all_sep = true;   // every member separable (the fused RK3 step)
-    long long kdof = 0;    // values of the RK3 stage workspace (every member's state)
-    double* d_k = nullptr;  // RK3 stage workspace: k1 then k2, kdof each (first kmx_sweep_batch_rk3)
+    long long kdof = 0;    // values of each RK3 stage derivative (every member's box)
+    double* d_k = nullptr;  // RK3 stage workspace: k1 then k2, kdof FP64 values each (first kmx_sweep_batch_rk3)
 };
 
 kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, int wave_batches, bool transpose) {
@@ -1945,7 +1947,8 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                 // Workgroups in classes of their longest wave's steps (powers of two), longer
                 // classes first, emission order (instance by instance) inside a class: the long
                 // tiles of every instance start early without scattering each instance's tiles
-                // over the whole launch.
+                // over the whole launch.  (Dealing each op's workgroups to one XCD, balanced by
+                // work, was measured 1-7 % slower on the 64-instance batch.)
                 const size_t rw = size_t(UD_WAVES) * UD_REC;  // ints per workgroup
                 auto rec_int = [&](size_t wg, int w, int k) {  // int k of wave w's record
                     const size_t b = wg * rw + size_t(w) * UD_REC;
@@ -1992,7 +1995,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
             }
             jobs.push_back(BoxJob{sw->d_xmap, sw->d_ymap, {sw->d_arena[0], sw->d_arena[1]}, sw->nb, koff,
                                   int(sw->desc.size()), int(q), int(blk)});
-            koff += (long long)members[q]->N * members[q]->B;
+            koff += sw->nb;
             blk += (sw->nb + 255) / 256;
             if (blk >= (1LL << 31)) throw std::runtime_error("sweep batch too large");
             wgjobs.resize(size_t(blk), jobs.back());
@@ -2098,7 +2101,10 @@ int kmx_sweep_batch_rk3(kmx_sweep_batch* sb, int slot, double* const* fs, const 
         a.dt[q] = dts[q];
     }
     a.k1 = sb->d_k;
-    a.k2 = sb->d_k + sb->kdof;
+    if (slot == 0)
+        a.k2 = sb->d_k + sb->kdof;
+    else
+        a.k2 = reinterpret_cast<float*>(sb->d_k) + sb->kdof;  // FP32 stage values
     const dim3 bg(unsigned(sb->box_wgs)), bb(256);
     auto ops = [&]() {
         for (const auto& P : sb->phases) {

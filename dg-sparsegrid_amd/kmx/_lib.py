@@ -21,6 +21,13 @@ PREC_FP32 = 1
 PREC_BF16 = 3
 
 KRON_SCRATCH_ELEMS = 1 << 20
+ARNOLDI_MAX_K = 127
+
+
+def arnoldi_work(n):
+    """KMX_ARNOLDI_WORK(n): doubles of caller-owned scratch for kmx_arnoldi_step."""
+    return int(n) + 16448
+
 
 PLAN_FP32 = 1
 PLAN_ADJOINT = 2
@@ -145,6 +152,8 @@ PROTOTYPES = {
                                 C.c_void_p]),
     "kmx_kronmult": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.c_void_p,
                                C.c_int, C.c_void_p, C.c_void_p]),
+    "kmx_arnoldi_step": (C.c_int, [C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                   C.c_void_p, C.c_void_p, C.c_void_p]),
     "kmx_kronmult_noalloc": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_void_p),
                                        C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "kmx_gemm_vbatched": (C.c_int, [C.c_char, C.c_char, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),

@@ -297,6 +297,19 @@ class RealSpace:
             pass
 
 
+def arnoldi_step(n, k, V, u, s0, s1, h, work, stream=None):
+    """One Arnoldi step on device tensors (kmx_arnoldi_step): w = s0 V[k] + s1 u
+    orthogonalised against V[0..k] (classical Gram-Schmidt twice), h[0..k+1] its
+    coefficients and norm, V[k+1] = w / |w|.  V: (>= k+2, n) float64, u: n,
+    h: >= k+2, work: kmx._lib.arnoldi_work(n) float64, all CUDA tensors."""
+    if V.dim() != 2 or V.shape[1] != n or V.shape[0] < k + 2 or not V.is_contiguous():
+        raise ValueError("arnoldi_step: V must be a contiguous (>= k+2, n) tensor")
+    if u.numel() != n or h.numel() < k + 2 or work.numel() < _lib.arnoldi_work(n):
+        raise ValueError("arnoldi_step: u, h or work too small")
+    check(lib().kmx_arnoldi_step(int(n), int(k), _ptr(V), _ptr(u), float(s0), float(s1), _ptr(h), _ptr(work),
+                                 _stream(stream)))
+
+
 def kronmult(mats, X, stream=None, noalloc=False, Y=None):
     """Y = kron(A1..Ak) X on the device (kron_multd.m semantics, k = 1..16).
 

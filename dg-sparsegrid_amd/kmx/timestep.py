@@ -181,68 +181,66 @@ class RK3Graph:
 
 # ------------------------------------------------------------ implicit (BE)
 class DeviceGMRES:
-    """Restarted GMRES with the Arnoldi steps on the device, FP64 vectors.
+    """Restarted GMRES for M x = b with M v = s0 v + s1 (A v), FP64 vectors on the
+    device.
 
-    apply_into(x, y) writes y = M x (device, capturable).  One Arnoldi step --
-    the product with basis vector k, classical Gram-Schmidt against the whole
-    (restart + 1)-row basis twice (rows past k are zero, so the shapes never
-    change), the norm and the next basis vector -- is captured `chunk` times in
-    a row in one HIP graph, with k held on the device; each replay copies the
-    chunk's Hessenberg columns to pinned host memory, where the Givens rotations
-    and the residual estimate are updated (one synchronisation per chunk, no
-    per-dot round trips).  A chunk may run up to chunk - 1 steps past
-    convergence; their basis vectors are not used.  restart must be a multiple
-    of chunk."""
+    apply_into(x, y) writes y = A x (device, capturable).  Arnoldi step k is the
+    product u = A v_k and kmx_arnoldi_step (w = s0 v_k + s1 u, classical
+    Gram-Schmidt against v_0 .. v_k twice, v_{k+1} = w / |w|, the coefficients to
+    the device); the steps k0 .. k0 + chunk - 1 of each chunk of a restart cycle
+    are captured in one HIP graph of their own (k fixed in each, so nothing is
+    indexed on the device), whose replay ends by copying the chunk's Hessenberg
+    columns to pinned host memory.  The host applies the Givens rotations and
+    checks the residual estimate once per chunk; a chunk may run up to chunk - 1
+    steps past convergence, whose basis vectors are not used.  restart must be a
+    multiple of chunk and at most KMX_ARNOLDI_MAX_K."""
 
-    def __init__(self, apply_into, n, restart=40, device="cuda", chunk=4):
+    def __init__(self, apply_into, n, restart=40, device="cuda", chunk=4, shift=(0.0, 1.0)):
         torch = _torch()
-        if restart % chunk:
-            raise ValueError("restart must be a multiple of chunk")
+        from . import _lib
+
+        if restart % chunk or restart > _lib.ARNOLDI_MAX_K:
+            raise ValueError("restart must be a multiple of chunk and at most KMX_ARNOLDI_MAX_K")
         self.apply_into, self.n, self.m, self.chunk = apply_into, int(n), int(restart), int(chunk)
+        self.s0, self.s1 = float(shift[0]), float(shift[1])
         m = self.m
         self.V = torch.zeros((m + 1, self.n), dtype=torch.float64, device=device)
-        self.w = torch.empty(self.n, dtype=torch.float64, device=device)
-        self.xin = torch.empty(self.n, dtype=torch.float64, device=device)
-        self.k = torch.zeros(1, dtype=torch.int64, device=device)
-        self.hdev = torch.zeros((self.chunk, m + 2), dtype=torch.float64, device=device)
-        self.hhost = torch.zeros((self.chunk, m + 2), dtype=torch.float64).pin_memory()
-        self.graph = None
+        self.u = torch.empty(self.n, dtype=torch.float64, device=device)
+        self.Mx = torch.empty(self.n, dtype=torch.float64, device=device)
+        self.work = torch.empty(_lib.arnoldi_work(self.n), dtype=torch.float64, device=device)
+        self.hdev = torch.zeros((m, m + 2), dtype=torch.float64, device=device)
+        self.hhost = torch.zeros((m, m + 2), dtype=torch.float64).pin_memory()
+        self.graphs = [None] * (m // self.chunk)
 
-    def _step(self, c):
+    def _chunk(self, c0):
+        from .plan import arnoldi_step
+
+        for k in range(c0, c0 + self.chunk):
+            self.apply_into(self.V[k], self.u)
+            arnoldi_step(self.n, k, self.V, self.u, self.s0, self.s1, self.hdev[k], self.work)
+        self.hhost[c0:c0 + self.chunk].copy_(self.hdev[c0:c0 + self.chunk], non_blocking=True)
+
+    def _graph(self, ci):
         torch = _torch()
-        V, w = self.V, self.w
-        self.xin.copy_(V.index_select(0, self.k).view(-1))
-        self.apply_into(self.xin, w)
-        h = torch.mv(V, w)
-        w.sub_(torch.mv(V.t(), h))
-        h2 = torch.mv(V, w)
-        w.sub_(torch.mv(V.t(), h2))
-        h.add_(h2)
-        nrm = torch.sqrt(torch.mv(w.view(1, -1), w))  # a one-row product: no reduction workspace
-        V.index_copy_(0, self.k + 1, (w / nrm).view(1, -1))
-        self.hdev[c, : self.m + 1].copy_(h)
-        self.hdev[c, self.m + 1:].copy_(nrm)
-        self.k.add_(1)
+        if self.graphs[ci] is None:
+            c0 = ci * self.chunk
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                self._chunk(c0)  # warm the kernels outside the capture (the same steps the replay repeats)
+            torch.cuda.current_stream().wait_stream(side)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._chunk(c0)
+            torch.cuda.synchronize()
+            self.graphs[ci] = g
+        return self.graphs[ci]
 
-    def _chunk(self):
-        for c in range(self.chunk):
-            self._step(c)
-        self.hhost.copy_(self.hdev, non_blocking=True)
-
-    def _capture(self):
-        torch = _torch()
-        side = torch.cuda.Stream()
-        side.wait_stream(torch.cuda.current_stream())
-        saved = self.V.clone(), self.k.clone()
-        with torch.cuda.stream(side):
-            self._chunk()  # warm the kernels outside the capture
-        torch.cuda.current_stream().wait_stream(side)
-        self.V.copy_(saved[0])
-        self.k.copy_(saved[1])
-        self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            self._chunk()
-        torch.cuda.synchronize()
+    def matvec(self, x, out):
+        """out = M x."""
+        self.apply_into(x, out)
+        out.mul_(self.s1).add_(x, alpha=self.s0)
+        return out
 
     def solve(self, b, tol=1e-6, maxiter=400):
         """x with ||b - M x|| <= tol ||b|| (estimate), from x = 0.  Returns
@@ -253,33 +251,28 @@ class DeviceGMRES:
         bnorm = float(torch.linalg.vector_norm(b))
         if bnorm == 0.0:
             return x, 0.0, 0
-        if self.graph is None:
-            self._capture()
         it, rel, r = 0, 1.0, b.clone()
         while it < maxiter:
             beta = float(torch.linalg.vector_norm(r))
             rel = beta / bnorm
             if rel <= tol:
                 break
-            self.V.zero_()
             self.V[0].copy_(r / beta)
-            self.k.zero_()
             H = np.zeros((m + 1, m))
             cs, sn = np.zeros(m), np.zeros(m)
             g = np.zeros(m + 1)
             g[0] = beta
             k_used, done = 0, False
-            for k0 in range(0, m, self.chunk):
+            for ci in range(m // self.chunk):
                 if done or it >= maxiter:
                     break
-                self.graph.replay()
+                self._graph(ci).replay()
                 torch.cuda.current_stream().synchronize()
                 hv_all = self.hhost.numpy()
                 for c in range(self.chunk):
-                    k = k0 + c
-                    hv = hv_all[c]
-                    H[: k + 1, k] = hv[: k + 1]
-                    H[k + 1, k] = hv[m + 1]
+                    k = ci * self.chunk + c
+                    hv = hv_all[k]
+                    H[: k + 2, k] = hv[: k + 2]
                     for j in range(k):
                         tmp = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
                         H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
@@ -292,13 +285,12 @@ class DeviceGMRES:
                     g[k] = cs[k] * g[k]
                     it += 1
                     k_used = k + 1
-                    if abs(g[k + 1]) / bnorm <= tol or hv[m + 1] == 0.0 or it >= maxiter:
+                    if abs(g[k + 1]) / bnorm <= tol or hv[k + 1] == 0.0 or it >= maxiter:
                         done = True
                         break
             yk = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used])
             x += torch.from_numpy(yk).to(b.device) @ self.V[:k_used]
-            self.apply_into(x, self.w)
-            r = b - self.w
+            r = b - self.matvec(x, self.Mx)
         return x, rel, it
 
 
@@ -309,14 +301,14 @@ def gmres(matvec, b, x0=None, tol=1e-6, restart=40, maxiter=400):
     torch = _torch()
     x0 = torch.zeros_like(b) if x0 is None else x0
     r0 = b - matvec(x0)
-    solver = DeviceGMRES(lambda x, y: y.copy_(matvec(x)), b.numel(), restart, b.device)
+    solver = DeviceGMRES(lambda x, y: y.copy_(matvec(x)), b.numel(), restart, b.device)  # M = A
     d, _, it = solver.solve(r0, tol=tol * float(torch.linalg.vector_norm(b)) / max(
         float(torch.linalg.vector_norm(r0)), 1e-300), maxiter=maxiter)
     x = x0 + d
     return x, float(torch.linalg.vector_norm(b - matvec(x))) / max(float(torch.linalg.vector_norm(b)), 1e-300), it
 
 
-_SOLVERS = weakref.WeakKeyDictionary()  # plan -> {(dt, inner precision, restart, n): (DeviceGMRES, its buffer)}
+_SOLVERS = weakref.WeakKeyDictionary()  # plan -> {(dt, inner precision, restart, n): DeviceGMRES}
 
 
 def backward_euler(plan, f0, dt, rhs=None, tol=1e-10, inner_prec="fp32", inner_tol=1e-5, restart=40,
@@ -325,29 +317,24 @@ def backward_euler(plan, f0, dt, rhs=None, tol=1e-10, inner_prec="fp32", inner_t
 
     Iterative refinement: the FP64 residual r = b - (I - dt A) f1 (plain FP64
     apply) drives a correction solved by GMRES with `inner_prec` applies to
-    relative accuracy inner_tol, until ||r|| / ||b|| <= tol.
+    relative accuracy inner_tol, until ||r|| / ||b|| <= tol.  The GMRES solver
+    (its captured Arnoldi graphs) is kept per plan and step size for later steps.
     Returns (f1, history of FP64 relative residuals, inner iterations)."""
-    b = f0.clone() if rhs is None else f0 + dt * rhs
-    bnorm = float(_torch().linalg.vector_norm(b))
-
     torch = _torch()
+    b = f0.clone() if rhs is None else f0 + dt * rhs
+    bnorm = float(torch.linalg.vector_norm(b))
 
     def op64(v):
         return v - dt * plan.apply(v, prec="fp64")
 
-    tmp = torch.empty_like(f0)
-
-    def op_in_into(x, y):  # y = (I - dt A) x with inner_prec applies, capturable
-        plan.apply(x, tmp, prec=inner_prec)
-        torch.add(x, tmp, alpha=-dt, out=y)
-
     key = (float(dt), inner_prec, int(restart), f0.numel())
     cache = _SOLVERS.setdefault(plan, {})
-    if key not in cache:  # the captured Arnoldi graph is reused by later steps with the same dt
-        cache[key] = (DeviceGMRES(op_in_into, f0.numel(), restart, f0.device), tmp)
-    solver, tmp_cached = cache[key]
-    if tmp_cached is not tmp:
-        tmp = tmp_cached  # the captured graph writes the cached buffer
+    solver = cache.get(key)
+    if solver is None:
+        pref = weakref.ref(plan)  # the cache must not keep its own key alive
+        solver = DeviceGMRES(lambda x, y: pref().apply(x, y, prec=inner_prec), f0.numel(), restart, f0.device,
+                             shift=(1.0, -float(dt)))
+        cache[key] = solver
     f = f0.clone()  # f0 is a good first guess for small dt
     hist, inner = [], 0
     for _ in range(max_refine):

@@ -211,9 +211,25 @@ int kmx_batch_get_sweep(const kmx_batch* batch, int prec, int* members, int* act
  * FP64, advanced in place; dts[i]: its step (host array).  Needs the batch's merged
  * sweep on for prec (kmx_batch_set_sweep) with every plan a separable member (else
  * KMX_E_UNSUPPORTED).  The first call allocates the stage workspace (2 x the
- * batch's values, FP64); later calls allocate nothing.  Asynchronous on `stream`.
- * No counterpart in the reference (its time steps call apply_A three times). */
+ * values of the members' boxes, FP64); later calls allocate nothing.  Asynchronous
+ * on `stream`.  No counterpart in the reference (its time steps call apply_A three times). */
 int kmx_batch_rk3(kmx_batch* batch, double* const* fs, const double* dts, int nsteps, int prec, void* stream);
+
+/* One Arnoldi step of a restarted GMRES on the device, for the matrix-free
+ * backward-Euler solve that replaces time_advance.m:328-349's direct solve
+ * ((I - dt A) f1 = f0 + dt (s + bc)).  V: DEVICE, rows 0..k+1 of n FP64 values
+ * (row j at V + j*n), rows 0..k orthonormal; u: DEVICE, the caller's product
+ * A v_k.  Forms w = s0 v_k + s1 u (s0 = 1, s1 = -dt for I - dt A), orthogonalises
+ * it against rows 0..k by classical Gram-Schmidt twice, writes h[0..k] (the
+ * summed coefficients) and h[k+1] = |w| (DEVICE, k+2 values), and row k+1 of V =
+ * w / |w| (zero when |w| = 0: breakdown).  work: DEVICE scratch of
+ * KMX_ARNOLDI_WORK(n) doubles, owned by the caller.  Allocates nothing; four
+ * launches of fixed shape for given (n, k), asynchronous on `stream` (graph
+ * capturable).  0 <= k <= KMX_ARNOLDI_MAX_K. */
+#define KMX_ARNOLDI_MAX_K 127
+#define KMX_ARNOLDI_WORK(n) ((n) + 16448)
+int kmx_arnoldi_step(int64_t n, int k, double* V, const double* u, double s0, double s1, double* h, double* work,
+                     void* stream);
 
 /* Y = kron(A1,...,Ak) X for k = 1..KMX_KRON_MAX_FACTORS (kron_multd.m:30-42,
  * Kronmult/kronmult{k}.m; k > 6 as kron_multd.m:43-86's recursion).

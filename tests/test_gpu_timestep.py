@@ -180,3 +180,85 @@ def test_rk3_dirichlet_on_gpu_matches_gold(kmx_gpu, gold, name, pfn, lev, deg):
         for got in (f.cpu().numpy(), fg.cpu().numpy()):
             assert rel(got, g) <= 1e-12, (i, rel(got, g))
             assert rel(got, fo) <= 1e-12, i
+
+
+@pytest.mark.parametrize("n,k", [(7, 0), (300, 3), (32064, 17), (100003, 39), (5000, 127)])
+def test_arnoldi_step_matches_two_pass_gram_schmidt(kmx_gpu, n, k):
+    """kmx_arnoldi_step against the same step in numpy: w = s0 v_k + s1 u, two
+    classical Gram-Schmidt passes against v_0..v_k, h = both passes' coefficients,
+    v_{k+1} = w / |w|; h within 1e-13 of |w|, v_{k+1} within 1e-13, and v_{k+1}
+    orthogonal to v_0..v_k to 1e-14.  A u in the span of v_0..v_k (s1 = 1, s0 = 0)
+    breaks down: |w| ~ 0."""
+    import torch
+
+    from kmx import _lib
+    from kmx.plan import arnoldi_step
+
+    rng = np.random.default_rng(n + k)
+    q, _ = np.linalg.qr(rng.standard_normal((n, k + 1)))
+    V = np.zeros((k + 2, n))
+    V[: k + 1] = q.T
+    u = rng.standard_normal(n)
+    s0, s1 = 1.0, -0.37
+    w = s0 * V[k] + s1 * u
+    h1 = V[: k + 1] @ w
+    w = w - V[: k + 1].T @ h1
+    h2 = V[: k + 1] @ w
+    w = w - V[: k + 1].T @ h2
+    nrm = np.linalg.norm(w)
+    Vd = torch.from_numpy(V).cuda()
+    hd = torch.zeros(k + 2, dtype=torch.float64, device="cuda")
+    work = torch.empty(_lib.arnoldi_work(n), dtype=torch.float64, device="cuda")
+    arnoldi_step(n, k, Vd, torch.from_numpy(u).cuda(), s0, s1, hd, work)
+    h = hd.cpu().numpy()
+    Vn = Vd.cpu().numpy()
+    assert np.max(np.abs(h[: k + 1] - (h1 + h2))) <= 1e-13 * max(1.0, np.abs(h1).max())
+    assert abs(h[k + 1] - nrm) <= 1e-13 * nrm
+    assert np.max(np.abs(Vn[k + 1] - w / nrm)) <= 1e-13
+    assert np.max(np.abs(Vn[: k + 1] @ Vn[k + 1])) <= 1e-14 * np.sqrt(n)
+    assert np.array_equal(Vn[: k + 1], V[: k + 1])  # the basis rows are left alone
+    # breakdown: u in the span of the basis
+    u2 = V[: k + 1].T @ rng.standard_normal(k + 1)
+    arnoldi_step(n, k, Vd, torch.from_numpy(u2).cuda(), 0.0, 1.0, hd, work)
+    h = hd.cpu().numpy()
+    assert h[k + 1] <= 1e-13 * np.linalg.norm(u2)
+
+
+def test_arnoldi_step_rejects_bad_arguments(kmx_gpu):
+    import torch
+
+    from kmx import _lib
+
+    V = torch.zeros((3, 10), dtype=torch.float64, device="cuda")
+    u = torch.zeros(10, dtype=torch.float64, device="cuda")
+    h = torch.zeros(3, dtype=torch.float64, device="cuda")
+    work = torch.empty(_lib.arnoldi_work(10), dtype=torch.float64, device="cuda")
+    from kmx.plan import arnoldi_step
+
+    with pytest.raises(ValueError):
+        arnoldi_step(10, 2, V, u, 1.0, 1.0, h, work)  # V has k+1 rows, not k+2
+    with pytest.raises(kmx_gpu.KmxError):
+        arnoldi_step(10, _lib.ARNOLDI_MAX_K + 1, torch.zeros((_lib.ARNOLDI_MAX_K + 3, 10), dtype=torch.float64,
+                                                             device="cuda"), u,
+                     1.0, 1.0, torch.zeros(_lib.ARNOLDI_MAX_K + 3, dtype=torch.float64, device="cuda"), work)
+
+
+def test_backward_euler_second_call_reuses_the_solver(kmx_gpu):
+    """The GMRES solver is kept per plan and step size: a second BE step with the
+    same dt reuses its graphs and gives the same FP64-refined result."""
+    import torch
+
+    from kmx import timestep as TS
+
+    pde = K.continuity2(4)
+    mats = K.get_coeff_mats(pde, 3)
+    tab = G.hash_table_sparse_nd([4, 4], 4, "SG")
+    plan = kmx_gpu.Plan(mats, tab.idx1d, 3, None, None, index_base=1, fp32=True)
+    f0 = torch.from_numpy(np.random.default_rng(1).standard_normal(plan.dof)).cuda()
+    dt = 50 * pde.set_dt(pde, 0.01)
+    f1, hist1, it1 = TS.backward_euler(plan, f0, dt, tol=1e-10)
+    assert len(TS._SOLVERS[plan]) == 1
+    f2, hist2, it2 = TS.backward_euler(plan, f0, dt, tol=1e-10)
+    assert len(TS._SOLVERS[plan]) == 1
+    assert hist1[-1] <= 1e-10 and hist2[-1] <= 1e-10
+    assert torch.equal(f1, f2) and it1 == it2
