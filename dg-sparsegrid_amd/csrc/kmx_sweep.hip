@@ -145,7 +145,11 @@ enum { R_RUN0 = R_LA, R_NRUN = R_LB, R_NCT = R_TLO };
 // tiles of its group with the panel's A values held in registers
 constexpr int SEP_KMAX = 16;
 constexpr int UD_MAXLEV = UD_REC - UD_BOX - 1;  // highest level a record holds a box for
-constexpr int UD_WAVES = 8;                     // waves per workgroup
+// Waves per workgroup.  Two: a workgroup holds its slots until its slowest wave
+// ends (the split tiles' partial sums meet at a barrier), so small workgroups keep
+// the CUs full; on the 64-instance batch an FP32 RK3 step took 0.372 ms with eight
+// waves, 0.319 with four, 0.305 with two and 0.336 with one (no tile split).
+constexpr int UD_WAVES = 2;
 
 // Slot b of term t of an intermediate starts at base + (b * T + t) * nb; slot 0
 // of an input with x0 set is x itself, read through the box map.  part: which

@@ -258,10 +258,9 @@ class DeviceGMRES:
             if rel <= tol:
                 break
             self.V[0].copy_(r / beta)
-            H = np.zeros((m + 1, m))
-            cs, sn = np.zeros(m), np.zeros(m)
-            g = np.zeros(m + 1)
-            g[0] = beta
+            R = np.zeros((m, m))  # the rotated Hessenberg columns (upper triangle)
+            cs, sn = [], []
+            g = [beta]
             k_used, done = 0, False
             for ci in range(m // self.chunk):
                 if done or it >= maxiter:
@@ -271,24 +270,27 @@ class DeviceGMRES:
                 hv_all = self.hhost.numpy()
                 for c in range(self.chunk):
                     k = ci * self.chunk + c
-                    hv = hv_all[k]
-                    H[: k + 2, k] = hv[: k + 2]
+                    col = hv_all[k, : k + 2].tolist()  # plain floats: the rotations run in Python
+                    breakdown = col[k + 1] == 0.0
                     for j in range(k):
-                        tmp = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
-                        H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
-                        H[j, k] = tmp
-                    den = math.hypot(H[k, k], H[k + 1, k])
-                    cs[k], sn[k] = (1.0, 0.0) if den == 0 else (H[k, k] / den, H[k + 1, k] / den)
-                    H[k, k] = cs[k] * H[k, k] + sn[k] * H[k + 1, k]
-                    H[k + 1, k] = 0.0
-                    g[k + 1] = -sn[k] * g[k]
-                    g[k] = cs[k] * g[k]
+                        a, b, cj, sj = col[j], col[j + 1], cs[j], sn[j]
+                        col[j] = cj * a + sj * b
+                        col[j + 1] = cj * b - sj * a
+                    a, b = col[k], col[k + 1]
+                    den = math.hypot(a, b)
+                    ck, sk = (1.0, 0.0) if den == 0 else (a / den, b / den)
+                    cs.append(ck)
+                    sn.append(sk)
+                    col[k] = ck * a + sk * b
+                    R[: k + 1, k] = col[: k + 1]
+                    g.append(-sk * g[k])
+                    g[k] = ck * g[k]
                     it += 1
                     k_used = k + 1
-                    if abs(g[k + 1]) / bnorm <= tol or hv[k + 1] == 0.0 or it >= maxiter:
+                    if abs(g[k + 1]) / bnorm <= tol or breakdown or it >= maxiter:
                         done = True
                         break
-            yk = np.linalg.solve(np.triu(H[:k_used, :k_used]), g[:k_used])
+            yk = np.linalg.solve(np.triu(R[:k_used, :k_used]), np.asarray(g[:k_used]))
             x += torch.from_numpy(yk).to(b.device) @ self.V[:k_used]
             r = b - self.matvec(x, self.Mx)
         return x, rel, it
