@@ -114,6 +114,28 @@ struct SepMma<float, bf16_bits> {
     }
 };
 
+// Loads and stores through global-address-space pointers: every buffer of these
+// kernels is device memory, and generic pointers would compile to flat accesses
+// (which also occupy the LDS counter and check the LDS aperture).
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+    *(__attribute__((address_space(1))) T*)p = v;
+}
+__device__ __forceinline__ int2 gld(const int2* p) {
+    typedef int v2 __attribute__((ext_vector_type(2)));
+    const v2 v = *(const __attribute__((address_space(1))) v2*)p;
+    return make_int2(v.x, v.y);
+}
+__device__ __forceinline__ int4 gld(const int4* p) {
+    typedef int v4 __attribute__((ext_vector_type(4)));
+    const v4 v = *(const __attribute__((address_space(1))) v4*)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
 // positions of level l along a dimension: deg * cells(l)
 __device__ __forceinline__ int deg_cells(int deg, int l) { return deg * (l ? 1 << (l - 1) : 1); }
 
@@ -143,7 +165,10 @@ enum {
 enum { R_RUN0 = R_LA, R_NRUN = R_LB, R_NCT = R_TLO };
 // a separable tile whose panel has at most SEP_KMAX steps may sweep R_NCT column
 // tiles of its group with the panel's A values held in registers
-constexpr int SEP_KMAX = 16;
+// (8: the packed body's registers set the launch's occupancy -- 92 VGPRs at 16
+// steps, 68 at 8; an FP32 RK3 step of the 64-instance batch took 0.297 ms at 16,
+// 0.288 at 12, 0.279 at 8 and 4)
+constexpr int SEP_KMAX = 8;
 constexpr int UD_MAXLEV = UD_REC - UD_BOX - 1;  // highest level a record holds a box for
 // Waves per workgroup.  Two: a workgroup holds its slots until its slowest wave
 // ends (the split tiles' partial sums meet at a barrier), so small workgroups keep
@@ -387,9 +412,9 @@ __global__ void __launch_bounds__(256) k_box_gather_batch(const BoxJob* __restri
     const BoxJob& j = wg_jobs[blockIdx.x];
     const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
     if (i < j.nb) {
-        const int m = j.xmap[i];
+        const int m = gld(j.xmap + i);
         T* xb = static_cast<T*>(j.arena[sizeof(T) == 8 ? 0 : 1]);
-        xb[i] = m >= 0 ? T(io.x[j.inst][m]) : T(0);
+        gst(xb + i, m >= 0 ? T(gld(io.x[j.inst] + m)) : T(0));
     }
 }
 // y = the ops' slots summed (in op order), at the present cells
@@ -401,9 +426,9 @@ __global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict_
     if (i < j.nb) {
         const T* sl = static_cast<const T*>(j.arena[sizeof(T) == 8 ? 0 : 1]) + j.nb + i;
         double v = 0.0;
-        for (int k = 0; k < j.nslot; ++k) v += double(sl[(long long)k * j.nb]);
-        const int m = j.ymap[i];
-        if (m >= 0) io.y[j.inst][m] = v;
+        for (int k = 0; k < j.nslot; ++k) v += double(gld(sl + (long long)k * j.nb));
+        const int m = gld(j.ymap + i);
+        if (m >= 0) gst(io.y[j.inst] + m, v);
     }
 }
 
@@ -435,29 +460,29 @@ __global__ void __launch_bounds__(256) k_box_rk3(const BoxJob* __restrict__ wg_j
     double k = 0.0;
     if (STAGE > 0) {  // the slots first, so their loads and the map's are in flight together
         const T* sl = box + j.nb + i;
-        for (int q = 0; q < j.nslot; ++q) k += double(sl[(long long)q * j.nb]);
+        for (int q = 0; q < j.nslot; ++q) k += double(gld(sl + (long long)q * j.nb));
     }
-    const int m = j.ymap[i];
+    const int m = gld(j.ymap + i);
     if (m < 0) return;
     double* f = a.f[j.inst];
-    const double fm = f[m];
+    const double fm = gld(f + m);
     if (STAGE == 0) {
-        box[i] = T(fm);
+        gst(box + i, T(fm));
         return;
     }
     const double dt = a.dt[j.inst];
     T* k1 = static_cast<T*>(a.k1) + j.koff;
     T* k2 = static_cast<T*>(a.k2) + j.koff;
     if (STAGE == 1) {
-        k1[i] = T(k);
-        box[i] = T(fm + dt * RK_A21 * k);
+        gst(k1 + i, T(k));
+        gst(box + i, T(fm + dt * RK_A21 * k));
     } else if (STAGE == 2) {
-        k2[i] = T(k);
-        box[i] = T(fm + dt * (RK_A31 * double(k1[i]) + RK_A32 * k));
+        gst(k2 + i, T(k));
+        gst(box + i, T(fm + dt * (RK_A31 * double(gld(k1 + i)) + RK_A32 * k)));
     } else {
-        const double fn = fm + dt * (RK_B1 * double(k1[i]) + RK_B2 * double(k2[i]) + RK_B3 * k);
-        f[m] = fn;
-        if (next_box) box[i] = T(fn);
+        const double fn = fm + dt * (RK_B1 * double(gld(k1 + i)) + RK_B2 * double(gld(k2 + i)) + RK_B3 * k);
+        gst(f + m, fn);
+        if (next_box) gst(box + i, T(fn));
     }
 }
 
@@ -605,12 +630,12 @@ __device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int4*
             const T* bsrc = arena + op.in_base;
             // the tile's runs, one per lane (64 at a time)
             int c0 = run0;
-            int4 myr = runs[c0 + min(lane, rend - 1 - c0)];
+            int4 myr = gld(runs + c0 + min(lane, rend - 1 - c0));
             int rr = run0, nst = 0, npos = 0, a = 0, b = 0;
             auto load_run = [&]() {
                 if (rr - c0 >= 64) {
                     c0 = rr;
-                    myr = runs[c0 + min(lane, rend - 1 - c0)];
+                    myr = gld(runs + c0 + min(lane, rend - 1 - c0));
                 }
                 const int k = rr - c0;
                 const int l = __builtin_amdgcn_readlane(myr.x, k);
@@ -650,9 +675,9 @@ __device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int4*
                 TA av[NB];
                 T bv[NB];
 #pragma unroll
-                for (int c = 0; c < NB; ++c) av[c] = amat[ao[c]];
+                for (int c = 0; c < NB; ++c) av[c] = gld(amat + ao[c]);
 #pragma unroll
-                for (int c = 0; c < NB; ++c) bv[c] = bsrc[bo[c]];
+                for (int c = 0; c < NB; ++c) bv[c] = gld(bsrc + bo[c]);
 #pragma unroll
                 for (int c = 0; c < NB; ++c) {
                     av[c] = pa[c] ? av[c] : TA(0);
@@ -674,7 +699,7 @@ __device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int4*
     T* out = arena + op.out_base;
 #pragma unroll
     for (int qq = 0; qq < 4; ++qq)
-        if (ook[qq]) out[oaddr[qq]] = acc[qq];
+        if (ook[qq]) gst(out + oaddr[qq], acc[qq]);
 }
 
 // One wave of a separable op over R_NCT consecutive 16-column tiles of one group
@@ -705,7 +730,7 @@ __device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const 
     const int run0 = __builtin_amdgcn_readlane(rv.x, R_RUN0), nrun = __builtin_amdgcn_readlane(rv.x, R_NRUN);
     const TA* amat = mat + op.a_off[0] + lane;
     const T* bsrc = arena + op.in_base;
-    const int4 myr = runs[run0 + min(lane, nrun - 1)];  // nrun <= SEP_KMAX
+    const int4 myr = gld(runs + run0 + min(lane, nrun - 1));  // nrun <= SEP_KMAX
     TA av[SEP_KMAX];
     int bp[SEP_KMAX], nli[SEP_KMAX];
     bool pb[SEP_KMAX];
@@ -725,7 +750,7 @@ __device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const 
         for (int c = 0; c < SEP_KMAX; ++c) {
             const bool on = c < ns;
             pb[c] = on && 4 * sin + lk < npos;
-            av[c] = on ? amat[a + 64 * sin] : TA(0);
+            av[c] = on ? gld(amat + a + 64 * sin) : TA(0);
             bp[c] = pb[c] ? base + 4 * sin * I : 0;
             nli[c] = nl * I;
             if (on && ++sin == nst && c + 1 < ns) {
@@ -759,7 +784,7 @@ __device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const 
                     for (int j = 0; j < 4; ++j) {
                         const bool ok = pb[c0 + j] && fok[u];
                         const int ad = ok ? bp[c0 + j] + fo[u] * nli[c0 + j] + fi[u] : 0;
-                        const T v = bsrc[ad];
+                        const T v = gld(bsrc + ad);
                         bv[u][j] = ok ? v : T(0);
                     }
                 acc[0] = SepMma<T, TA>::run4(av + c0, bv[0], acc[0]);
@@ -770,14 +795,17 @@ __device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const 
         for (int u = 0; u < 2; ++u)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (fok[u] && rok[q]) out[rb[q] + fo[u] * rn[q] + fi[u]] = acc[u][q];
+                if (fok[u] && rok[q]) gst(out + rb[q] + fo[u] * rn[q] + fi[u], acc[u][q]);
     }
     __syncthreads();  // the one barrier every wave of the workgroup takes (sep_tile's partial sums)
 }
 
 // A phase of separable ops only (every op UD_SEP): the separable tile bodies
 // alone, so the launch keeps the occupancy their registers allow.  TA = T, or
-// BF16 values on the FP32 arena (T = float).
+// BF16 values on the FP32 arena (T = float).  (The packed body's registers set the
+// launch's occupancy; running the packed workgroups as a launch of their own, so
+// that the plain one reaches 8 waves per SIMD, measured 20 % slower on the
+// 64-instance batch: the packed launch's tail leaves the device idle.)
 template <typename T, typename TA>
 __global__ void __launch_bounds__(UD_WAVES * 64) k_sep_batch(const UdBatchOp* __restrict__ ops,
                                                               const int* __restrict__ wg_op,
@@ -785,7 +813,7 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_sep_batch(const UdBatchOp* __
     __shared__ T part[(UD_WAVES - 1) * 4 * 64];
     const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
-    const int2 rv = recs[((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane];
+    const int2 rv = gld(recs + ((long long)blockIdx.x * UD_WAVES + w) * (UD_REC / 2) + lane);
     const UdBatchOp& o = ops[wg_op[blockIdx.x]];
     constexpr int sl = sizeof(T) == 8 ? 0 : 1;            // arena
     constexpr int ml = sizeof(TA) == 2 ? 2 : sl;          // values
@@ -1097,7 +1125,7 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
             a_lo = sd.a_base[size_t(sep_k)];
             a_hi = size_t(sep_k) + 1 < sd.a_base.size() ? sd.a_base[size_t(sep_k) + 1] : sd.a_elems;
         }
-        for (int wg = op.wg0; wg < op.wg1; ++wg)
+        for (int wg = op.wg0; wg < op.wg1; ++wg) {
             for (int w = 0; w < UD_WAVES; ++w) {
                 const std::vector<int>& r = recs[size_t(wg) * UD_WAVES + w];
                 if (r[R_GSIZE] == 0) continue;
@@ -1186,6 +1214,7 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
                     }
                 }
             }
+        }
         if (sep) ++sep_k;
     }
 }
@@ -1305,10 +1334,6 @@ void sep_runs(const kmx_plan* p, kmx_sweep& sw, int s) {
     sd.a_elems = abase;
 }
 
-// The combined values (terms summed in order, scales included) of op `only` (or
-// every op) of orientation `s` into its compact storage, FP64 and FP32, and the run
-// table; allocates the device buffers on first use.  Waits for the device first
-// (queued applies may read them).
 // FP64 -> BF16 bit pattern, rounded once to nearest even at 8 significant bits
 // (no detour through FP32, whose own rounding could move a tie).
 unsigned short host_bf16(double v) {
@@ -1323,6 +1348,10 @@ unsigned short host_bf16(double v) {
     return (unsigned short)(u >> 16);
 }
 
+// The combined values (terms summed in order, scales included) of op `only` (or
+// every op) of orientation `s` into its compact storage (FP64, FP32 and BF16) and
+// the run table; allocates the device buffers on first use.  Waits for the device first
+// (queued applies may read them).
 void sep_values(const kmx_plan* p, kmx_sweep& sw, int s, int only = -1) {
     SepSide& sd = sw.side[s];
     const int deg = p->deg;
@@ -2023,6 +2052,16 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
     return sb;
 }
 
+// A separable phase: one launch of k_sep_batch over its workgroups.
+static void launch_sep_phase(const kmx_sweep_batch* sb, const kmx_sweep_batch::Phase& P, int slot, hipStream_t s) {
+    const int* wgop = sb->d_wgop + P.wg0;
+    const int2* rc = reinterpret_cast<const int2*>(sb->d_recs) + size_t(P.wg0) * UD_WAVES * (UD_REC / 2);
+    const dim3 grid(static_cast<unsigned>(P.nwg)), block(UD_WAVES * 64);
+    if (slot == 0) hipLaunchKernelGGL((k_sep_batch<double, double>), grid, block, 0, s, sb->d_ops, wgop, rc);
+    else if (slot == 1) hipLaunchKernelGGL((k_sep_batch<float, float>), grid, block, 0, s, sb->d_ops, wgop, rc);
+    else hipLaunchKernelGGL((k_sep_batch<float, bf16_bits>), grid, block, 0, s, sb->d_ops, wgop, rc);
+}
+
 int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io, hipStream_t s) {
     for (const kmx_plan* p : sb->members)
         if (p->sw && p->sw->stale) {
@@ -2049,9 +2088,7 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
         const int2* rc = reinterpret_cast<const int2*>(sb->d_recs) + size_t(P.wg0) * UD_WAVES * (UD_REC / 2);
         const dim3 grid(unsigned(P.nwg)), block(UD_WAVES * 64);
         if (P.sep_only) {
-            if (slot == 0) hipLaunchKernelGGL((k_sep_batch<double, double>), grid, block, 0, s, sb->d_ops, wgop, rc);
-            else if (slot == 1) hipLaunchKernelGGL((k_sep_batch<float, float>), grid, block, 0, s, sb->d_ops, wgop, rc);
-            else hipLaunchKernelGGL((k_sep_batch<float, bf16_bits>), grid, block, 0, s, sb->d_ops, wgop, rc);
+            launch_sep_phase(sb, P, slot, s);
         } else if (slot == 0) {
             if (P.maxnbr <= 2) hipLaunchKernelGGL((k_ud_batch<double, 2>), grid, block, 0, s, sb->d_ops, wgop, rc, io);
             else hipLaunchKernelGGL((k_ud_batch<double, 4>), grid, block, 0, s, sb->d_ops, wgop, rc, io);
@@ -2113,12 +2150,7 @@ int kmx_sweep_batch_rk3(kmx_sweep_batch* sb, int slot, double* const* fs, const 
     auto ops = [&]() {
         for (const auto& P : sb->phases) {
             if (P.nwg == 0) continue;
-            const int* wgop = sb->d_wgop + P.wg0;
-            const int2* rc = reinterpret_cast<const int2*>(sb->d_recs) + size_t(P.wg0) * UD_WAVES * (UD_REC / 2);
-            const dim3 grid(unsigned(P.nwg)), block(UD_WAVES * 64);
-            if (slot == 0) hipLaunchKernelGGL((k_sep_batch<double, double>), grid, block, 0, s, sb->d_ops, wgop, rc);
-            else if (slot == 1) hipLaunchKernelGGL((k_sep_batch<float, float>), grid, block, 0, s, sb->d_ops, wgop, rc);
-            else hipLaunchKernelGGL((k_sep_batch<float, bf16_bits>), grid, block, 0, s, sb->d_ops, wgop, rc);
+            launch_sep_phase(sb, P, slot, s);
         }
     };
     auto stage = [&](auto kern, int next) { hipLaunchKernelGGL(kern, bg, bb, 0, s, sb->d_wgjobs, a, next); };

@@ -180,6 +180,44 @@ class RK3Graph:
 
 
 # ------------------------------------------------------------ implicit (BE)
+class HessenbergLSQ:
+    """The small least-squares problem of GMRES, min |beta e1 - H y|, with H the
+    (k+1) x k Hessenberg matrix of the Arnoldi coefficients, kept as Givens
+    rotations applied column by column (plain floats: the columns are short).
+    add_column returns |beta e1 - H y| for the columns so far."""
+
+    def __init__(self, beta, m):
+        self.R = np.zeros((m, m))  # the rotated columns (upper triangle)
+        self.cs, self.sn = [], []
+        self.g = [float(beta)]
+        self.k = 0
+
+    def add_column(self, col):
+        """col: H[0..k+1, k] as a list of k + 2 floats."""
+        k, cs, sn, g = self.k, self.cs, self.sn, self.g
+        col = list(col)
+        for j in range(k):
+            p, q, cj, sj = col[j], col[j + 1], cs[j], sn[j]
+            col[j] = cj * p + sj * q
+            col[j + 1] = cj * q - sj * p
+        p, q = col[k], col[k + 1]
+        den = math.hypot(p, q)
+        ck, sk = (1.0, 0.0) if den == 0 else (p / den, q / den)
+        cs.append(ck)
+        sn.append(sk)
+        col[k] = ck * p + sk * q
+        self.R[: k + 1, k] = col[: k + 1]
+        g.append(-sk * g[k])
+        g[k] = ck * g[k]
+        self.k = k + 1
+        return abs(g[k + 1])
+
+    def solve(self):
+        """y minimising |beta e1 - H y| over the columns so far."""
+        k = self.k
+        return np.linalg.solve(np.triu(self.R[:k, :k]), np.asarray(self.g[:k]))
+
+
 class DeviceGMRES:
     """Restarted GMRES for M x = b with M v = s0 v + s1 (A v), FP64 vectors on the
     device.
@@ -258,10 +296,8 @@ class DeviceGMRES:
             if rel <= tol:
                 break
             self.V[0].copy_(r / beta)
-            R = np.zeros((m, m))  # the rotated Hessenberg columns (upper triangle)
-            cs, sn = [], []
-            g = [beta]
-            k_used, done = 0, False
+            ls = HessenbergLSQ(beta, m)
+            done = False
             for ci in range(m // self.chunk):
                 if done or it >= maxiter:
                     break
@@ -270,28 +306,12 @@ class DeviceGMRES:
                 hv_all = self.hhost.numpy()
                 for c in range(self.chunk):
                     k = ci * self.chunk + c
-                    col = hv_all[k, : k + 2].tolist()  # plain floats: the rotations run in Python
-                    breakdown = col[k + 1] == 0.0
-                    for j in range(k):
-                        a, b, cj, sj = col[j], col[j + 1], cs[j], sn[j]
-                        col[j] = cj * a + sj * b
-                        col[j + 1] = cj * b - sj * a
-                    a, b = col[k], col[k + 1]
-                    den = math.hypot(a, b)
-                    ck, sk = (1.0, 0.0) if den == 0 else (a / den, b / den)
-                    cs.append(ck)
-                    sn.append(sk)
-                    col[k] = ck * a + sk * b
-                    R[: k + 1, k] = col[: k + 1]
-                    g.append(-sk * g[k])
-                    g[k] = ck * g[k]
+                    res = ls.add_column(hv_all[k, : k + 2].tolist())
                     it += 1
-                    k_used = k + 1
-                    if abs(g[k + 1]) / bnorm <= tol or breakdown or it >= maxiter:
+                    if res / bnorm <= tol or hv_all[k, k + 1] == 0.0 or it >= maxiter:
                         done = True
                         break
-            yk = np.linalg.solve(np.triu(R[:k_used, :k_used]), np.asarray(g[:k_used]))
-            x += torch.from_numpy(yk).to(b.device) @ self.V[:k_used]
+            x += torch.from_numpy(ls.solve()).to(b.device) @ self.V[: ls.k]
             r = b - self.matvec(x, self.Mx)
         return x, rel, it
 

@@ -278,3 +278,4 @@ def test_product_connectivity_gold(gold):
     rp, cc = G.connect_nd(t, 6, 3, "SG")
     for i in range(64):
         assert np.array_equal(gold[f"connectivity/connect_n_2_3_FG_{i + 1}"].ravel().astype(int), cc[rp[i]:rp[i + 1]])
+
