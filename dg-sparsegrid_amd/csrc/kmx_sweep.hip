@@ -440,9 +440,12 @@ __global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict_
 // f + dt a21 k1, stage 2 k2 and the box copy of f + dt (a31 k1 + a32 k2), stage 3
 // the new state f + dt (b1 k1 + b2 k2 + b3 k3) and (next_box) its box copy for the
 // next step; stage 0 only copies f into the box.  Absent cells of the box copy are
-// never written (zero since the plan's set-up).  k1 and k2 are kept in box order
-// (coalesced with the slots) and in T, the precision the ops computed them in;
-// the combinations are FP64.
+// never written (zero since the plan's set-up).  During one call the state itself
+// is kept in box order too (fb, FP64): stage 0 reads the caller's state in, every
+// later stage reads and writes it coalesced with the slots, and the last step's
+// stage 3 writes the caller's state back; present cells are flagged by one byte
+// each.  k1 and k2 are kept in box order in T, the precision the ops computed them
+// in; the combinations are FP64.
 constexpr double RK_A21 = 0.5, RK_A31 = -1.0, RK_A32 = 2.0;
 constexpr double RK_B1 = 1.0 / 6.0, RK_B2 = 2.0 / 3.0, RK_B3 = 1.0 / 6.0;
 struct Rk3Args {
@@ -450,6 +453,8 @@ struct Rk3Args {
     double dt[KMX_BATCH_MAX];
     void* k1;  // stage workspace (T), the instance's box values at koff
     void* k2;
+    double* fb;                   // the state in box order (FP64), at koff
+    const unsigned char* present;  // 1 at the present cells, at koff
 };
 template <typename T, int STAGE>
 __global__ void __launch_bounds__(256) k_box_rk3(const BoxJob* __restrict__ wg_jobs, Rk3Args a, int next_box) {
@@ -457,19 +462,20 @@ __global__ void __launch_bounds__(256) k_box_rk3(const BoxJob* __restrict__ wg_j
     const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
     if (i >= j.nb) return;
     T* box = static_cast<T*>(j.arena[sizeof(T) == 8 ? 0 : 1]);
-    double k = 0.0;
-    if (STAGE > 0) {  // the slots first, so their loads and the map's are in flight together
-        const T* sl = box + j.nb + i;
-        for (int q = 0; q < j.nslot; ++q) k += double(gld(sl + (long long)q * j.nb));
-    }
-    const int m = gld(j.ymap + i);
-    if (m < 0) return;
-    double* f = a.f[j.inst];
-    const double fm = gld(f + m);
-    if (STAGE == 0) {
+    double* fb = a.fb + j.koff;
+    if (STAGE == 0) {  // the caller's state in, and its box copy
+        const int m = gld(j.ymap + i);
+        if (m < 0) return;
+        const double fm = gld(a.f[j.inst] + m);
+        gst(fb + i, fm);
         gst(box + i, T(fm));
         return;
     }
+    double k = 0.0;  // the slots first, so their loads and the flag's are in flight together
+    const T* sl = box + j.nb + i;
+    for (int q = 0; q < j.nslot; ++q) k += double(gld(sl + (long long)q * j.nb));
+    if (!gld(a.present + j.koff + i)) return;
+    const double fm = gld(fb + i);
     const double dt = a.dt[j.inst];
     T* k1 = static_cast<T*>(a.k1) + j.koff;
     T* k2 = static_cast<T*>(a.k2) + j.koff;
@@ -481,9 +487,20 @@ __global__ void __launch_bounds__(256) k_box_rk3(const BoxJob* __restrict__ wg_j
         gst(box + i, T(fm + dt * (RK_A31 * double(gld(k1 + i)) + RK_A32 * k)));
     } else {
         const double fn = fm + dt * (RK_B1 * double(gld(k1 + i)) + RK_B2 * double(gld(k2 + i)) + RK_B3 * k);
-        gst(f + m, fn);
-        if (next_box) gst(box + i, T(fn));
+        if (next_box) {  // another step follows: the state stays in box order
+            gst(fb + i, fn);
+            gst(box + i, T(fn));
+        } else {  // the call's last step: the caller's state
+            gst(a.f[j.inst] + gld(j.ymap + i), fn);
+        }
     }
+}
+
+// The present-cell flags of every member (1 where the box maps to a value of y).
+__global__ void __launch_bounds__(256) k_box_present(const BoxJob* __restrict__ wg_jobs, unsigned char* present) {
+    const BoxJob& j = wg_jobs[blockIdx.x];
+    const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
+    if (i < j.nb) gst(present + j.koff + i, (unsigned char)(gld(j.ymap + i) >= 0 ? 1 : 0));
 }
 
 // Op kinds of the batched launch: (part, final, branches per tile).
@@ -1892,7 +1909,9 @@ struct kmx_sweep_batch {
     bool has16 = true;     // every op separable with BF16 values (BF16 applies on the FP32 arenas)
     bool all_sep = true;   // every member separable (the fused RK3 step)
     long long kdof = 0;    // values of each RK3 stage derivative (every member's box)
-    double* d_k = nullptr;  // RK3 stage workspace: k1 then k2, kdof FP64 values each (first kmx_sweep_batch_rk3)
+    // RK3 workspace (first kmx_sweep_batch_rk3): k1, k2, the box-order state (kdof FP64
+    // values each), then the present-cell flags (kdof bytes)
+    double* d_k = nullptr;
 };
 
 kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, int wave_batches, bool transpose) {
@@ -2129,11 +2148,16 @@ int kmx_sweep_batch_rk3(kmx_sweep_batch* sb, int slot, double* const* fs, const 
                       "near-identity factors under FP64)");
             return KMX_E_UNSUPPORTED;
         }
-    if (!sb->d_k && sb->kdof > 0) {
-        if (hipMalloc(&sb->d_k, size_t(2 * sb->kdof) * sizeof(double)) != hipSuccess) {
+    const dim3 bg(unsigned(sb->box_wgs)), bb(256);
+    unsigned char* present = nullptr;
+    if (sb->kdof > 0) {
+        const bool fresh = !sb->d_k;
+        if (fresh && hipMalloc(&sb->d_k, size_t(3 * sb->kdof) * sizeof(double) + size_t(sb->kdof)) != hipSuccess) {
             set_error("kmx_batch_rk3: allocating the stage workspace failed");
             return KMX_E_NOMEM;
         }
+        present = reinterpret_cast<unsigned char*>(sb->d_k + 3 * sb->kdof);
+        if (fresh) hipLaunchKernelGGL(k_box_present, bg, bb, 0, s, sb->d_wgjobs, present);
     }
     Rk3Args a;
     std::memset(&a, 0, sizeof(a));
@@ -2146,7 +2170,8 @@ int kmx_sweep_batch_rk3(kmx_sweep_batch* sb, int slot, double* const* fs, const 
         a.k2 = sb->d_k + sb->kdof;
     else
         a.k2 = reinterpret_cast<float*>(sb->d_k) + sb->kdof;  // FP32 stage values
-    const dim3 bg(unsigned(sb->box_wgs)), bb(256);
+    a.fb = sb->d_k + 2 * sb->kdof;
+    a.present = present;
     auto ops = [&]() {
         for (const auto& P : sb->phases) {
             if (P.nwg == 0) continue;

@@ -210,8 +210,9 @@ int kmx_batch_get_sweep(const kmx_batch* batch, int prec, int* members, int* act
  * launch per call that reads the states in).  fs[i]: plan i's DEVICE state, N*deg^d
  * FP64, advanced in place; dts[i]: its step (host array).  Needs the batch's merged
  * sweep on for prec (kmx_batch_set_sweep) with every plan a separable member (else
- * KMX_E_UNSUPPORTED).  The first call allocates the stage workspace (2 x the
- * values of the members' boxes, FP64); later calls allocate nothing.  Asynchronous
+ * KMX_E_UNSUPPORTED).  The first call allocates the stage workspace (3 x the
+ * values of the members' boxes in FP64, plus one byte each); later calls allocate
+ * nothing.  Asynchronous
  * on `stream`.  No counterpart in the reference (its time steps call apply_A three times). */
 int kmx_batch_rk3(kmx_batch* batch, double* const* fs, const double* dts, int nsteps, int prec, void* stream);
 
