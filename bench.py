@@ -56,10 +56,11 @@ MEASURED_TRAFFIC = {
 
 # HBM bytes of one FP32 merged-sweep batched apply of the 64-instance workload
 # (its three launches together: box copies of x, every dimension's op, slot
-# sums into y), from the same FETCH_SIZE / WRITE_SIZE counter runs and scale
-# factors.  Source: profiles/batch_sweep_hbm_traffic.md.
-MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = None
-TRAFFIC_SOURCE = None
+# sums into y): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE of the third batched apply
+# of `bench.py --pmc-pass`, taken from a profile of commit c7523cd -- a recorded
+# figure, not measured by this run (it goes stale when the kernels change).
+MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = 534236096
+TRAFFIC_SOURCE = "profile-derived: profiles/batch_sweep_hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, commit c7523cd)"
 
 
 def host_threads():
@@ -149,13 +150,17 @@ def separable_sweep_model(plans, elem):
     box copy of x (map, x, copy) and the slot sum (map, slots, y).  Returns
     (flops, algorithmic bytes, operator bytes)."""
     flops = nbytes = opbytes = 0
+    parts = {"gather": 0, "ops": 0, "sum": 0}
     for p in plans:
         info = p.sweep_info()
         D, nb, dof = p.ndim, info["box_values"], p.dof
         flops += info["flops"]
         opbytes += info["operator_values"] * elem
-        nbytes += info["operator_values"] * elem + D * 2 * nb * elem
-        nbytes += nb * (4 + elem) + dof * 8 + nb * (4 + D * elem) + dof * 8
+        parts["ops"] += info["operator_values"] * elem + D * 2 * nb * elem
+        parts["gather"] += nb * (4 + elem) + dof * 8
+        parts["sum"] += nb * (4 + D * elem) + dof * 8
+    nbytes = sum(parts.values())
+    separable_sweep_model.parts = parts  # per launch of a batched apply
     return flops, nbytes, opbytes
 
 
@@ -552,6 +557,7 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": TRAFFIC_SOURCE,
                 "algorithmic_bytes_per_apply": alg_bytes, "operator_bytes_per_apply": op_bytes,
+                "algorithmic_bytes_by_launch": dict(separable_sweep_model.parts),
                 "intensity_flops_per_byte": round(mflops / alg_bytes, 3),
                 "traffic_GBs": round(traffic / (apply_ms * 1e-3) / 1e9, 1) if traffic else None,
                 "kernel": "one FP32 batched apply = k_box_gather_batch<float> + k_sep_batch<float> "
