@@ -162,13 +162,24 @@ enum {
     UD_REC = 128
 };
 // a separable op's record: its panel's runs in the plan's run table instead of a level range
-enum { R_RUN0 = R_LA, R_NRUN = R_LB, R_NCT = R_TLO };
+// (R_RUN0 / R_NRUN for the host's checks; the kernels read the panel's steps from
+// R_STEP0, its first step inside the op's storage)
+enum { R_RUN0 = R_LA, R_NRUN = R_LB, R_NCT = R_TLO, R_STEP0 = R_TOUT };
 // a separable tile whose panel has at most SEP_KMAX steps may sweep R_NCT column
 // tiles of its group with the panel's A values held in registers
 // (8: the packed body's registers set the launch's occupancy -- 92 VGPRs at 16
 // steps, 68 at 8; an FP32 RK3 step of the 64-instance batch took 0.297 ms at 16,
 // 0.288 at 12, 0.279 at 8 and 4)
 constexpr int SEP_KMAX = 8;
+// zero steps after the last of a separable storage (a tile's last load batch, or a
+// packed tile's SEP_KMAX steps, reads up to 7 steps past its own)
+constexpr int SEP_PAD = 8;
+// A step's code: level l (bits 0-4), max(l - 1, 0) (bits 5-9: the shift that turns
+// a column-o offset at level 1 into level l's), valid positions (bits 10-12; 0 for
+// no step), first position inside the level (bits 13-30).
+__host__ __device__ constexpr int sep_code(int l, int valid, int pos) {
+    return l | (l > 1 ? l - 1 : 0) << 5 | valid << 10 | pos << 13;
+}
 constexpr int UD_MAXLEV = UD_REC - UD_BOX - 1;  // highest level a record holds a box for
 // Waves per workgroup.  Two: a workgroup holds its slots until its slowest wave
 // ends (the split tiles' partial sums meet at a barrier), so small workgroups keep
@@ -528,17 +539,17 @@ struct UdBatchOp {
     void* arena[2];
     const int* xmap;
     const int* ymap;
-    const int4* runs;  // separable ops: the plan's run table (forward or transposed)
+    const int* codes;  // separable ops: the plan's step codes (forward or transposed)
     int inst;
     int kind;
 };
 
 template <typename T, typename TA, int NB>
-__device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int4* __restrict__ runs,
+__device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int* __restrict__ codes,
                                          T* __restrict__ arena, const int2 rv, const int w, const int lane,
                                          const UdOp& op, T* __restrict__ part);
 template <typename T, typename TA>
-__device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const int4* __restrict__ runs,
+__device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const int* __restrict__ codes,
                                               T* __restrict__ arena, const int2 rv, const int lane, const UdOp& op);
 
 // One phase of a batch: ops of many instances, each on whole workgroups
@@ -571,9 +582,9 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
         case UK_UP_FINAL: ud_tile<T, UD_UP, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
         case UK_SEP:
             if (__builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
-                sep_tile_cols<T, T>(mat, o.runs, arena, rv, lane, o.a);
+                sep_tile_cols<T, T>(mat, o.codes, arena, rv, lane, o.a);
             else
-                sep_tile<T, T, 8>(mat, o.runs, arena, rv, w, lane, o.a, part);
+                sep_tile<T, T, 8>(mat, o.codes, arena, rv, w, lane, o.a, part);
             break;
         default:
             if constexpr (MAXNBR == 4) {
@@ -592,22 +603,22 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
 // m0 .. m0 + 15 of the group's positions along the op dimension, columns f0 .. f0 + 15
 // of its O x I columns) from the compact row-panel storage of the dimension's
 // combined operator (its terms summed, scales included).  The storage keeps, per
-// 16-row panel, only the steps of four source positions that meet a kept block;
-// they come as runs (level, first position in the level, positions, storage
-// offset), and the tile uses the panel's runs of the levels its group holds.
-// Element (row li, position 4 s + lk) of a run's step s sits at offset
-// 64 s + 16 lk + li = 64 s + lane, so each step is one contiguous 64-value read
-// per wave; B comes from the box of the run's level at the lane's column.  The
-// wave's share is steps [slo, shi), counted over runs run0 .. run0 + nrun - 1; NB
-// steps are addressed on the scalar unit (crossing runs as needed), then all
-// their loads are issued, then their matrix operations.  Rows past the group
-// read the panel's other rows (finite; never stored), rows past the matrix read
-// the storage's zero padding; positions past a run's end (its last step when
-// the run's positions are not a multiple of 4) read zero A and B is masked.
+// 16-row panel, only the steps of four source positions that meet a kept block,
+// consecutive from the panel's first step (record R_STEP0, inside the op's
+// storage), levels in order; the group's steps are the panel's first R_SPER.
+// Element (row li, position 4 s + lk) of step s sits at offset 64 s + 16 lk + li =
+// 64 s + lane, so each step is one contiguous 64-value read per wave; the step's
+// code (level, valid positions, first position inside the level) gives the box
+// offset of B at the lane's column: box[l] + (fo * nl + p + lk) * I + fi, where
+// fo * nl * I = (fo * deg * I) << max(l - 1, 0).  The wave's share is steps
+// [slo, shi); NB steps are loaded per batch, then multiplied.  Rows past the
+// group read the panel's other rows (finite; never stored), rows past the matrix
+// read the storage's zero padding; B is zero at positions past a step's valid ones
+// and at the steps past shi of the last batch (their A is a step of the panel).
 // TA: the storage type of A (T, or BF16 bit patterns with T = float; NB is a
 // multiple of 4, the steps of one BF16 product).
 template <typename T, typename TA, int NB>
-__device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int4* __restrict__ runs,
+__device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int* __restrict__ codes,
                                          T* __restrict__ arena, const int2 rv, const int w, const int lane,
                                          const UdOp& op, T* __restrict__ part) {
     static_assert(NB % 4 == 0, "steps come in groups of four");
@@ -639,69 +650,37 @@ __device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int4*
             }
         const int tn = __builtin_amdgcn_readlane(rv.x, R_TN);
         if (tn > 0) {
-            const int deg = op.deg;
-            const int run0 = __builtin_amdgcn_readlane(rv.x, R_RUN0);
-            const int rend = run0 + __builtin_amdgcn_readlane(rv.x, R_NRUN);
             const int slo = __builtin_amdgcn_readlane(rv.x, R_SLO), shi = __builtin_amdgcn_readlane(rv.x, R_SHI);
-            const TA* amat = mat + op.a_off[0] + lane;
+            const long long g0 = op.a_off[0] / 64 + __builtin_amdgcn_readlane(rv.x, R_STEP0) + slo;
+            const TA* amat = mat + 64 * g0 + lane;  // the storage runs SEP_PAD steps past its last
+            const int* cd = codes + g0;
             const T* bsrc = arena + op.in_base;
-            // the tile's runs, one per lane (64 at a time)
-            int c0 = run0;
-            int4 myr = gld(runs + c0 + min(lane, rend - 1 - c0));
-            int rr = run0, nst = 0, npos = 0, a = 0, b = 0;
-            auto load_run = [&]() {
-                if (rr - c0 >= 64) {
-                    c0 = rr;
-                    myr = gld(runs + c0 + min(lane, rend - 1 - c0));
-                }
-                const int k = rr - c0;
-                const int l = __builtin_amdgcn_readlane(myr.x, k);
-                const int pos0 = __builtin_amdgcn_readlane(myr.y, k);
-                npos = __builtin_amdgcn_readlane(myr.z, k);
-                a = __builtin_amdgcn_readlane(myr.w, k);
-                nst = (npos + 3) >> 2;
-                const int nl = deg * (l ? 1 << (l - 1) : 1);
-                const int box = __builtin_amdgcn_readlane(rv.y, min(l, UD_MAXLEV));
-                b = box + (fo * nl + pos0 + lk) * I + fi;
-            };
-            load_run();
-            int sin = slo;  // step inside run rr
-            while (sin >= nst) {
-                sin -= nst;
-                ++rr;
-                load_run();
-            }
-            for (int q = slo; q < shi; q += NB) {
-                int ao[NB], bo[NB];
-                bool pa[NB], pb[NB];
+            int lane_off = lk * I + fi;          // the lane's part of a B offset
+            asm("" : "+v"(lane_off));            // (kept whole: not refolded into (p + lk) * I)
+            const int fo_dI = fo * op.deg * I;   // fo * nl * I at levels 0 and 1
+            const int ns = shi - slo;
+            for (int q0 = 0; q0 < ns; q0 += 64) {
+                const int qn = min(64, ns - q0);
+                const int cv = lane < qn ? gld(cd + q0 + lane) : 0;  // lane t: the code of step q0 + t
+                for (int q = 0; q < qn; q += NB) {
+                    const TA* ab = amat + 64 * (q0 + q);
+                    TA av[NB];
+                    T bv[NB];
 #pragma unroll
-                for (int c = 0; c < NB; ++c) {
-                    const bool on = q + c < shi;
-                    pa[c] = on;
-                    pb[c] = on && 4 * sin + lk < npos;
-                    ao[c] = on ? a + 64 * sin : 0;
-                    bo[c] = pb[c] ? b + 4 * sin * I : 0;
-                    if (on && ++sin == nst) {
-                        sin = 0;
-                        if (q + c + 1 < shi) {
-                            ++rr;
-                            load_run();
-                        }
+                    for (int c = 0; c < NB; ++c) av[c] = gld(ab + 64 * c);
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) {
+                        const int code = __builtin_amdgcn_readlane(cv, q + c);
+                        const int nv = (code >> 10) & 7;  // valid positions (0 past the wave's steps)
+                        const int sb = __builtin_amdgcn_readlane(rv.y, code & 31) + (code >> 13) * I;
+                        const unsigned ad = unsigned(sb + (fo_dI << ((code >> 5) & 31)) + lane_off);
+                        const bool ok = lk < nv;
+                        const T b = gld(bsrc + (ok ? ad : 0u));
+                        bv[c] = ok ? b : T(0);
                     }
+#pragma unroll
+                    for (int c = 0; c < NB; c += 4) acc = SepMma<T, TA>::run4(av + c, bv + c, acc);
                 }
-                TA av[NB];
-                T bv[NB];
-#pragma unroll
-                for (int c = 0; c < NB; ++c) av[c] = gld(amat + ao[c]);
-#pragma unroll
-                for (int c = 0; c < NB; ++c) bv[c] = gld(bsrc + bo[c]);
-#pragma unroll
-                for (int c = 0; c < NB; ++c) {
-                    av[c] = pa[c] ? av[c] : TA(0);
-                    bv[c] = pb[c] ? bv[c] : T(0);
-                }
-#pragma unroll
-                for (int c = 0; c < NB; c += 4) acc = SepMma<T, TA>::run4(av + c, bv + c, acc);
             }
         }
         if (gsize > 1 && !leader)
@@ -720,13 +699,13 @@ __device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int4*
 }
 
 // One wave of a separable op over R_NCT consecutive 16-column tiles of one group
-// and one row panel whose kept steps (at most SEP_KMAX, from one run list, not
-// split over waves) are loaded once: the panel's A values and the lane's B
-// offsets without the column term stay in registers, and each column tile then
-// costs its B loads, its matrix operations and its stores.  Two column tiles are
-// in flight at a time.  Holds one workgroup barrier, like sep_tile.
+// and one row panel whose kept steps (at most SEP_KMAX, not split over waves) are
+// loaded once: the panel's A values and the lane's B offsets without the column
+// term stay in registers, and each column tile then costs its B loads, its matrix
+// operations and its stores.  Two column tiles are in flight at a time.  Holds
+// one workgroup barrier, like sep_tile.
 template <typename T, typename TA>
-__device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const int4* __restrict__ runs,
+__device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const int* __restrict__ codes,
                                               T* __restrict__ arena, const int2 rv, const int lane, const UdOp& op) {
     static_assert(SEP_KMAX % 4 == 0, "steps come in groups of four");
     using V = typename SwMfma<T>::V;
@@ -744,43 +723,26 @@ __device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const 
         rok[q] = __shfl(rv.x, R_ROWLEV + row) >= 0;
     }
     const int ns = __builtin_amdgcn_readlane(rv.x, R_SHI);  // the panel's steps (from step 0)
-    const int run0 = __builtin_amdgcn_readlane(rv.x, R_RUN0), nrun = __builtin_amdgcn_readlane(rv.x, R_NRUN);
-    const TA* amat = mat + op.a_off[0] + lane;
+    const long long g0 = op.a_off[0] / 64 + __builtin_amdgcn_readlane(rv.x, R_STEP0);
+    const TA* amat = mat + 64 * g0 + lane;  // the storage runs SEP_PAD steps past its last
+    const int cv = lane < ns ? gld(codes + g0 + lane) : 0;  // lane t: the code of step t
     const T* bsrc = arena + op.in_base;
-    const int4 myr = gld(runs + run0 + min(lane, nrun - 1));  // nrun <= SEP_KMAX
     TA av[SEP_KMAX];
-    int bp[SEP_KMAX], nli[SEP_KMAX];
+    int bp[SEP_KMAX], sh[SEP_KMAX];
     bool pb[SEP_KMAX];
-    {
-        int rr = 0, sin = 0, nst = 0, npos = 0, a = 0, base = 0, nl = 0;
-        auto load_run = [&]() {
-            const int l = __builtin_amdgcn_readlane(myr.x, rr);
-            const int pos0 = __builtin_amdgcn_readlane(myr.y, rr);
-            npos = __builtin_amdgcn_readlane(myr.z, rr);
-            a = __builtin_amdgcn_readlane(myr.w, rr);
-            nst = (npos + 3) >> 2;
-            nl = deg_cells(op.deg, l);
-            base = __builtin_amdgcn_readlane(rv.y, min(l, UD_MAXLEV)) + (pos0 + lk) * I;
-        };
-        load_run();
 #pragma unroll
-        for (int c = 0; c < SEP_KMAX; ++c) {
-            const bool on = c < ns;
-            pb[c] = on && 4 * sin + lk < npos;
-            av[c] = on ? gld(amat + a + 64 * sin) : TA(0);
-            bp[c] = pb[c] ? base + 4 * sin * I : 0;
-            nli[c] = nl * I;
-            if (on && ++sin == nst && c + 1 < ns) {
-                ++rr;
-                sin = 0;
-                load_run();
-            }
-        }
+    for (int c = 0; c < SEP_KMAX; ++c) {
+        const int code = __builtin_amdgcn_readlane(cv, c);
+        const int nv = (code >> 10) & 7;
+        pb[c] = lk < nv;
+        av[c] = gld(amat + 64 * c);
+        bp[c] = __builtin_amdgcn_readlane(rv.y, code & 31) + ((code >> 13) + lk) * I;
+        sh[c] = (code >> 5) & 31;  // fo * nl * I = (fo * deg * I) << sh
     }
     T* out = arena + op.out_base;
     for (int ct = 0; ct < nct; ct += 2) {
         const bool two = ct + 1 < nct;
-        int fo[2], fi[2];
+        int fo[2], fi[2], fod[2];
         bool fok[2];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
@@ -789,6 +751,7 @@ __device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const 
             const int fc = fok[u] ? f : 0;
             fo[u] = I == 1 ? fc : magic ? int(__umulhi(unsigned(fc), magic)) : fc / I;
             fi[u] = fc - fo[u] * I;
+            fod[u] = fo[u] * op.deg * I;
         }
         V acc[2] = {V{T(0), T(0), T(0), T(0)}, V{T(0), T(0), T(0), T(0)}};
 #pragma unroll
@@ -800,7 +763,7 @@ __device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const 
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const bool ok = pb[c0 + j] && fok[u];
-                        const int ad = ok ? bp[c0 + j] + fo[u] * nli[c0 + j] + fi[u] : 0;
+                        const unsigned ad = ok ? unsigned(bp[c0 + j] + fi[u] + (fod[u] << sh[c0 + j])) : 0u;
                         const T v = gld(bsrc + ad);
                         bv[u][j] = ok ? v : T(0);
                     }
@@ -836,9 +799,9 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_sep_batch(const UdBatchOp* __
     constexpr int ml = sizeof(TA) == 2 ? 2 : sl;          // values
     const TA* mat = static_cast<const TA*>(o.mat[ml]);
     if (__builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
-        sep_tile_cols<T, TA>(mat, o.runs, static_cast<T*>(o.arena[sl]), rv, lane, o.a);
+        sep_tile_cols<T, TA>(mat, o.codes, static_cast<T*>(o.arena[sl]), rv, lane, o.a);
     else
-        sep_tile<T, TA, 8>(mat, o.runs, static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
+        sep_tile<T, TA, 8>(mat, o.codes, static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
 }
 
 }  // namespace
@@ -888,14 +851,17 @@ struct HostTile {
 // runs[run_base[k] + prun[k][p] ..  run_base[k] + prun[k][p + 1]) in level order:
 // (level, first position inside the level, positions, storage offset from
 // a_base[k]); a run holds 4 * ceil(positions / 4) positions x 16 rows, position
-// major, zero past the run's positions and past the matrix rows.
+// major, zero past the run's positions and past the matrix rows.  codes: per step
+// of the storage (its offset / 64), sep_code(level, valid positions, first position
+// inside the level) -- what the kernels read per step.
 struct SepSide {
     std::vector<int4> runs;
+    std::vector<int> codes;
     std::vector<int> run_base;
     std::vector<std::vector<int>> prun;
     std::vector<long long> a_base;
     long long a_elems = 0;
-    int4* d_runs = nullptr;
+    int* d_codes = nullptr;
     double* d_a64 = nullptr;
     float* d_a32 = nullptr;
     unsigned short* d_a16 = nullptr;  // BF16 bit patterns of the same values (with the FP32 copy)
@@ -1076,6 +1042,7 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
                 const int q0 = std::min(h.steps, w * qn), q1 = std::min(h.steps, (w + 1) * qn);
                 if (sep) {  // steps counted over the tile's runs; R_NCT column tiles
                     r[R_NCT] = h.nct;
+                    r[R_STEP0] = h.lb > 0 ? sw.side[side].runs[size_t(h.la)].w / 64 : 0;
                     r[R_SLO] = q0;
                     r[R_SHI] = q1;
                     r[R_TN] = q1 > q0 ? 1 : 0;
@@ -1165,8 +1132,8 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
                 const Group& g = *gp;
                 const int n = deg << g.m;
                 if (r[R_M0] < 0 || r[R_M0] >= n || r[R_M0] % 16 || r[R_F0] < 0 || r[R_F0] >= g.O * g.I ||
-                    r[R_B] < 0 || r[R_B] % op.nbr || r[R_B] + op.nbr > nbranch || r[R_TOUT] < 0 || r[R_TOUT] >= T ||
-                    r[R_MAGIC] != int(g.magic))
+                    r[R_B] < 0 || r[R_B] % op.nbr || r[R_B] + op.nbr > nbranch || r[R_TOUT] < 0 ||
+                    (!sep && r[R_TOUT] >= T) || r[R_MAGIC] != int(g.magic))  // (R_STEP0 on separable records)
                     throw std::logic_error("sweep record outside its group");
                 for (int i = 0; i < 16; ++i) {
                     const int row = r[R_M0] + i;
@@ -1191,13 +1158,20 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
                         r[R_NRUN] > pr[size_t(p) + 1] - pr[size_t(p)])
                         throw std::logic_error("sweep separable record runs");
                     long long steps = 0;
+                    const long long st0 = r[R_NRUN] > 0 ? sd.runs[size_t(r[R_RUN0])].w / 64 : 0;
+                    if (r[R_STEP0] != st0) throw std::logic_error("sweep separable record first step");
                     for (int q = r[R_RUN0]; q < r[R_RUN0] + r[R_NRUN]; ++q) {
                         const int4& ru = sd.runs[size_t(q)];
                         const long long nl = deg * cells_of(ru.x);
                         const long long nst = (ru.z + 3) / 4;
                         if (ru.x < 0 || ru.x > g.m || ru.x > UD_MAXLEV || ru.y < 0 || ru.y % 4 || ru.z < 1 ||
-                            ru.y + ru.z > nl || a_lo + ru.w < a_lo || a_lo + ru.w + 64 * nst > a_hi || ru.w % 64)
+                            ru.y + ru.z > nl || a_lo + ru.w < a_lo || a_lo + ru.w + 64 * nst > a_hi || ru.w % 64 ||
+                            ru.w / 64 != st0 + steps)
                             throw std::logic_error("sweep separable run");
+                        for (long long t = 0; t < nst; ++t)  // the codes the kernels read for these steps
+                            if (sd.codes[size_t(a_lo / 64 + st0 + steps + t)] !=
+                                sep_code(ru.x, int(std::min(4LL, ru.z - 4 * t)), int(ru.y + 4 * t)))
+                                throw std::logic_error("sweep separable step code");
                         steps += nst;
                     }
                     if (r[R_SPER] != steps || r[R_TN] != (r[R_SHI] > r[R_SLO] ? 1 : 0) || r[R_SLO] < 0 ||
@@ -1292,6 +1266,7 @@ std::vector<unsigned char> sep_pattern(const kmx_plan* p, const kmx_sweep& sw, i
 void sep_runs(const kmx_plan* p, kmx_sweep& sw, int s) {
     SepSide& sd = sw.side[s];
     sd.runs.clear();
+    sd.codes.clear();
     sd.run_base.clear();
     sd.prun.clear();
     sd.a_base.clear();
@@ -1338,6 +1313,9 @@ void sep_runs(const kmx_plan* p, kmx_sweep& sw, int s) {
                     const long long pos0 = 4 * s0, npos = std::min(nl, 4 * s1) - pos0;
                     if (aoff > (1LL << 30)) throw std::runtime_error("separable op storage beyond 32-bit offsets");
                     sd.runs.push_back(make_int4(l, int(pos0), int(npos), int(aoff)));
+                    if (l > 31 || pos0 + npos >= (1LL << 18)) throw std::runtime_error("separable op beyond the step codes");
+                    for (long long s = 0; 4 * s < npos; ++s)
+                        sd.codes.push_back(sep_code(l, int(std::min(4LL, npos - 4 * s)), int(pos0 + 4 * s)));
                     aoff += 64 * (s1 - s0);
                     s0 = s1;
                 }
@@ -1345,10 +1323,12 @@ void sep_runs(const kmx_plan* p, kmx_sweep& sw, int s) {
         }
         pr[size_t(npan)] = int(sd.runs.size()) - sd.run_base.back();
         sd.prun.push_back(std::move(pr));
+        if (aoff == 0) sd.codes.push_back(0);    // the dummy step's code
         abase += std::max<long long>(aoff, 64);  // an op without runs still owns 64 values (its tiles' dummy reads)
         if (sd.runs.size() > size_t(1) << 30) throw std::runtime_error("separable op runs beyond 32-bit indices");
     }
     sd.a_elems = abase;
+    if ((long long)sd.codes.size() * 64 != abase) throw std::logic_error("separable step codes out of step");
 }
 
 // FP64 -> BF16 bit pattern, rounded once to nearest even at 8 significant bits
@@ -1375,14 +1355,22 @@ void sep_values(const kmx_plan* p, kmx_sweep& sw, int s, int only = -1) {
     (void)deg;
     if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
     const bool want32 = p->bs->d_mat32 != nullptr;
-    if (!sd.d_runs) {
-        if (hipMalloc(&sd.d_runs, std::max<size_t>(1, sd.runs.size()) * sizeof(int4)) != hipSuccess ||
-            hipMalloc(&sd.d_a64, size_t(std::max(1LL, sd.a_elems)) * sizeof(double)) != hipSuccess ||
-            (want32 && hipMalloc(&sd.d_a32, size_t(std::max(1LL, sd.a_elems)) * sizeof(float)) != hipSuccess) ||
-            (want32 && hipMalloc(&sd.d_a16, size_t(std::max(1LL, sd.a_elems)) * sizeof(unsigned short)) != hipSuccess))
+    if (!sd.d_codes) {
+        // zero padding after the values (SEP_PAD steps)
+        const size_t nv = size_t(sd.a_elems) + 64 * SEP_PAD, nc = std::max<size_t>(1, sd.codes.size());
+        if (hipMalloc(&sd.d_codes, nc * sizeof(int)) != hipSuccess ||
+            hipMalloc(&sd.d_a64, nv * sizeof(double)) != hipSuccess ||
+            (want32 && hipMalloc(&sd.d_a32, nv * sizeof(float)) != hipSuccess) ||
+            (want32 && hipMalloc(&sd.d_a16, nv * sizeof(unsigned short)) != hipSuccess))
             throw std::bad_alloc();
-        if (!sd.runs.empty() &&
-            hipMemcpy(sd.d_runs, sd.runs.data(), sd.runs.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemset(sd.d_codes, 0, nc * sizeof(int)) != hipSuccess ||
+            hipMemset(sd.d_a64, 0, nv * sizeof(double)) != hipSuccess ||
+            (want32 && hipMemset(sd.d_a32, 0, nv * sizeof(float)) != hipSuccess) ||
+            (want32 && hipMemset(sd.d_a16, 0, nv * sizeof(unsigned short)) != hipSuccess) ||
+            hipDeviceSynchronize() != hipSuccess)
+            throw std::runtime_error("hipMemset failed");
+        if (!sd.codes.empty() &&
+            hipMemcpy(sd.d_codes, sd.codes.data(), sd.codes.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
             throw std::runtime_error("hipMemcpy failed");
     }
     for (size_t k = 0; k < sw.desc.size(); ++k) {
@@ -1423,11 +1411,11 @@ void sep_values(const kmx_plan* p, kmx_sweep& sw, int s, int only = -1) {
 }
 
 void sep_side_free(SepSide& sd) {
-    if (sd.d_runs) hipFree(sd.d_runs);
+    if (sd.d_codes) hipFree(sd.d_codes);
     if (sd.d_a64) hipFree(sd.d_a64);
     if (sd.d_a32) hipFree(sd.d_a32);
     if (sd.d_a16) hipFree(sd.d_a16);
-    sd.d_runs = nullptr;
+    sd.d_codes = nullptr;
     sd.d_a64 = nullptr;
     sd.d_a32 = nullptr;
     sd.d_a16 = nullptr;
@@ -1444,9 +1432,9 @@ void sep_build(kmx_plan* p, kmx_sweep& sw) {
         sw.total_blocks += (long long)sw.keep.back().size();
     }
     for (int s = 0; s < 2; ++s) {
-        if (sw.side[s].d_runs) {
+        if (sw.side[s].d_codes) {
             sw.retired.push_back(sw.side[s]);
-            sw.side[s].d_runs = nullptr;
+            sw.side[s].d_codes = nullptr;
             sw.side[s].d_a64 = nullptr;
             sw.side[s].d_a32 = nullptr;
         }
@@ -1966,7 +1954,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                     o.mat[0] = sep ? static_cast<const void*>(sw->side[side].d_a64) : p->bs->d_mat64;
                     o.mat[1] = sep ? static_cast<const void*>(sw->side[side].d_a32) : p->bs->d_mat32;
                     o.mat[2] = sep ? static_cast<const void*>(sw->side[side].d_a16) : nullptr;
-                    o.runs = sep ? sw->side[side].d_runs : nullptr;
+                    o.codes = sep ? sw->side[side].d_codes : nullptr;
                     if (sep && !sw->side[side].d_a32) sb->has32 = false;
                     if (!sep || !sw->side[side].d_a16 || !sw->d_arena[1]) sb->has16 = false;
                     o.arena[0] = sw->d_arena[0];
