@@ -44,15 +44,31 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// partial dots of rows 0..k with w over this workgroup's chunk: wave q takes rows q, q + 4, ...
+// partial dots of rows 0..k with w over this workgroup's chunk: wave q takes rows
+// q, q + 4, q + 8, ..., AR_RB of them at a time, so a lane keeps AR_RB independent
+// sums and loads in flight
+constexpr int AR_RB = 8;
+constexpr int AR_WAVES = AR_THREADS / 64;
 __device__ __forceinline__ void chunk_dots(const ArnoldiArgs& a, long long i0, long long i1, double* part) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int j = wave; j <= a.k; j += AR_THREADS / 64) {
-        const double* v = a.V + (long long)j * a.n;
-        double s = 0.0;
-        for (long long i = i0 + lane; i < i1; i += 64) s = __builtin_fma(v[i], a.w[i], s);
-        s = wave_sum(s);
-        if (lane == 0) part[(long long)blockIdx.x * (a.k + 1) + j] = s;
+    for (int j0 = wave; j0 <= a.k; j0 += AR_WAVES * AR_RB) {
+        double s[AR_RB];
+#pragma unroll
+        for (int r = 0; r < AR_RB; ++r) s[r] = 0.0;
+        for (long long i = i0 + lane; i < i1; i += 64) {
+            const double wi = a.w[i];
+#pragma unroll
+            for (int r = 0; r < AR_RB; ++r)
+                if (j0 + AR_WAVES * r <= a.k) s[r] = __builtin_fma(a.V[(long long)(j0 + AR_WAVES * r) * a.n + i], wi, s[r]);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int r = 0; r < AR_RB; ++r) s[r] += __shfl_xor(s[r], off);
+        if (lane == 0)
+#pragma unroll
+            for (int r = 0; r < AR_RB; ++r)
+                if (j0 + AR_WAVES * r <= a.k) part[(long long)blockIdx.x * (a.k + 1) + j0 + AR_WAVES * r] = s[r];
     }
 }
 
@@ -60,6 +76,7 @@ __device__ __forceinline__ void chunk_dots(const ArnoldiArgs& a, long long i0, l
 __device__ __forceinline__ void sum_partials(const ArnoldiArgs& a, const double* part, double* sh) {
     for (int j = threadIdx.x; j <= a.k; j += AR_THREADS) {
         double s = 0.0;
+#pragma unroll 8
         for (int g = 0; g < a.G; ++g) s += part[(long long)g * (a.k + 1) + j];
         sh[j] = s;
     }
@@ -70,6 +87,7 @@ __device__ __forceinline__ void sum_partials(const ArnoldiArgs& a, const double*
 __device__ __forceinline__ void chunk_update(const ArnoldiArgs& a, long long i0, long long i1, const double* sh) {
     for (long long i = i0 + threadIdx.x; i < i1; i += AR_THREADS) {
         double s = a.w[i];
+#pragma unroll 8
         for (int j = 0; j <= a.k; ++j) s = __builtin_fma(-a.V[(long long)j * a.n + i], sh[j], s);
         a.w[i] = s;
     }

@@ -229,9 +229,10 @@ class DeviceGMRES:
     are captured in one HIP graph of their own (k fixed in each, so nothing is
     indexed on the device), whose replay ends by copying the chunk's Hessenberg
     columns to pinned host memory.  The host applies the Givens rotations and
-    checks the residual estimate once per chunk; a chunk may run up to chunk - 1
-    steps past convergence, whose basis vectors are not used.  restart must be a
-    multiple of chunk and at most KMX_ARNOLDI_MAX_K."""
+    checks the residual estimate once per chunk, while the next chunk's replay
+    already runs; so up to 2 chunk - 1 steps may run past convergence, whose basis
+    vectors are not used.  restart must be a multiple of chunk and at most
+    KMX_ARNOLDI_MAX_K."""
 
     def __init__(self, apply_into, n, restart=40, device="cuda", chunk=4, shift=(0.0, 1.0)):
         torch = _torch()
@@ -297,13 +298,20 @@ class DeviceGMRES:
                 break
             self.V[0].copy_(r / beta)
             ls = HessenbergLSQ(beta, m)
-            done = False
-            for ci in range(m // self.chunk):
-                if done or it >= maxiter:
-                    break
-                self._graph(ci).replay()
-                torch.cuda.current_stream().synchronize()
+            nch = m // self.chunk
+            self._graph(0).replay()
+            ev = torch.cuda.Event()
+            ev.record()
+            for ci in range(nch):
+                ev.synchronize()  # chunk ci done: its Hessenberg columns are on the host
+                if ci + 1 < nch and it + self.chunk < maxiter:
+                    # the next chunk runs while the host rotates this one's columns
+                    # (after convergence it is wasted work whose basis vectors go unused)
+                    self._graph(ci + 1).replay()
+                    ev = torch.cuda.Event()
+                    ev.record()
                 hv_all = self.hhost.numpy()
+                done = False
                 for c in range(self.chunk):
                     k = ci * self.chunk + c
                     res = ls.add_column(hv_all[k, : k + 2].tolist())
@@ -311,6 +319,8 @@ class DeviceGMRES:
                     if res / bnorm <= tol or hv_all[k, k + 1] == 0.0 or it >= maxiter:
                         done = True
                         break
+                if done or it >= maxiter:
+                    break
             x += torch.from_numpy(ls.solve()).to(b.device) @ self.V[: ls.k]
             r = b - self.matvec(x, self.Mx)
         return x, rel, it
