@@ -418,20 +418,31 @@ struct BoxJob {
     long long koff;  // the instance's first box value in the batch's RK3 stage workspace
     int nslot, inst, blk0;
 };
-template <typename T>
-__global__ void __launch_bounds__(256) k_box_gather_batch(const BoxJob* __restrict__ wg_jobs, BatchIO io) {
+// x / y of a batch of one plan: a 16-byte kernel argument instead of BatchIO's 2 KB
+// (a lone plan's apply is bound by its launches)
+struct IoOne {
+    const double* x;
+    double* y;
+};
+__device__ __forceinline__ const double* io_x(const BatchIO& io, int i) { return io.x[i]; }
+__device__ __forceinline__ double* io_y(const BatchIO& io, int i) { return io.y[i]; }
+__device__ __forceinline__ const double* io_x(const IoOne& io, int) { return io.x; }
+__device__ __forceinline__ double* io_y(const IoOne& io, int) { return io.y; }
+
+template <typename T, typename IO>
+__global__ void __launch_bounds__(256) k_box_gather_batch(const BoxJob* __restrict__ wg_jobs, IO io) {
     const BoxJob& j = wg_jobs[blockIdx.x];
     const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
     if (i < j.nb) {
         const int m = gld(j.xmap + i);
         T* xb = static_cast<T*>(j.arena[sizeof(T) == 8 ? 0 : 1]);
-        gst(xb + i, m >= 0 ? T(gld(io.x[j.inst] + m)) : T(0));
+        gst(xb + i, m >= 0 ? T(gld(io_x(io, j.inst) + m)) : T(0));
     }
 }
 // y = the ops' slots summed (in op order), at the present cells
 // (the slots are read before the map, so both loads are in flight together)
-template <typename T>
-__global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict__ wg_jobs, BatchIO io) {
+template <typename T, typename IO>
+__global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict__ wg_jobs, IO io) {
     const BoxJob& j = wg_jobs[blockIdx.x];
     const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
     if (i < j.nb) {
@@ -439,7 +450,7 @@ __global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict_
         double v = 0.0;
         for (int k = 0; k < j.nslot; ++k) v += double(gld(sl + (long long)k * j.nb));
         const int m = gld(j.ymap + i);
-        if (m >= 0) gst(io.y[j.inst] + m, v);
+        if (m >= 0) gst(io_y(io, j.inst) + m, v);
     }
 }
 
@@ -2085,9 +2096,16 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
         return KMX_E_UNSUPPORTED;
     }
     const dim3 bg(unsigned(sb->box_wgs)), bb(256);
+    const bool one = sb->members.size() == 1;
+    const IoOne io1{io.x[0], io.y[0]};
     if (sb->njobs > 0) {
-        if (slot == 0) hipLaunchKernelGGL(k_box_gather_batch<double>, bg, bb, 0, s, sb->d_wgjobs, io);
-        else hipLaunchKernelGGL(k_box_gather_batch<float>, bg, bb, 0, s, sb->d_wgjobs, io);
+        if (one) {
+            if (slot == 0) hipLaunchKernelGGL((k_box_gather_batch<double, IoOne>), bg, bb, 0, s, sb->d_wgjobs, io1);
+            else hipLaunchKernelGGL((k_box_gather_batch<float, IoOne>), bg, bb, 0, s, sb->d_wgjobs, io1);
+        } else {
+            if (slot == 0) hipLaunchKernelGGL((k_box_gather_batch<double, BatchIO>), bg, bb, 0, s, sb->d_wgjobs, io);
+            else hipLaunchKernelGGL((k_box_gather_batch<float, BatchIO>), bg, bb, 0, s, sb->d_wgjobs, io);
+        }
     }
     for (const auto& P : sb->phases) {
         if (P.nwg == 0) continue;
@@ -2105,8 +2123,13 @@ int kmx_sweep_batch_apply(const kmx_sweep_batch* sb, int slot, const BatchIO& io
         }
     }
     if (sb->njobs > 0) {
-        if (slot == 0) hipLaunchKernelGGL(k_box_sum_batch<double>, bg, bb, 0, s, sb->d_wgjobs, io);
-        else hipLaunchKernelGGL(k_box_sum_batch<float>, bg, bb, 0, s, sb->d_wgjobs, io);
+        if (one) {
+            if (slot == 0) hipLaunchKernelGGL((k_box_sum_batch<double, IoOne>), bg, bb, 0, s, sb->d_wgjobs, io1);
+            else hipLaunchKernelGGL((k_box_sum_batch<float, IoOne>), bg, bb, 0, s, sb->d_wgjobs, io1);
+        } else {
+            if (slot == 0) hipLaunchKernelGGL((k_box_sum_batch<double, BatchIO>), bg, bb, 0, s, sb->d_wgjobs, io);
+            else hipLaunchKernelGGL((k_box_sum_batch<float, BatchIO>), bg, bb, 0, s, sb->d_wgjobs, io);
+        }
     }
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) {

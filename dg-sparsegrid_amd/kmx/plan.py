@@ -24,12 +24,18 @@ def _prec(p):
     return int(p)
 
 
-def _torch():
-    import torch
+_TORCH = None  # torch, once a device was seen (the check is not repeated per call)
 
-    if not torch.cuda.is_available():
-        raise RuntimeError("kmx: no HIP device visible to torch")
-    return torch
+
+def _torch():
+    global _TORCH
+    if _TORCH is None:
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("kmx: no HIP device visible to torch")
+        _TORCH = torch
+    return _TORCH
 
 
 def _stream(stream):
