@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "libkmx.so")
-SOURCES = ["kmx_api.hip", "kmx_apply.hip", "kmx_apply_f64.hip", "kmx_apply_f32.hip", "kmx_kron.hip", "kmx_blocks.hip", "kmx_realspace.hip", "kmx_sweep.hip", "kmx_coeff.hip", "kmx_gmres.hip"]
+SOURCES = ["kmx_api.hip", "kmx_apply.hip", "kmx_apply_f64.hip", "kmx_apply_f32.hip", "kmx_kron.hip", "kmx_blocks.hip", "kmx_realspace.hip", "kmx_sweep.hip", "kmx_coeff.hip", "kmx_gmres.hip", "kmx_assemble.hip"]
 ARCH = "gfx950"
 
 

@@ -4,6 +4,7 @@ Same names and argument meaning as the reference, executing on the gfx950
 library:
 
     ftmp = apply_A(pde, opts, A_data, f, deg, Vmax, Emax, imex_flag)   apply_A.m:1
+    A    = build_A(pde, opts, A_data, deg, imex_flag, sparse)          apply_A.m:170-180, 246-248
     f    = time_advance(pde, opts, A_data, f, t, dt, deg, hash_table)   time_advance.m:1
     Meval, nodes = matrix_plot_D(pde, opts, dimension)                matrix_plot_D.m:1
     fr   = wavelet_to_realspace(pde, opts, Meval_D, f, hash_table)      wavelet_to_realspace.m:1
@@ -99,6 +100,41 @@ def apply_A(pde, opts, A_data, f, deg, Vmax=None, Emax=None, imex_flag="N"):
     else:
         y = plan.apply(x, prec=opts.prec)
     return y.cpu().numpy() if host else y
+
+
+def build_A(pde, opts, A_data, deg, imex_flag="N", sparse=True):
+    """apply_A.m's build_A branch (apply_A.m:150-155, 170-180, 246-248): the
+    assembled operator of the terms whose imex flag matches, from the device's
+    per-pair blocks (kmx_plan_assemble).  sparse=True gives what use_sparse_A
+    builds, sparse(A_s2, A_s1, A_s3, dof, dof), as a scipy CSC matrix; False the
+    dense A."""
+    plan = _plan_for(pde, opts, A_data, deg, imex_flag)
+    N = len(np.asarray(A_data["element_local_index_D"]))
+    B = deg ** np.asarray(A_data["element_local_index_D"]).reshape(N, -1).shape[1]
+    dof = N * B
+    if plan is None:
+        if sparse:
+            from scipy import sparse as sp
+            return sp.csc_matrix((dof, dof))
+        return np.zeros((dof, dof))
+    blocks = plan.assemble().cpu().numpy()  # [q, c, r]
+    if opts.use_connectivity:
+        rp, cols = pde.connectivity
+        rp = np.asarray(rp, dtype=np.int64)
+        e = np.repeat(np.arange(N), np.diff(rp))
+        j = np.asarray(cols, dtype=np.int64) - 1
+    else:
+        e = np.repeat(np.arange(N), N)
+        j = np.tile(np.arange(N), N)
+    r = np.arange(B)
+    rows = (e[:, None, None] * B + r[None, None, :]).repeat(B, axis=1)  # [q, c, r] -> e*B + r
+    colg = (j[:, None, None] * B + r[None, :, None]).repeat(B, axis=2)  # [q, c, r] -> j*B + c
+    if sparse:
+        from scipy import sparse as sp
+        return sp.csc_matrix((blocks.ravel(), (rows.ravel(), colg.ravel())), shape=(dof, dof))
+    A = np.zeros((dof, dof))
+    np.add.at(A, (rows.ravel(), colg.ravel()), blocks.ravel())
+    return A
 
 
 def time_advance(pde, opts, A_data, f, t, dt, deg, hash_table=None, Vmax=None, Emax=None):

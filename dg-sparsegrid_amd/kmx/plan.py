@@ -146,6 +146,15 @@ class Plan:
         check(lib().kmx_apply_transpose(self.handle, _ptr(x), _ptr(y), _prec(prec), _stream(stream)))
         return y
 
+    def assemble(self, stream=None):
+        """The assembled operator's blocks (kmx_plan_assemble; apply_A.m's build_A
+        branch): a (P, B, B) FP64 device tensor, blocks[q, c, r] = entry (r, c) of
+        pair q's block (column-major per block, A_s3's order)."""
+        torch = _torch()
+        out = torch.empty((self.P, self.B, self.B), dtype=torch.float64, device="cuda")
+        check(lib().kmx_plan_assemble(self.handle, _ptr(out) if self.P else C.c_void_p(0), _stream(stream)))
+        return out
+
     def apply_host(self, x, prec="fp64"):
         """numpy in, numpy out (copies through the library's staging buffers)."""
         x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))

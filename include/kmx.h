@@ -170,6 +170,18 @@ int kmx_apply(kmx_plan* plan, const double* x, double* y, int prec, void* stream
  * then runs with the transposed combined 1D matrices). */
 int kmx_apply_transpose(kmx_plan* plan, const double* x, double* y, int prec, void* stream);
 
+/* The assembled operator of apply_A.m's build_A branch (apply_A.m:150-155 and
+ * 170-180, sparse triplets at 246-248; krond.m:7-17), on the device:
+ * blocks[q * B*B + c * B + r] = sum_t scale_t krond(A_{t,1}(I_1, J_1), ...,
+ * A_{t,d}(I_d, J_d)) at entry (r, c) of pair q's block, B = deg^d -- the values
+ * A_s3 collects, in its order.  Pairs in the plan's order: rows in element order,
+ * a row's pairs as given at plan creation (all-pairs plans: q = e * N + j); the
+ * entry's global row / column are e * B + r and j * B + c (0-based).  Each entry
+ * multiplies its factors dimension by dimension and adds the terms in order, as
+ * krond and A = A + view do.  blocks: DEVICE pointer to num_pairs * B^2 doubles.
+ * Asynchronous on `stream`. */
+int kmx_plan_assemble(kmx_plan* plan, double* blocks, void* stream);
+
 /* Host-pointer convenience for callers without device memory (a MEX gateway):
  * copies x in, applies, copies y out, synchronises. */
 int kmx_apply_host(kmx_plan* plan, const double* x_host, double* y_host, int prec);
