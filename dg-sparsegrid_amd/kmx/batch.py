@@ -171,26 +171,64 @@ class BatchRK3Graph:
         return f.clone().addcmul_(self.b1, k1).addcmul_(self.b2, k2).addcmul_(self.b3, k3)
 
 
+def balanced_groups(plans, k, prec="fp32"):
+    """The plans' indices in k groups of about equal work (sweep flops plus the
+    stage launches' per-value traffic), heaviest first to the lightest group."""
+    w = [p.sweep(prec)["flops"] + 4.0 * p.dof for p in plans]
+    load, groups = [0.0] * k, [[] for _ in range(k)]
+    for i in sorted(range(len(plans)), key=lambda i: -w[i]):
+        j = min(range(k), key=lambda j: load[j])
+        groups[j].append(i)
+        load[j] += w[i]
+    return [sorted(g) for g in groups if g]
+
+
 class BatchRK3Fused:
     """steps_per_replay fused RK3 steps of every instance (kmx_batch_rk3: per
     stage one merged-sweep launch and one stage launch) captured in a HIP graph
     and replayed on the system's state.  Needs every instance on the merged
-    separable sweep for prec."""
+    separable sweep for prec.  split > 1 runs the instances as that many batches
+    of about equal work on streams of their own inside the graph, so one group's
+    stage launch (bound by HBM) can overlap another's op launch (bound by its
+    texture path); each instance's arithmetic is the same, so the states are
+    identical to split = 1."""
 
-    def __init__(self, system, prec="fp32", steps_per_replay=1):
+    def __init__(self, system, prec="fp32", steps_per_replay=1, split=1):
         import torch
 
         self.s, self.prec, self.n = system, prec, int(steps_per_replay)
+        dts = [inst["dt"] for inst in system.instances]
+        if split > 1:
+            self.groups = balanced_groups(system.plans, int(split), prec)
+            self.batches = []
+            for g in self.groups:
+                b = Batch([system.plans[i] for i in g])
+                b.set_sweep(prec, True)
+                self.batches.append(b)
+        else:
+            self.groups, self.batches = [list(range(len(system.plans)))], [system.batch]
+        self.streams = [torch.cuda.Stream() for _ in self.batches]
+
+        def run(f, n):
+            views = system.views(f)
+            cur = torch.cuda.current_stream()
+            for b, g, st in zip(self.batches, self.groups, self.streams):
+                st.wait_stream(cur)
+                with torch.cuda.stream(st):
+                    b.rk3([views[i] for i in g], [dts[i] for i in g], n, prec=prec)
+            for st in self.streams:
+                cur.wait_stream(st)
+
         scratch = system.f.clone()
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
-            system.rk3(1, prec, f=scratch)  # allocates the stage workspace outside the capture
+            run(scratch, 1)  # allocates the stage workspaces outside the capture
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            system.rk3(self.n, prec)
+            run(system.f, self.n)
 
     def step(self):
         self.graph.replay()

@@ -244,3 +244,13 @@ def test_fused_rk3_matches_applies(kmx_gpu, prec):
         e = rel(fused.f.cpu().numpy(), ref.f.cpu().numpy())
         print(f"fused rk3 {prec} after {2 * (r + 1)} steps: {e:.2e} vs the fp64 pair path", flush=True)
         assert e <= tol, (r, e)
+    # the instances as two batches on streams of their own: the same states
+    g2 = BT.BatchRK3Fused(fused, prec=prec, steps_per_replay=2, split=2)
+    assert len(g2.batches) == 2 and sorted(i for grp in g2.groups for i in grp) == list(range(len(inst)))
+    f_before = fused.f.clone()
+    g.step()
+    one = fused.f.clone()
+    fused.f.copy_(f_before)
+    g2.step()
+    torch.cuda.synchronize()
+    assert torch.equal(fused.f, one)
