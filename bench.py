@@ -246,6 +246,7 @@ def batched_leg(steps=50, warmup=5, inst=None, err_steps=6):
         sysm = BT.BatchedSystem(inst, sweep=sweep)
         setup_s = time.perf_counter() - t1
         pairs = sweep is None
+        tuned_wb = None if pairs else sysm.batch.autotune(prec)  # load batches per wave of the merged sweep
         apply_ms = time_batch_apply(sysm, prec, 1 if pairs else 3, 3 if pairs else max(10, steps // 2),
                                     per_graph=1 if pairs else 10)
         # pair paths: RK3 from three batched applies and FP64 vector updates per step;
@@ -253,14 +254,14 @@ def batched_leg(steps=50, warmup=5, inst=None, err_steps=6):
         g = BT.BatchRK3Graph(sysm, prec=prec) if pairs else BT.BatchRK3Fused(sysm, prec=prec)
         ms_err = timed_steps(g, err_steps)
         states[name] = sysm.f.cpu().numpy()
-        run = {"setup_s": round(setup_s, 1), "ms_per_batched_apply": round(apply_ms, 4),
+        run = {"setup_s": round(setup_s, 1), "ms_per_batched_apply": round(apply_ms, 4), "wave_batches": tuned_wb,
                "flops_per_batched_apply": sysm.flops_by_prec[prec]}
         if pairs:
             run["ms_per_rk3_step"] = round(ms_err, 4)
             run["timed_steps"] = err_steps
         else:
             spr = max(d for d in range(1, STEPS_PER_REPLAY + 1) if steps % d == 0)  # exactly `steps` timed
-            gk = BT.BatchRK3Fused(sysm, prec=prec, steps_per_replay=spr, split=RK3_STREAM_GROUPS)
+            gk = BT.BatchRK3Fused(sysm, prec=prec, steps_per_replay=spr, split=RK3_STREAM_GROUPS, autotune=True)
             for _ in range(max(1, warmup // spr)):
                 gk.step()
             ms = timed_steps(gk, steps // spr) / spr
@@ -268,6 +269,7 @@ def batched_leg(steps=50, warmup=5, inst=None, err_steps=6):
             run["timed_steps"] = steps
             run["steps_per_graph_replay"] = spr
             run["stream_groups"] = RK3_STREAM_GROUPS
+            run["wave_batches_autotuned"] = gk.wave_batches
             run["merged_sweep"] = sysm.sweep[prec]
             if name == "fp32_sweep":  # the same steps unfused: applies + torch vector updates
                 gu = BT.BatchRK3Graph(sysm, prec=prec)

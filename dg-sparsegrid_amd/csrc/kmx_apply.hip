@@ -369,6 +369,8 @@ struct kmx_batch {
     bool sweep_on[2] = {false, false};
     // plan versions the tables above were built from (kmx_plan::version)
     std::vector<unsigned long long> version;
+    // load batches per wave of the merged sweeps (0: the defaults; kmx_batch_autotune)
+    int wave_batches[2] = {0, 0};
 };
 
 // The switch's register budget is that of its largest case: blocks of up to 27
@@ -499,7 +501,7 @@ static int build_batch_sweep(kmx_batch* b, int slot) {
         return KMX_E_UNSUPPORTED;
     }
     try {
-        b->sweep[slot] = kmx_sweep_batch_build(mp);
+        b->sweep[slot] = kmx_sweep_batch_build(mp, b->wave_batches[slot]);
     } catch (const std::exception& ex) {
         set_error(std::string("kmx_batch_set_sweep: ") + ex.what());
         return KMX_E_HIP;
@@ -638,6 +640,93 @@ extern "C" int kmx_batch_set_sweep(kmx_batch* b, int prec, int on) {
         return KMX_E_HIP;
     }
     return KMX_OK;
+}
+
+extern "C" int kmx_batch_autotune(kmx_batch* b, int prec, int* wave_batches, void* stream) {
+    if (!b || (prec != KMX_PREC_FP64 && prec != KMX_PREC_FP32 && prec != KMX_PREC_BF16)) {
+        set_error("kmx_batch_autotune: bad arguments");
+        return KMX_E_ARG;
+    }
+    const int slot = prec_slot(prec);
+    if (!b->sweep_on[slot]) {
+        set_error("kmx_batch_autotune: the merged sweep is off for this precision (kmx_batch_set_sweep)");
+        return KMX_E_UNSUPPORTED;
+    }
+    {
+        const int rc = refresh_batch(b);
+        if (rc != KMX_OK) return rc;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // scratch x / y of every plan (zero x: the timing does not depend on the values)
+    std::vector<double*> xs(b->plans.size(), nullptr), ys(b->plans.size(), nullptr);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = KMX_OK;
+    auto cleanup = [&]() {
+        for (double* p : xs) if (p) (void)hipFree(p);
+        for (double* p : ys) if (p) (void)hipFree(p);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    for (size_t i = 0; i < b->plans.size(); ++i) {
+        const size_t n = std::max<size_t>(1, size_t(b->plans[i]->N) * b->plans[i]->B);
+        if (hipMalloc(&xs[i], n * sizeof(double)) != hipSuccess || hipMalloc(&ys[i], n * sizeof(double)) != hipSuccess ||
+            hipMemset(xs[i], 0, n * sizeof(double)) != hipSuccess) {
+            cleanup();
+            set_error("kmx_batch_autotune: out of device memory");
+            return KMX_E_HIP;
+        }
+    }
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        cleanup();
+        set_error("kmx_batch_autotune: event set-up failed");
+        return KMX_E_HIP;
+    }
+    // load batches per wave before a tile is split: 4, 8 (the separable default), 16
+    const int cand[3] = {4, 8, 16};
+    int best = b->wave_batches[slot];
+    float best_ms = 0.0f;
+    for (int c : cand) {
+        b->wave_batches[slot] = c;
+        rc = build_batch_sweep(b, slot);
+        if (rc != KMX_OK) break;
+        float ms = 0.0f;
+        for (int round = 0; round < 3 && rc == KMX_OK; ++round) {  // best of three rounds of 10 applies
+            for (int w = 0; w < (round ? 0 : 3) && rc == KMX_OK; ++w) rc = kmx_batch_apply(b, xs.data(), ys.data(), prec, s);
+            if (rc != KMX_OK) break;
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < 10 && rc == KMX_OK; ++r) rc = kmx_batch_apply(b, xs.data(), ys.data(), prec, s);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                rc = KMX_E_HIP;
+                set_error("kmx_batch_autotune: timing failed");
+                break;
+            }
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, e0, e1);
+            ms = round ? std::min(ms, t) : t;
+        }
+        if (rc != KMX_OK) break;
+        if (best_ms == 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    if (rc == KMX_OK) {
+        b->wave_batches[slot] = best;
+        rc = build_batch_sweep(b, slot);
+    }
+    if (rc == KMX_OK) {
+        free_set(b->set[slot]);
+        const hipError_t e = slot == 0 ? build_set<double>(b, b->set[0], true) : build_set<float>(b, b->set[1], false);
+        if (e != hipSuccess) {
+            set_error(std::string("kmx_batch_autotune: ") + hipGetErrorString(e));
+            rc = KMX_E_HIP;
+        }
+    }
+    (void)hipDeviceSynchronize();
+    cleanup();
+    if (rc == KMX_OK && wave_batches) *wave_batches = best;
+    return rc;
 }
 
 extern "C" int kmx_batch_rk3(kmx_batch* b, double* const* fs, const double* dts, int nsteps, int prec, void* stream) {

@@ -149,6 +149,7 @@ PROTOTYPES = {
     "kmx_batch_set_sweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "kmx_batch_get_sweep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
+    "kmx_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_void_p]),
     "kmx_batch_rk3": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_double), C.c_int, C.c_int,
                                 C.c_void_p]),
     "kmx_kronmult": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.c_void_p,

@@ -191,9 +191,10 @@ class BatchRK3Fused:
     of about equal work on streams of their own inside the graph, so one group's
     stage launch (bound by HBM) can overlap another's op launch (bound by its
     texture path); each instance's arithmetic is the same, so the states are
-    identical to split = 1."""
+    identical to split = 1.  autotune: time each batch's load batches per wave
+    first (kmx_batch_autotune)."""
 
-    def __init__(self, system, prec="fp32", steps_per_replay=1, split=1):
+    def __init__(self, system, prec="fp32", steps_per_replay=1, split=1, autotune=False):
         import torch
 
         self.s, self.prec, self.n = system, prec, int(steps_per_replay)
@@ -207,6 +208,8 @@ class BatchRK3Fused:
                 self.batches.append(b)
         else:
             self.groups, self.batches = [list(range(len(system.plans)))], [system.batch]
+        # each batch's load batches per wave timed on itself (kmx_batch_autotune)
+        self.wave_batches = [b.autotune(prec) for b in self.batches] if autotune else None
         self.streams = [torch.cuda.Stream() for _ in self.batches]
 
         def run(f, n):

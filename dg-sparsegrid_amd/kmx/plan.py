@@ -238,6 +238,13 @@ class Batch:
         check(lib().kmx_batch_get_sweep(self._h, _prec(prec), C.byref(m), C.byref(a), C.byref(ph)))
         return {"members": m.value, "active": bool(a.value), "phases": ph.value}
 
+    def autotune(self, prec="fp32", stream=None):
+        """Keep the fastest load batches per wave of the merged sweep for prec
+        (kmx_batch_autotune); returns the choice.  Not under capture."""
+        wb = C.c_int()
+        check(lib().kmx_batch_autotune(self._h, _prec(prec), C.byref(wb), _stream(stream)))
+        return wb.value
+
     def apply(self, xs, ys, prec="fp32", stream=None):
         n = len(self.plans)
         px = (C.c_void_p * n)(*[_ptr(x).value for x in xs])

@@ -213,6 +213,14 @@ int kmx_batch_destroy(kmx_batch* batch);
 int kmx_batch_set_sweep(kmx_batch* batch, int prec, int on);
 int kmx_batch_get_sweep(const kmx_batch* batch, int prec, int* members, int* active, int* phases);
 
+/* Times the batch's merged sweep for prec (on, kmx_batch_set_sweep) with 4, 8 and 16
+ * load batches per wave before a tile is split over more waves (best of three
+ * rounds of 10 applies on zero vectors each), keeps the fastest and returns it in
+ * *wave_batches (may be null).  Rebuilds the sweep's tables (synchronous; not under
+ * graph capture: graphs captured before must be captured again).  No counterpart in
+ * the reference. */
+int kmx_batch_autotune(kmx_batch* batch, int prec, int* wave_batches, void* stream);
+
 /* nsteps explicit RK3 steps of every plan of the batch, y' = A y (the Kutta RK3
  * of time_advance.m:151-204 -- a21 = 1/2, a31 = -1, a32 = 2, b = 1/6, 2/3, 1/6 --
  * without source or boundary terms; contractions in prec, states and stage

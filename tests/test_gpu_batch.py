@@ -174,3 +174,26 @@ def test_graph_rk3_batched_sweep(kmx_gpu):
     err = rel(s32.f.cpu().numpy(), ref.f.cpu().numpy())
     print(f"fp32 merged-sweep RK3 vs fp64 pairs after 8 steps: {err:.3e}")
     assert err <= 1e-6
+
+
+def test_batch_autotune_keeps_results(kmx_gpu):
+    """kmx_batch_autotune picks one of the timed load-batch counts for the merged
+    sweep and the batched apply still matches the oracle per instance (FP32 bar)."""
+    import torch
+
+    from kmx import batch as BT
+
+    inst = BT.make_instances(count=8, seed=4, lev2d=(4, 6), lev3d=(3, 4))
+    sysm = BT.BatchedSystem(inst, sweep=("fp32",))
+    x = torch.from_numpy(np.random.default_rng(6).standard_normal(sysm.dof)).cuda()
+    y0 = sysm.apply(x, torch.empty_like(x), "fp32").cpu().numpy()
+    wb = sysm.batch.autotune("fp32")
+    assert wb in (4, 8, 16)
+    y1 = sysm.apply(x, torch.empty_like(x), "fp32").cpu().numpy()
+    assert rel(y1, y0) <= 1e-6
+    xs = x.cpu().numpy()
+    for i, (p, it) in enumerate(zip(sysm.plans, inst)):
+        lo, hi = sysm.offsets[i], sysm.offsets[i + 1]
+        mats = [[m * s if d == 0 else m for d, m in enumerate(ms)] for ms, s in zip(it["mats"], it["scales"])]
+        yo = O.apply_A(O.ApplyInputs(mats, it["table"].idx1d, it["deg"]), xs[lo:hi], nthreads=8)
+        assert rel(y1[lo:hi], yo) <= 5e-6, i
