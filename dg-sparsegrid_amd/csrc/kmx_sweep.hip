@@ -965,10 +965,11 @@ void panel_prefix(const SepSide& sd, int k, int p, int m, int& nrun, int& steps)
 // Work records of every op of a sweep.  A tile (16 rows x 16 columns x nbr
 // branches) whose K steps exceed wave_batches load batches is split over up to
 // UD_WAVES waves of one workgroup: a lone plan (latency-bound, few tiles) uses
-// one batch per wave, a batch of plans (many tiles) four (general sweeps) or eight
-// (separable ones).  Widest tiles first
+// one batch per wave, a batch of plans (many tiles) four (64-instance batch, FP32
+// apply: 0.1028 ms at four, 0.1046 at three, 0.1109 at two, 0.1079 at eight;
+// kmx_batch_autotune times 4, 8 and 16 on the batch itself).  Widest tiles first
 // (power-of-two wave counts then never straddle a workgroup), longest first.
-// Separable ops take their steps from the runs of orientation `side`.
+// Separable ops take their steps from the panels of orientation `side`.
 void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector<int>>& recs,
                   std::vector<kmx_sweep::Op>& ops, int side = 0) {
     const int deg = sw.deg;
@@ -1918,8 +1919,8 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
     for (const kmx_plan* p : members)
         if (transpose && (!p || !p->sw || !p->sw->separable))
             throw std::invalid_argument("the adjoint sweep needs the separable form");
-    // load batches per wave (0: a batch's defaults): separable members eight, general members four
-    const int wb_sep = wave_batches > 0 ? wave_batches : sweep_wave_batches(8);
+    // load batches per wave (0: the batch default, four)
+    const int wb_sep = wave_batches > 0 ? wave_batches : sweep_wave_batches(4);
     const int wb_gen = wave_batches > 0 ? wave_batches : sweep_wave_batches(4);
     const int side = transpose ? 1 : 0;
     for (size_t q = 0; q < members.size(); ++q) {
