@@ -8,8 +8,8 @@
 // Four launches, each over the same G contiguous chunks of the vectors, one
 // workgroup per chunk: the dot products of a pass leave one partial per
 // (chunk, row), and every workgroup of the next launch sums the G partials of
-// each row itself (G <= 64: at most 64 x 128 values, from L2), so no launch waits
-// on a separate reduction.  Every launch has a fixed shape for given (n, k) and
+// each row itself (G <= 128: at most 128 x 128 values, from L2, four threads per
+// row each summing a quarter), so no launch waits on a separate reduction.  Every launch has a fixed shape for given (n, k) and
 // allocates nothing, so the step can be captured in a graph.
 #include "kmx_internal.h"
 
@@ -19,7 +19,7 @@ namespace kmx {
 namespace {
 
 constexpr int AR_THREADS = 256;
-constexpr int AR_MAX_G = 64;
+constexpr int AR_MAX_G = 128;
 
 struct ArnoldiArgs {
     const double* u;
@@ -55,6 +55,7 @@ __device__ __forceinline__ void chunk_dots(const ArnoldiArgs& a, long long i0, l
         double s[AR_RB];
 #pragma unroll
         for (int r = 0; r < AR_RB; ++r) s[r] = 0.0;
+#pragma unroll 2
         for (long long i = i0 + lane; i < i1; i += 64) {
             const double wi = a.w[i];
 #pragma unroll
@@ -72,15 +73,28 @@ __device__ __forceinline__ void chunk_dots(const ArnoldiArgs& a, long long i0, l
     }
 }
 
-// the G partials of each row summed, into LDS
+// the G partials of each row summed, into LDS: rows 64 at a time, each by four
+// threads that sum every fourth partial, their sums added in slice order
 __device__ __forceinline__ void sum_partials(const ArnoldiArgs& a, const double* part, double* sh) {
-    for (int j = threadIdx.x; j <= a.k; j += AR_THREADS) {
+    __shared__ double red[AR_WAVES][64];
+    const int lane = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    for (int j0 = 0; j0 <= a.k; j0 += 64) {
+        const int j = j0 + lane;
         double s = 0.0;
+        if (j <= a.k) {
 #pragma unroll 8
-        for (int g = 0; g < a.G; ++g) s += part[(long long)g * (a.k + 1) + j];
-        sh[j] = s;
+            for (int g = sl; g < a.G; g += AR_WAVES) s += part[(long long)g * (a.k + 1) + j];
+        }
+        red[sl][lane] = s;
+        __syncthreads();
+        if (sl == 0 && j <= a.k) {
+            double t = red[0][lane];
+#pragma unroll
+            for (int q = 1; q < AR_WAVES; ++q) t += red[q][lane];
+            sh[j] = t;
+        }
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 // w -= V^T h over this workgroup's chunk
@@ -126,21 +140,23 @@ __global__ void __launch_bounds__(AR_THREADS) k_arnoldi_third(ArnoldiArgs a) {
         for (int q = 0; q < AR_THREADS / 64; ++q) t += red[q];
         a.part3[blockIdx.x] = t;
     }
-    if (blockIdx.x == 0)  // h = the two passes' coefficients (part1, part2 are complete here)
-        for (int j = threadIdx.x; j <= a.k; j += AR_THREADS) {
-            double s1 = 0.0;
-            for (int g = 0; g < a.G; ++g) s1 += a.part1[(long long)g * (a.k + 1) + j];
-            a.h[j] = s1 + sh[j];
-        }
+    if (blockIdx.x == 0) {  // h = the two passes' coefficients (part1, part2 are complete here)
+        __shared__ double sh1[KMX_ARNOLDI_MAX_K + 1];
+        sum_partials(a, a.part1, sh1);
+        for (int j = threadIdx.x; j <= a.k; j += AR_THREADS) a.h[j] = sh1[j] + sh[j];
+    }
 }
 
 __global__ void __launch_bounds__(AR_THREADS) k_arnoldi_normalize(ArnoldiArgs a) {
     __shared__ double nrm;
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int g = 0; g < a.G; ++g) t += a.part3[g];
-        nrm = sqrt(t);
-        if (blockIdx.x == 0) a.h[a.k + 1] = nrm;
+    if (threadIdx.x < 64) {  // the G norm partials, two per lane, summed across the wave
+        const int l = threadIdx.x;
+        double t = (l < a.G ? a.part3[l] : 0.0) + (l + 64 < a.G ? a.part3[l + 64] : 0.0);
+        t = wave_sum(t);
+        if (l == 0) {
+            nrm = sqrt(t);
+            if (blockIdx.x == 0) a.h[a.k + 1] = nrm;
+        }
     }
     __syncthreads();
     const long long i0 = (long long)blockIdx.x * a.chunk, i1 = chunk_end(a, i0);

@@ -26,7 +26,7 @@ ARNOLDI_MAX_K = 127
 
 def arnoldi_work(n):
     """KMX_ARNOLDI_WORK(n): doubles of caller-owned scratch for kmx_arnoldi_step."""
-    return int(n) + 16448
+    return int(n) + 32896
 
 
 PLAN_FP32 = 1

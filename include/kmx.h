@@ -248,7 +248,7 @@ int kmx_batch_rk3(kmx_batch* batch, double* const* fs, const double* dts, int ns
  * launches of fixed shape for given (n, k), asynchronous on `stream` (graph
  * capturable).  0 <= k <= KMX_ARNOLDI_MAX_K. */
 #define KMX_ARNOLDI_MAX_K 127
-#define KMX_ARNOLDI_WORK(n) ((n) + 16448)
+#define KMX_ARNOLDI_WORK(n) ((n) + 32896)
 int kmx_arnoldi_step(int64_t n, int k, double* V, const double* u, double s0, double s1, double* h, double* work,
                      void* stream);
 
