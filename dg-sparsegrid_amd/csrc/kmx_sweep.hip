@@ -136,6 +136,32 @@ __device__ __forceinline__ int4 gld(const int4* p) {
     return make_int4(v.x, v.y, v.z, v.w);
 }
 
+// Four consecutive values of one lane, in one vector load (the box values of a step
+// are only element-aligned: gfx950 loads unaligned vectors)
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+    typedef float v __attribute__((ext_vector_type(4), aligned(4)));
+};
+template <>
+struct Vec4<double> {
+    typedef double v __attribute__((ext_vector_type(4), aligned(8)));
+};
+template <>
+struct Vec4<unsigned short> {
+    typedef unsigned short v __attribute__((ext_vector_type(4), aligned(2)));
+};
+template <typename T>
+__device__ __forceinline__ void gld4(const T* p, T out[4]) {
+    typedef typename Vec4<T>::v V;
+    const V x = *(const __attribute__((address_space(1))) V*)p;
+    out[0] = x[0];
+    out[1] = x[1];
+    out[2] = x[2];
+    out[3] = x[3];
+}
+
 // positions of level l along a dimension: deg * cells(l)
 __device__ __forceinline__ int deg_cells(int deg, int l) { return deg * (l ? 1 << (l - 1) : 1); }
 
@@ -553,6 +579,7 @@ struct UdBatchOp {
     const int* codes;  // separable ops: the plan's step codes (forward or transposed)
     int inst;
     int kind;
+    int quad;  // separable op of the last dimension in the quad storage (sep_tile_quad)
 };
 
 template <typename T, typename TA, int NB>
@@ -562,6 +589,10 @@ __device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int* 
 template <typename T, typename TA>
 __device__ __forceinline__ void sep_tile_cols(const TA* __restrict__ mat, const int* __restrict__ codes,
                                               T* __restrict__ arena, const int2 rv, const int lane, const UdOp& op);
+template <typename T, typename TA, int NG>
+__device__ __forceinline__ void sep_tile_quad(const TA* __restrict__ mat, const int* __restrict__ codes,
+                                              T* __restrict__ arena, const int2 rv, const int w, const int lane,
+                                              const UdOp& op, T* __restrict__ part);
 
 // One phase of a batch: ops of many instances, each on whole workgroups
 // (wg_op[workgroup] = its op); the workgroup's op kind picks the tile body.
@@ -592,7 +623,9 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
             break;
         case UK_UP_FINAL: ud_tile<T, UD_UP, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
         case UK_SEP:
-            if (__builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
+            if (o.quad)
+                sep_tile_quad<T, T, 2>(mat, o.codes, arena, rv, w, lane, o.a, part);
+            else if (__builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
                 sep_tile_cols<T, T>(mat, o.codes, arena, rv, lane, o.a);
             else
                 sep_tile<T, T, 8>(mat, o.codes, arena, rv, w, lane, o.a, part);
@@ -691,6 +724,90 @@ __device__ __forceinline__ void sep_tile(const TA* __restrict__ mat, const int* 
                     }
 #pragma unroll
                     for (int c = 0; c < NB; c += 4) acc = SepMma<T, TA>::run4(av + c, bv + c, acc);
+                }
+            }
+        }
+        if (gsize > 1 && !leader)
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) part[((w - 1) * 4 + qq) * 64 + lane] = acc[qq];
+    }
+    __syncthreads();  // every wave of the workgroup
+    if (!leader) return;
+    for (int v = 1; v < gsize; ++v)  // the tile's partial sums in wave order
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) acc[qq] += part[((w + v - 1) * 4 + qq) * 64 + lane];
+    T* out = arena + op.out_base;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq)
+        if (ook[qq]) gst(out + oaddr[qq], acc[qq]);
+}
+
+// One wave of a separable op of the last dimension (inner extent I = 1, the op's
+// "quad" storage): the tile of sep_tile with the matrix operations' k index taken
+// as (step of a group of four, position inside the step) instead of (position,
+// step).  A panel's steps come in groups of four (zero steps pad the panel); the
+// storage keeps lane (li, q)'s values A(row li, positions 0..3 of step q) of a group
+// together, so one vector load per lane brings a group's operator values, and at
+// I = 1 a step's four positions are consecutive box values of one column, so one
+// vector load per lane (column li, step q) brings its B values: operation j of the
+// four multiplies position j of every step of the group.  The wave's share
+// [slo, shi) starts at a group.
+template <typename T, typename TA, int NG>
+__device__ __forceinline__ void sep_tile_quad(const TA* __restrict__ mat, const int* __restrict__ codes,
+                                              T* __restrict__ arena, const int2 rv, const int w, const int lane,
+                                              const UdOp& op, T* __restrict__ part) {
+    using V = typename SwMfma<T>::V;
+    const int gsize = __builtin_amdgcn_readlane(rv.x, R_GSIZE);
+    const int gbase = __builtin_amdgcn_readlane(rv.x, R_GBASE);
+    const bool leader = gsize > 0 && w == gbase;
+    V acc = V{T(0), T(0), T(0), T(0)};
+    int oaddr[4] = {0, 0, 0, 0};
+    bool ook[4] = {false, false, false, false};
+    if (gsize > 0) {
+        const int f0 = __builtin_amdgcn_readlane(rv.x, R_F0);
+        const int O = __builtin_amdgcn_readlane(rv.x, R_O);
+        const int li = lane & 15, lk = lane >> 4;
+        const int f = f0 + li;
+        const bool fok = f < O;
+        const int fo = fok ? f : 0;  // I = 1: the column is o
+        if (leader)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = SwMfma<T>::row(lane, q);
+                const int rb = __shfl(rv.x, R_ROWBASE + row), rn = __shfl(rv.x, R_ROWNL + row);
+                const int rl = __shfl(rv.x, R_ROWLEV + row);
+                ook[q] = fok && rl >= 0;
+                oaddr[q] = ook[q] ? rb + fo * rn : 0;
+            }
+        const int tn = __builtin_amdgcn_readlane(rv.x, R_TN);
+        if (tn > 0) {
+            const int slo = __builtin_amdgcn_readlane(rv.x, R_SLO), shi = __builtin_amdgcn_readlane(rv.x, R_SHI);
+            const long long g0 = op.a_off[0] / 64 + __builtin_amdgcn_readlane(rv.x, R_STEP0) + slo;
+            const TA* amat = mat + 64 * g0 + 4 * lane;  // group g of the share at + 256 g
+            const int* cd = codes + g0;
+            const T* bsrc = arena + op.in_base;
+            const int fo_d = fo * op.deg;  // fo * nl at levels 0 and 1
+            const int ns = shi - slo;
+            for (int q0 = 0; q0 < ns; q0 += 64) {
+                const int qn = min(64, ns - q0);
+                const int cv = lane < qn ? gld(cd + q0 + lane) : 0;  // lane t: the code of step q0 + t
+                for (int q = 0; q < qn; q += 4 * NG) {
+                    TA av[NG][4];
+                    T bv[NG][4];
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) gld4(amat + 64 * (q0 + q + 4 * g), av[g]);
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+                        const int code = __shfl(cv, q + 4 * g + lk);  // step lk of group g
+                        const int nv = (code >> 10) & 7;
+                        const int box = __shfl(rv.y, code & 31);
+                        const unsigned ad = unsigned(box + (fo_d << ((code >> 5) & 31)) + (code >> 13));
+                        gld4(bsrc + (nv > 0 ? ad : 0u), bv[g]);
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) bv[g][j] = j < nv ? bv[g][j] : T(0);
+                    }
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) acc = SepMma<T, TA>::run4(av[g], bv[g], acc);
                 }
             }
         }
@@ -809,7 +926,9 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_sep_batch(const UdBatchOp* __
     constexpr int sl = sizeof(T) == 8 ? 0 : 1;            // arena
     constexpr int ml = sizeof(TA) == 2 ? 2 : sl;          // values
     const TA* mat = static_cast<const TA*>(o.mat[ml]);
-    if (__builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
+    if (o.quad)
+        sep_tile_quad<T, TA, 2>(mat, o.codes, static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
+    else if (__builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
         sep_tile_cols<T, TA>(mat, o.codes, static_cast<T*>(o.arena[sl]), rv, lane, o.a);
     else
         sep_tile<T, TA, 8>(mat, o.codes, static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
@@ -868,6 +987,9 @@ struct HostTile {
 struct SepSide {
     std::vector<int4> runs;
     std::vector<int> codes;
+    // per op: the quad storage (the last dimension's op, I = 1): each panel's steps
+    // padded to a multiple of four, a group of four steps stored [lane][position]
+    std::vector<char> quad;
     std::vector<int> run_base;
     std::vector<std::vector<int>> prun;
     std::vector<long long> a_base;
@@ -943,6 +1065,13 @@ int sweep_wave_batches(int dflt) {
     return dflt;
 }
 
+// The quad storage for the separable op of the last dimension (KMX_SEP_QUAD=0: the
+// plain storage for every op).
+bool sep_quad_on() {
+    const char* e = std::getenv("KMX_SEP_QUAD");
+    return !(e && e[0] == '0');
+}
+
 double env_double(const char* name, double dflt) {
     if (const char* e = std::getenv(name)) return std::atof(e);
     return dflt;
@@ -985,6 +1114,7 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
         const int nbr = int(std::min<long long>(nbranch, sw.max_nbr));  // branches per tile (a power of two)
         kmx_sweep::Op op{part, fin, od.a, int(recs.size() / UD_WAVES), 0, nbr, od.terms};
         if (sep) op.a.a_off[0] = sw.side[side].a_base[size_t(sep_k)];
+        const bool quad = sep && sw.side[side].quad[size_t(sep_k)] != 0;
         const int d = od.a.d;
         const bool multi = fin || sep;  // one tile sums every term of the op
         const int nterm = multi ? T : 1;
@@ -1012,7 +1142,7 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
                         }
                         const int steps = nterm * sper;
                         const int ctiles = (g.O * g.I + 15) / 16;
-                        if (sep && sper >= 1 && sper <= SEP_KMAX && ctiles > 1) {
+                        if (sep && !quad && sper >= 1 && sper <= SEP_KMAX && ctiles > 1) {
                             // a short panel: one wave sweeps up to batch / steps column tiles
                             const int per = std::max(1, std::min(ctiles, batch / sper));
                             for (int c0 = 0; c0 < ctiles; c0 += per) {
@@ -1036,7 +1166,8 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
             if (slot % h.gsize) throw std::logic_error("sweep tile straddles a workgroup");
             const Group& g = sw.groups[h.group];
             const int n = deg << g.m;
-            const int qn = (h.steps + h.gsize - 1) / h.gsize;
+            int qn = (h.steps + h.gsize - 1) / h.gsize;
+            if (quad) qn = (qn + 3) / 4 * 4;  // shares start at a group of four steps
             for (int w = 0; w < h.gsize; ++w) {
                 std::vector<int> r(UD_REC, 0);
                 r[R_M0] = h.m0;
@@ -1190,6 +1321,8 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
                         r[R_SHI] < r[R_SLO] || r[R_SHI] > steps)
                         throw std::logic_error("sweep separable record steps");
                     const int nct = r[R_NCT];
+                    if (sd.quad[size_t(sep_k)] && (nct != 1 || r[R_SLO] % 4 || g.I != 1))
+                        throw std::logic_error("sweep quad record");
                     if (nct < 1 || r[R_F0] % 16 || r[R_F0] + 16 * (nct - 1) >= g.O * g.I ||
                         (nct > 1 && (gsize != 1 || r[R_SLO] != 0 || r[R_SHI] != steps || steps < 1 ||
                                      steps > SEP_KMAX || r[R_NRUN] > SEP_KMAX)))
@@ -1279,6 +1412,7 @@ void sep_runs(const kmx_plan* p, kmx_sweep& sw, int s) {
     SepSide& sd = sw.side[s];
     sd.runs.clear();
     sd.codes.clear();
+    sd.quad.clear();
     sd.run_base.clear();
     sd.prun.clear();
     sd.a_base.clear();
@@ -1292,6 +1426,8 @@ void sep_runs(const kmx_plan* p, kmx_sweep& sw, int s) {
         while ((deg << L) < n) ++L;
         if ((deg << L) != n) throw std::logic_error("separable op: matrix size is not deg * 2^L");
         const int npan = (n + 15) / 16;
+        const bool quad = sep_quad_on() && d == p->D - 1;
+        sd.quad.push_back(quad ? 1 : 0);
         sd.run_base.push_back(int(sd.runs.size()));
         sd.a_base.push_back(abase);
         std::vector<int> pr(size_t(npan) + 1, 0);
@@ -1299,6 +1435,7 @@ void sep_runs(const kmx_plan* p, kmx_sweep& sw, int s) {
         std::vector<unsigned char> colneed(static_cast<size_t>(nbk), 0);
         for (int pnl = 0; pnl < npan; ++pnl) {
             pr[size_t(pnl)] = int(sd.runs.size()) - sd.run_base.back();
+            const long long panel_a0 = aoff;
             const int I0 = (16 * pnl) / deg, I1 = std::min(n - 1, 16 * pnl + 15) / deg;
             for (int J = 0; J < nbk; ++J) {
                 bool any = false;
@@ -1332,6 +1469,8 @@ void sep_runs(const kmx_plan* p, kmx_sweep& sw, int s) {
                     s0 = s1;
                 }
             }
+            if (quad)  // zero steps (no positions) complete the panel's last group of four
+                for (; (aoff - panel_a0) % 256; aoff += 64) sd.codes.push_back(sep_code(0, 0, 0));
         }
         pr[size_t(npan)] = int(sd.runs.size()) - sd.run_base.back();
         sd.prun.push_back(std::move(pr));
@@ -1392,7 +1531,9 @@ void sep_values(const kmx_plan* p, kmx_sweep& sw, int s, int only = -1) {
         const long long lo = sd.a_base[k], hi = k + 1 < sd.a_base.size() ? sd.a_base[k + 1] : sd.a_elems;
         std::vector<double> v64(size_t(hi - lo), 0.0);
         const std::vector<int>& pr = sd.prun[k];
-        for (size_t pnl = 0; pnl + 1 < pr.size(); ++pnl)
+        const bool quad = sd.quad[k] != 0;
+        for (size_t pnl = 0; pnl + 1 < pr.size(); ++pnl) {
+            const long long panel_a0 = pr[pnl] < pr[pnl + 1] ? sd.runs[size_t(sd.run_base[k] + pr[pnl])].w : 0;
             for (int q = pr[pnl]; q < pr[pnl + 1]; ++q) {
                 const int4& ru = sd.runs[size_t(sd.run_base[k] + q)];
                 const long long k0 = p->deg * start_of(ru.x) + ru.y;
@@ -1404,9 +1545,15 @@ void sep_values(const kmx_plan* p, kmx_sweep& sw, int s, int only = -1) {
                         double v = 0.0;
                         for (int t : od.terms)
                             v += p->scale[t] * (s == 0 ? masked_entry(p, t, d, r, c) : masked_entry(p, t, d, c, r));
-                        v64[size_t(ru.w + 16 * j + li)] = v;
+                        // (row li, position j % 4 of step t): [step][position][row], or in the quad
+                        // storage [group t / 4][lane 16 (t % 4) + li][position]
+                        const long long t = (ru.w - panel_a0) / 64 + j / 4;
+                        const long long at = quad ? panel_a0 + 256 * (t / 4) + 4 * (16 * (t % 4) + li) + j % 4
+                                                  : ru.w + 16 * j + li;
+                        v64[size_t(at)] = v;
                     }
             }
+        }
         if (!v64.empty() && hipMemcpy(sd.d_a64 + lo, v64.data(), v64.size() * sizeof(double), hipMemcpyHostToDevice) !=
                                 hipSuccess)
             throw std::runtime_error("hipMemcpy failed");
@@ -1967,6 +2114,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                     o.mat[1] = sep ? static_cast<const void*>(sw->side[side].d_a32) : p->bs->d_mat32;
                     o.mat[2] = sep ? static_cast<const void*>(sw->side[side].d_a16) : nullptr;
                     o.codes = sep ? sw->side[side].d_codes : nullptr;
+                    o.quad = sep && sw->side[side].quad[k] ? 1 : 0;
                     if (sep && !sw->side[side].d_a32) sb->has32 = false;
                     if (!sep || !sw->side[side].d_a16 || !sw->d_arena[1]) sb->has16 = false;
                     o.arena[0] = sw->d_arena[0];
