@@ -593,6 +593,10 @@ template <typename T, typename TA, int NG>
 __device__ __forceinline__ void sep_tile_quad(const TA* __restrict__ mat, const int* __restrict__ codes,
                                               T* __restrict__ arena, const int2 rv, const int w, const int lane,
                                               const UdOp& op, T* __restrict__ part);
+template <typename T, typename TA>
+__device__ __forceinline__ void sep_tile_quad_cols(const TA* __restrict__ mat, const int* __restrict__ codes,
+                                                   T* __restrict__ arena, const int2 rv, const int lane,
+                                                   const UdOp& op);
 
 // One phase of a batch: ops of many instances, each on whole workgroups
 // (wg_op[workgroup] = its op); the workgroup's op kind picks the tile body.
@@ -623,7 +627,9 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_ud_batch(const UdBatchOp* __r
             break;
         case UK_UP_FINAL: ud_tile<T, UD_UP, true, 1>(mat, arena, rv, w, lane, o.a, x, o.xmap, y, o.ymap, part); break;
         case UK_SEP:
-            if (o.quad)
+            if (o.quad && __builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
+                sep_tile_quad_cols<T, T>(mat, o.codes, arena, rv, lane, o.a);
+            else if (o.quad)
                 sep_tile_quad<T, T, 2>(mat, o.codes, arena, rv, w, lane, o.a, part);
             else if (__builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
                 sep_tile_cols<T, T>(mat, o.codes, arena, rv, lane, o.a);
@@ -826,6 +832,76 @@ __device__ __forceinline__ void sep_tile_quad(const TA* __restrict__ mat, const 
         if (ook[qq]) gst(out + oaddr[qq], acc[qq]);
 }
 
+// sep_tile_cols in the quad storage: a panel of at most SEP_KMAX steps (two groups
+// of four) keeps its operator values and its B offsets without the column term in
+// registers and sweeps R_NCT column tiles of its group, two at a time.
+template <typename T, typename TA>
+__device__ __forceinline__ void sep_tile_quad_cols(const TA* __restrict__ mat, const int* __restrict__ codes,
+                                                   T* __restrict__ arena, const int2 rv, const int lane,
+                                                   const UdOp& op) {
+    static_assert(SEP_KMAX == 8, "two groups of four steps");
+    using V = typename SwMfma<T>::V;
+    const int f0 = __builtin_amdgcn_readlane(rv.x, R_F0), nct = __builtin_amdgcn_readlane(rv.x, R_NCT);
+    const int O = __builtin_amdgcn_readlane(rv.x, R_O);
+    const int li = lane & 15, lk = lane >> 4;
+    int rb[4], rn[4];
+    bool rok[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int row = SwMfma<T>::row(lane, q);
+        rb[q] = __shfl(rv.x, R_ROWBASE + row);
+        rn[q] = __shfl(rv.x, R_ROWNL + row);
+        rok[q] = __shfl(rv.x, R_ROWLEV + row) >= 0;
+    }
+    const int ns = __builtin_amdgcn_readlane(rv.x, R_SHI);  // the panel's steps (from step 0)
+    const long long g0 = op.a_off[0] / 64 + __builtin_amdgcn_readlane(rv.x, R_STEP0);
+    const int cv = lane < ns ? gld(codes + g0 + lane) : 0;
+    const T* bsrc = arena + op.in_base;
+    TA av[2][4];
+    int bp[2], sh[2], nv[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+        gld4(mat + 64 * (g0 + 4 * g) + 4 * lane, av[g]);
+        const int code = __shfl(cv, 4 * g + lk);
+        nv[g] = (code >> 10) & 7;
+        bp[g] = __shfl(rv.y, code & 31) + (code >> 13);
+        sh[g] = (code >> 5) & 31;
+    }
+    T* out = arena + op.out_base;
+    for (int ct = 0; ct < nct; ct += 2) {
+        const bool two = ct + 1 < nct;
+        int fo[2];
+        bool fok[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int f = f0 + 16 * (ct + u) + li;
+            fok[u] = (u == 0 || two) && f < O;
+            fo[u] = fok[u] ? f : 0;
+        }
+        V acc[2] = {V{T(0), T(0), T(0), T(0)}, V{T(0), T(0), T(0), T(0)}};
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            if (4 * g < ns) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    T bv[4];
+                    const bool ok = nv[g] > 0 && fok[u];
+                    gld4(bsrc + (ok ? unsigned(bp[g] + ((fo[u] * op.deg) << sh[g])) : 0u), bv);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) bv[j] = ok && j < nv[g] ? bv[j] : T(0);
+                    acc[u] = SepMma<T, TA>::run4(av[g], bv, acc[u]);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (fok[u] && rok[q]) gst(out + rb[q] + fo[u] * rn[q], acc[u][q]);
+    }
+    __syncthreads();  // the one barrier every wave of the workgroup takes (sep_tile's partial sums)
+}
+
 // One wave of a separable op over R_NCT consecutive 16-column tiles of one group
 // and one row panel whose kept steps (at most SEP_KMAX, not split over waves) are
 // loaded once: the panel's A values and the lane's B offsets without the column
@@ -926,7 +1002,9 @@ __global__ void __launch_bounds__(UD_WAVES * 64) k_sep_batch(const UdBatchOp* __
     constexpr int sl = sizeof(T) == 8 ? 0 : 1;            // arena
     constexpr int ml = sizeof(TA) == 2 ? 2 : sl;          // values
     const TA* mat = static_cast<const TA*>(o.mat[ml]);
-    if (o.quad)
+    if (o.quad && __builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
+        sep_tile_quad_cols<T, TA>(mat, o.codes, static_cast<T*>(o.arena[sl]), rv, lane, o.a);
+    else if (o.quad)
         sep_tile_quad<T, TA, 2>(mat, o.codes, static_cast<T*>(o.arena[sl]), rv, w, lane, o.a, part);
     else if (__builtin_amdgcn_readlane(rv.x, R_GSIZE) > 0 && __builtin_amdgcn_readlane(rv.x, R_NCT) > 1)
         sep_tile_cols<T, TA>(mat, o.codes, static_cast<T*>(o.arena[sl]), rv, lane, o.a);
@@ -1142,7 +1220,7 @@ void emit_records(const kmx_sweep& sw, int wave_batches, std::vector<std::vector
                         }
                         const int steps = nterm * sper;
                         const int ctiles = (g.O * g.I + 15) / 16;
-                        if (sep && !quad && sper >= 1 && sper <= SEP_KMAX && ctiles > 1) {
+                        if (sep && sper >= 1 && sper <= SEP_KMAX && ctiles > 1) {
                             // a short panel: one wave sweeps up to batch / steps column tiles
                             const int per = std::max(1, std::min(ctiles, batch / sper));
                             for (int c0 = 0; c0 < ctiles; c0 += per) {
@@ -1321,7 +1399,7 @@ void check_records(const kmx_sweep& sw, const std::vector<std::vector<int>>& rec
                         r[R_SHI] < r[R_SLO] || r[R_SHI] > steps)
                         throw std::logic_error("sweep separable record steps");
                     const int nct = r[R_NCT];
-                    if (sd.quad[size_t(sep_k)] && (nct != 1 || r[R_SLO] % 4 || g.I != 1))
+                    if (sd.quad[size_t(sep_k)] && (r[R_SLO] % 4 || g.I != 1))
                         throw std::logic_error("sweep quad record");
                     if (nct < 1 || r[R_F0] % 16 || r[R_F0] + 16 * (nct - 1) >= g.O * g.I ||
                         (nct > 1 && (gsize != 1 || r[R_SLO] != 0 || r[R_SHI] != steps || steps < 1 ||
