@@ -212,6 +212,10 @@ class HessenbergLSQ:
         self.k = k + 1
         return abs(g[k + 1])
 
+    def residual(self):
+        """|beta e1 - H y| at the minimising y over the columns so far."""
+        return abs(self.g[self.k])
+
     def solve(self):
         """y minimising |beta e1 - H y| over the columns so far."""
         k = self.k
@@ -229,18 +233,20 @@ class DeviceGMRES:
     are captured in one HIP graph of their own (k fixed in each, so nothing is
     indexed on the device), whose replay ends by copying the chunk's Hessenberg
     columns to pinned host memory.  The host applies the Givens rotations and
-    checks the residual estimate once per chunk, while the next chunk's replay
-    already runs; so up to 2 chunk - 1 steps may run past convergence, whose basis
-    vectors are not used.  restart must be a multiple of chunk and at most
-    KMX_ARNOLDI_MAX_K."""
+    checks the residual estimate once per chunk, while up to `depth` chunks'
+    replays are queued; the next chunk is queued unless the estimates, extrapolated,
+    say that the queued ones reach the tolerance.  Steps past convergence, whose
+    basis vectors are not used, are the price of never leaving the device idle.  restart must be a multiple of
+    chunk and at most KMX_ARNOLDI_MAX_K."""
 
-    def __init__(self, apply_into, n, restart=40, device="cuda", chunk=4, shift=(0.0, 1.0)):
+    def __init__(self, apply_into, n, restart=40, device="cuda", chunk=4, shift=(0.0, 1.0), depth=2):
         torch = _torch()
         from . import _lib
 
         if restart % chunk or restart > _lib.ARNOLDI_MAX_K:
             raise ValueError("restart must be a multiple of chunk and at most KMX_ARNOLDI_MAX_K")
         self.apply_into, self.n, self.m, self.chunk = apply_into, int(n), int(restart), int(chunk)
+        self.depth = max(1, int(depth))
         self.s0, self.s1 = float(shift[0]), float(shift[1])
         m = self.m
         self.V = torch.zeros((m + 1, self.n), dtype=torch.float64, device=device)
@@ -249,6 +255,7 @@ class DeviceGMRES:
         self.work = torch.empty(_lib.arnoldi_work(self.n), dtype=torch.float64, device=device)
         self.hdev = torch.zeros((m, m + 2), dtype=torch.float64, device=device)
         self.hhost = torch.zeros((m, m + 2), dtype=torch.float64).pin_memory()
+        self.ystage = torch.zeros(m, dtype=torch.float64).pin_memory()  # least-squares solution, host side
         self.graphs = [None] * (m // self.chunk)
 
     def _chunk(self, c0):
@@ -275,53 +282,87 @@ class DeviceGMRES:
             self.graphs[ci] = g
         return self.graphs[ci]
 
+    def _converges_in_flight(self, rs, target):
+        """Whether the residual estimates rs (one per step of the cycle so far),
+        extrapolated at the rate of their last chunk of steps, reach target within
+        the steps already queued ((depth - 1) chunks): then the next chunk is held
+        back until a later chunk shows it is needed.  Timing only: a wrong guess
+        costs one wait for the device, never a different result."""
+        span = min(self.chunk, len(rs) - 1)
+        if span <= 0 or rs[-1] <= 0.0 or rs[-1 - span] <= 0.0:
+            return False
+        rate = (rs[-1] / rs[-1 - span]) ** (1.0 / span)
+        if rate >= 1.0:
+            return False
+        return math.log(target / rs[-1]) / math.log(rate) <= (self.depth - 1) * self.chunk
+
     def matvec(self, x, out):
         """out = M x."""
         self.apply_into(x, out)
         out.mul_(self.s1).add_(x, alpha=self.s0)
         return out
 
-    def solve(self, b, tol=1e-6, maxiter=400):
-        """x with ||b - M x|| <= tol ||b|| (estimate), from x = 0.  Returns
-        (x, relative residual estimate, iterations used)."""
+    def solve(self, b, tol=1e-6, maxiter=400, bnorm=None):
+        """x with ||b - M x|| <= tol ||b|| (estimate), from x = 0 (bnorm: ||b|| when
+        the caller has it).  Returns (x, relative residual estimate, iterations
+        used).  A cycle that meets tol by the least-squares estimate returns without
+        forming b - M x (callers that need the true residual form it themselves)."""
         torch = _torch()
         m = self.m
         x = torch.zeros_like(b)
-        bnorm = float(torch.linalg.vector_norm(b))
+        bnorm = float(torch.linalg.vector_norm(b)) if bnorm is None else float(bnorm)
         if bnorm == 0.0:
             return x, 0.0, 0
-        it, rel, r = 0, 1.0, b.clone()
+        it, rel, r, beta = 0, 1.0, b, bnorm
         while it < maxiter:
-            beta = float(torch.linalg.vector_norm(r))
+            if r is not b:
+                beta = float(torch.linalg.vector_norm(r))
             rel = beta / bnorm
             if rel <= tol:
                 break
             self.V[0].copy_(r / beta)
             ls = HessenbergLSQ(beta, m)
             nch = m // self.chunk
-            self._graph(0).replay()
-            ev = torch.cuda.Event()
-            ev.record()
+            done_ev = {}
+
+            def launch(ci):
+                self._graph(ci).replay()
+                done_ev[ci] = torch.cuda.Event()
+                done_ev[ci].record()
+
+            for ci in range(min(self.depth, nch)):
+                if it + ci * self.chunk < maxiter:
+                    launch(ci)
+            rs = [beta]  # the residual estimate after each step of this cycle
             for ci in range(nch):
-                ev.synchronize()  # chunk ci done: its Hessenberg columns are on the host
-                if ci + 1 < nch and it + self.chunk < maxiter:
-                    # the next chunk runs while the host rotates this one's columns
-                    # (after convergence it is wasted work whose basis vectors go unused)
-                    self._graph(ci + 1).replay()
-                    ev = torch.cuda.Event()
-                    ev.record()
+                if ci not in done_ev:  # held back by the estimate below, needed after all
+                    launch(ci)
+                done_ev.pop(ci).synchronize()  # chunk ci done: its Hessenberg columns are on the host
                 hv_all = self.hhost.numpy()
                 done = False
                 for c in range(self.chunk):
                     k = ci * self.chunk + c
                     res = ls.add_column(hv_all[k, : k + 2].tolist())
+                    rs.append(res)
                     it += 1
                     if res / bnorm <= tol or hv_all[k, k + 1] == 0.0 or it >= maxiter:
                         done = True
                         break
                 if done or it >= maxiter:
                     break
-            x += torch.from_numpy(ls.solve()).to(b.device) @ self.V[: ls.k]
+                nxt = ci + self.depth
+                if nxt < nch and it + (self.depth - 1) * self.chunk < maxiter and not self._converges_in_flight(
+                        rs, tol * bnorm):
+                    # queued behind the chunks in flight, so the device never waits on the
+                    # host's rotations (steps past convergence are wasted work whose basis
+                    # vectors go unused)
+                    launch(nxt)
+            y = self.ystage[: ls.k]
+            y.copy_(torch.from_numpy(ls.solve()))
+            x.addmv_(self.V[: ls.k].t(), y.to(b.device, non_blocking=True))
+            est = ls.residual() / bnorm
+            if est <= tol or it >= maxiter:
+                return x, est, it
             r = b - self.matvec(x, self.Mx)
         return x, rel, it
 
@@ -371,11 +412,12 @@ def backward_euler(plan, f0, dt, rhs=None, tol=1e-10, inner_prec="fp32", inner_t
     hist, inner = [], 0
     for _ in range(max_refine):
         r = b - op64(f)
-        rel = float(torch.linalg.vector_norm(r)) / bnorm if bnorm else 0.0
+        rnorm = float(torch.linalg.vector_norm(r))
+        rel = rnorm / bnorm if bnorm else 0.0
         hist.append(rel)
         if rel <= tol:
             break
-        d, _, its = solver.solve(r, tol=inner_tol, maxiter=max_inner)
+        d, _, its = solver.solve(r, tol=inner_tol, maxiter=max_inner, bnorm=rnorm)
         inner += its
         f = f + d
     return f, hist, inner

@@ -244,9 +244,11 @@ int kmx_batch_rk3(kmx_batch* batch, double* const* fs, const double* dts, int ns
  * it against rows 0..k by classical Gram-Schmidt twice, writes h[0..k] (the
  * summed coefficients) and h[k+1] = |w| (DEVICE, k+2 values), and row k+1 of V =
  * w / |w| (zero when |w| = 0: breakdown).  work: DEVICE scratch of
- * KMX_ARNOLDI_WORK(n) doubles, owned by the caller.  Allocates nothing; four
+ * KMX_ARNOLDI_WORK(n) doubles, owned by the caller.  Allocates nothing; three
  * launches of fixed shape for given (n, k), asynchronous on `stream` (graph
- * capturable).  0 <= k <= KMX_ARNOLDI_MAX_K. */
+ * capturable).  |w| after the second pass is taken as sqrt(|w1|^2 - |h2|^2)
+ * (w1: w after the first pass, h2: the second pass's coefficients).
+ * 0 <= k <= KMX_ARNOLDI_MAX_K. */
 #define KMX_ARNOLDI_MAX_K 127
 #define KMX_ARNOLDI_WORK(n) ((n) + 32896)
 int kmx_arnoldi_step(int64_t n, int k, double* V, const double* u, double s0, double s1, double* h, double* work,

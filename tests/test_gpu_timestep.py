@@ -182,12 +182,16 @@ def test_rk3_dirichlet_on_gpu_matches_gold(kmx_gpu, gold, name, pfn, lev, deg):
             assert rel(got, fo) <= 1e-12, i
 
 
-@pytest.mark.parametrize("n,k", [(7, 0), (300, 3), (32064, 17), (100003, 39), (5000, 127)])
+@pytest.mark.parametrize("n,k", [(7, 0), (300, 3), (2000, 12), (32064, 17), (32768, 40), (32769, 5), (40000, 70),
+                                 (100003, 39), (5000, 126), (5000, 127)])
 def test_arnoldi_step_matches_two_pass_gram_schmidt(kmx_gpu, n, k):
     """kmx_arnoldi_step against the same step in numpy: w = s0 v_k + s1 u, two
     classical Gram-Schmidt passes against v_0..v_k, h = both passes' coefficients,
     v_{k+1} = w / |w|; h within 1e-13 of |w|, v_{k+1} within 1e-13, and v_{k+1}
-    orthogonal to v_0..v_k to 1e-14.  A u in the span of v_0..v_k (s1 = 1, s0 = 0)
+    orthogonal to v_0..v_k to 1e-14.  Sizes on both sides of n = 32768 (chunks of
+    at most 256 values and k <= 126: rows held in registers; otherwise rows
+    streamed) and every row bucket of the register kernels (k + 2 <= 8, 16, 32,
+    64, 128).  A u in the span of v_0..v_k (s1 = 1, s0 = 0)
     breaks down: |w| ~ 0."""
     import torch
 
