@@ -1150,6 +1150,16 @@ bool sep_quad_on() {
     return !(e && e[0] == '0');
 }
 
+// Workgroups of a merged separable launch dealt to one XCD in runs of this many
+// consecutive entries of the work list (KMX_SEP_XCD_BLOCK; 1: the list order as is).
+// 64-instance batch, FP32 (same box, results identical): fused RK3 step 0.228 ms in
+// list order, 0.220 at runs of 8, 0.214-0.217 at 16, 0.216-0.218 at 32, 0.215 at 64,
+// 0.219 at 128; batched apply 0.095-0.098 -> 0.094.
+int sep_xcd_block() {
+    if (const char* e = std::getenv("KMX_SEP_XCD_BLOCK")) return std::max(1, std::atoi(e));
+    return 16;
+}
+
 double env_double(const char* name, double dflt) {
     if (const char* e = std::getenv(name)) return std::atof(e);
     return dflt;
@@ -2226,7 +2236,8 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                 // classes first, emission order (instance by instance) inside a class: the long
                 // tiles of every instance start early without scattering each instance's tiles
                 // over the whole launch.  (Dealing each op's workgroups to one XCD, balanced by
-                // work, was measured 1-7 % slower on the 64-instance batch.)
+                // work, was measured 1-7 % slower on the 64-instance batch; runs of
+                // sep_xcd_block() neighbouring workgroups per XCD are faster.)
                 const size_t rw = size_t(UD_WAVES) * UD_REC;  // ints per workgroup
                 auto rec_int = [&](size_t wg, int w, int k) {  // int k of wave w's record
                     const size_t b = wg * rw + size_t(w) * UD_REC;
@@ -2250,6 +2261,22 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
                 std::vector<int> idx(size_t(P.nwg));
                 for (int g = 0; g < P.nwg; ++g) idx[size_t(g)] = g;
                 std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cls[size_t(a)] > cls[size_t(b)]; });
+                const int xb = sep_xcd_block();
+                if (xb > 1) {
+                    // runs of xb workgroups in list order to one XCD (launch position p runs
+                    // on XCD p % 8): in each block of 8 xb, run y goes to positions
+                    // y, y + 8, ..., so neighbours in the list -- the column tiles of one
+                    // panel -- read its operator values through one L2
+                    const size_t blk = size_t(8) * size_t(xb);
+                    std::vector<int> idx2(idx.size());
+                    for (size_t r = 0; r < idx.size(); ++r) {
+                        size_t pos = r;
+                        const size_t b = r / blk, o = r % blk;
+                        if ((b + 1) * blk <= idx.size()) pos = b * blk + (o % size_t(xb)) * 8 + o / size_t(xb);
+                        idx2[pos] = idx[r];
+                    }
+                    idx.swap(idx2);
+                }
                 std::vector<int> wg2(size_t(P.nwg)), fl2(size_t(P.nwg) * rw);
                 for (int g = 0; g < P.nwg; ++g) {
                     const size_t src = size_t(P.wg0 + idx[size_t(g)]);
