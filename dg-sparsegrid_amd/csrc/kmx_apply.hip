@@ -657,7 +657,8 @@ extern "C" int kmx_batch_autotune(kmx_batch* b, int prec, int* wave_batches, voi
         if (rc != KMX_OK) return rc;
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    // scratch x / y of every plan (zero x: the timing does not depend on the values)
+    // scratch x / y of every plan, x all ones (zero operands time up to 8 % faster on
+    // the matrix cores than data, and can rank the candidates differently)
     std::vector<double*> xs(b->plans.size(), nullptr), ys(b->plans.size(), nullptr);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = KMX_OK;
@@ -667,10 +668,13 @@ extern "C" int kmx_batch_autotune(kmx_batch* b, int prec, int* wave_batches, voi
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
     };
+    size_t nmax = 1;
+    for (const kmx_plan* p : b->plans) nmax = std::max(nmax, size_t(p->N) * p->B);
+    const std::vector<double> ones(nmax, 1.0);
     for (size_t i = 0; i < b->plans.size(); ++i) {
         const size_t n = std::max<size_t>(1, size_t(b->plans[i]->N) * b->plans[i]->B);
         if (hipMalloc(&xs[i], n * sizeof(double)) != hipSuccess || hipMalloc(&ys[i], n * sizeof(double)) != hipSuccess ||
-            hipMemset(xs[i], 0, n * sizeof(double)) != hipSuccess) {
+            hipMemcpy(xs[i], ones.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
             cleanup();
             set_error("kmx_batch_autotune: out of device memory");
             return KMX_E_HIP;
