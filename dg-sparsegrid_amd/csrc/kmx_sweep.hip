@@ -455,10 +455,25 @@ __device__ __forceinline__ double* io_y(const BatchIO& io, int i) { return io.y[
 __device__ __forceinline__ const double* io_x(const IoOne& io, int) { return io.x; }
 __device__ __forceinline__ double* io_y(const IoOne& io, int) { return io.y; }
 
+// The block of the box list a workgroup of the gather / slot-sum launches takes:
+// runs of BOX_XCD_RUN consecutive blocks on one XCD (workgroup b runs on XCD b % 8;
+// in each span of 8 BOX_XCD_RUN workgroups, XCD y takes blocks y BOX_XCD_RUN ..
+// (y + 1) BOX_XCD_RUN - 1; a last partial span in order).  Box order splits an
+// element's x / y values into runs of deg, the rest of whose lines the next blocks
+// read: on one XCD they come from its L2.
+constexpr int BOX_XCD_RUN = 16;
+__device__ __forceinline__ int box_block() {
+    const int b = int(blockIdx.x), span = 8 * BOX_XCD_RUN;
+    const int base = b - b % span;
+    if (base + span > int(gridDim.x)) return b;
+    return base + (b % 8) * BOX_XCD_RUN + (b % span) / 8;
+}
+
 template <typename T, typename IO>
 __global__ void __launch_bounds__(256) k_box_gather_batch(const BoxJob* __restrict__ wg_jobs, IO io) {
-    const BoxJob& j = wg_jobs[blockIdx.x];
-    const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
+    const int blk = box_block();
+    const BoxJob& j = wg_jobs[blk];
+    const long long i = (long long)(blk - j.blk0) * 256 + threadIdx.x;
     if (i < j.nb) {
         const int m = gld(j.xmap + i);
         T* xb = static_cast<T*>(j.arena[sizeof(T) == 8 ? 0 : 1]);
@@ -469,8 +484,9 @@ __global__ void __launch_bounds__(256) k_box_gather_batch(const BoxJob* __restri
 // (the slots are read before the map, so both loads are in flight together)
 template <typename T, typename IO>
 __global__ void __launch_bounds__(256) k_box_sum_batch(const BoxJob* __restrict__ wg_jobs, IO io) {
-    const BoxJob& j = wg_jobs[blockIdx.x];
-    const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
+    const int blk = box_block();
+    const BoxJob& j = wg_jobs[blk];
+    const long long i = (long long)(blk - j.blk0) * 256 + threadIdx.x;
     if (i < j.nb) {
         const T* sl = static_cast<const T*>(j.arena[sizeof(T) == 8 ? 0 : 1]) + j.nb + i;
         double v = 0.0;
