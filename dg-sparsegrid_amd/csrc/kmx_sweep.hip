@@ -161,6 +161,30 @@ __device__ __forceinline__ void gld4(const T* p, T out[4]) {
     out[2] = x[2];
     out[3] = x[3];
 }
+// four values at an address aligned to four of them (the RK3 stage workspace)
+template <typename T>
+struct Vec4A {
+    typedef T v __attribute__((ext_vector_type(4), aligned(4 * sizeof(T))));
+};
+template <typename T>
+__device__ __forceinline__ void gld4a(const T* p, T out[4]) {
+    typedef typename Vec4A<T>::v V;
+    const V x = *(const __attribute__((address_space(1))) V*)p;
+    out[0] = x[0];
+    out[1] = x[1];
+    out[2] = x[2];
+    out[3] = x[3];
+}
+template <typename T>
+__device__ __forceinline__ void gst4a(T* p, const T v[4]) {
+    typedef typename Vec4A<T>::v V;
+    V x;
+    x[0] = v[0];
+    x[1] = v[1];
+    x[2] = v[2];
+    x[3] = v[3];
+    *(__attribute__((address_space(1))) V*)p = x;
+}
 
 // positions of level l along a dimension: deg * cells(l)
 __device__ __forceinline__ int deg_cells(int deg, int l) { return deg * (l ? 1 << (l - 1) : 1); }
@@ -520,21 +544,11 @@ struct Rk3Args {
     double* fb;                   // the state in box order (FP64), at koff
     const unsigned char* present;  // 1 at the present cells, at koff
 };
+// stages 1..3 at box value i of job j (a present cell, or nothing is written)
 template <typename T, int STAGE>
-__global__ void __launch_bounds__(256) k_box_rk3(const BoxJob* __restrict__ wg_jobs, Rk3Args a, int next_box) {
-    const BoxJob& j = wg_jobs[blockIdx.x];
-    const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
-    if (i >= j.nb) return;
+__device__ __forceinline__ void rk3_cell(const BoxJob& j, const Rk3Args& a, int next_box, long long i) {
     T* box = static_cast<T*>(j.arena[sizeof(T) == 8 ? 0 : 1]);
     double* fb = a.fb + j.koff;
-    if (STAGE == 0) {  // the caller's state in, and its box copy
-        const int m = gld(j.ymap + i);
-        if (m < 0) return;
-        const double fm = gld(a.f[j.inst] + m);
-        gst(fb + i, fm);
-        gst(box + i, T(fm));
-        return;
-    }
     double k = 0.0;  // the slots first, so their loads and the flag's are in flight together
     const T* sl = box + j.nb + i;
     for (int q = 0; q < j.nslot; ++q) k += double(gld(sl + (long long)q * j.nb));
@@ -557,6 +571,101 @@ __global__ void __launch_bounds__(256) k_box_rk3(const BoxJob* __restrict__ wg_j
         } else {  // the call's last step: the caller's state
             gst(a.f[j.inst] + gld(j.ymap + i), fn);
         }
+    }
+}
+
+template <typename T, int STAGE>
+__global__ void __launch_bounds__(256) k_box_rk3(const BoxJob* __restrict__ wg_jobs, Rk3Args a, int next_box) {
+    const BoxJob& j = wg_jobs[blockIdx.x];
+    const long long i = (long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x;
+    if (i >= j.nb) return;
+    if (STAGE == 0) {  // the caller's state in, and its box copy
+        T* box = static_cast<T*>(j.arena[sizeof(T) == 8 ? 0 : 1]);
+        const int m = gld(j.ymap + i);
+        if (m < 0) return;
+        const double fm = gld(a.f[j.inst] + m);
+        gst(a.fb + j.koff + i, fm);
+        gst(box + i, T(fm));
+        return;
+    }
+    rk3_cell<T, STAGE>(j, a, next_box, i);
+}
+
+// Stages 1..3 four box values per thread (jobs in blocks of 1024 values): the
+// stage workspace is aligned to four values (every instance's koff is a multiple
+// of four), so k1, k2, the box-order state, the box copy and four present flags
+// move as one vector each; the slots (at multiples of nb) as unaligned vectors.
+// The arithmetic of every value is rk3_cell's; absent cells of the box copy are
+// never written (k1, k2 and the box-order state at absent cells are, and never
+// read).  An instance's last 1..3 values go one at a time.
+template <typename T, int STAGE>
+__global__ void __launch_bounds__(256) k_box_rk3v(const BoxJob* __restrict__ wg_jobs, Rk3Args a, int next_box) {
+    static_assert(STAGE >= 1 && STAGE <= 3, "stages 1..3");
+    const BoxJob& j = wg_jobs[blockIdx.x];
+    const long long i0 = ((long long)(int(blockIdx.x) - j.blk0) * 256 + threadIdx.x) * 4;
+    if (i0 >= j.nb) return;
+    if (i0 + 4 > j.nb) {
+        for (long long i = i0; i < j.nb; ++i) rk3_cell<T, STAGE>(j, a, next_box, i);
+        return;
+    }
+    T* box = static_cast<T*>(j.arena[sizeof(T) == 8 ? 0 : 1]);
+    double* fb = a.fb + j.koff;
+    double k[4] = {0.0, 0.0, 0.0, 0.0};
+    const T* sl = box + j.nb + i0;
+    for (int q = 0; q < j.nslot; ++q) {
+        T v[4];
+        gld4(sl + (long long)q * j.nb, v);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) k[e] += double(v[e]);
+    }
+    const unsigned pf = gld(reinterpret_cast<const unsigned*>(a.present + j.koff + i0));
+    if (pf == 0u) return;
+    double fm[4];
+    gld4a(fb + i0, fm);
+    const double dt = a.dt[j.inst];
+    T* k1 = static_cast<T*>(a.k1) + j.koff;
+    T* k2 = static_cast<T*>(a.k2) + j.koff;
+    T kk[4], bx[4];
+    if (STAGE == 1) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            kk[e] = T(k[e]);
+            bx[e] = T(fm[e] + dt * RK_A21 * k[e]);
+        }
+        gst4a(k1 + i0, kk);
+    } else if (STAGE == 2) {
+        T k1v[4];
+        gld4a(k1 + i0, k1v);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            kk[e] = T(k[e]);
+            bx[e] = T(fm[e] + dt * (RK_A31 * double(k1v[e]) + RK_A32 * k[e]));
+        }
+        gst4a(k2 + i0, kk);
+    } else {
+        T k1v[4], k2v[4];
+        gld4a(k1 + i0, k1v);
+        gld4a(k2 + i0, k2v);
+        double fn[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            fn[e] = fm[e] + dt * (RK_B1 * double(k1v[e]) + RK_B2 * double(k2v[e]) + RK_B3 * k[e]);
+        if (!next_box) {  // the call's last step: the caller's state at the present cells
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if ((pf >> (8 * e)) & 0xffu) gst(a.f[j.inst] + gld(j.ymap + i0 + e), fn[e]);
+            return;
+        }
+        gst4a(fb + i0, fn);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bx[e] = T(fn[e]);
+    }
+    if (pf == 0x01010101u) {
+        gst4a(box + i0, bx);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if ((pf >> (8 * e)) & 0xffu) gst(box + i0 + e, bx[e]);
     }
 }
 
@@ -1174,6 +1283,12 @@ bool sep_quad_on() {
 int sep_xcd_block() {
     if (const char* e = std::getenv("KMX_SEP_XCD_BLOCK")) return std::max(1, std::atoi(e));
     return 16;
+}
+
+// The RK3 stage launches four values per thread (KMX_RK3_VEC=0: one).
+bool rk3_vec_on() {
+    const char* e = std::getenv("KMX_RK3_VEC");
+    return !(e && e[0] == '0');
 }
 
 double env_double(const char* name, double dflt) {
@@ -2153,6 +2268,8 @@ struct kmx_sweep_batch {
     std::vector<const kmx_plan*> members;
     BoxJob* d_wgjobs = nullptr;  // separable members' box copies of x and slot sums: each workgroup's job
     int njobs = 0, box_wgs = 0;
+    BoxJob* d_wgjobs4 = nullptr;  // the same jobs in blocks of 1024 values (k_box_rk3v)
+    int box_wgs4 = 0;
     UdBatchOp* d_ops = nullptr;
     int* d_wgop = nullptr;  // op of every workgroup, phases in order
     int* d_recs = nullptr;  // UD_WAVES work records per workgroup
@@ -2307,7 +2424,8 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
         }
         if (flat.size() != wgop.size() * UD_WAVES * UD_REC) throw std::logic_error("sweep batch records");
         std::vector<BoxJob> jobs, wgjobs;
-        long long blk = 0, koff = 0;
+        long long blk = 0, koff = 0, blk4 = 0;
+        std::vector<BoxJob> wgjobs4;
         for (size_t q = 0; q < members.size(); ++q) {
             const kmx_sweep* sw = members[q]->sw;
             if (!sw->separable) {
@@ -2316,13 +2434,18 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
             }
             jobs.push_back(BoxJob{sw->d_xmap, sw->d_ymap, {sw->d_arena[0], sw->d_arena[1]}, sw->nb, koff,
                                   int(sw->desc.size()), int(q), int(blk)});
-            koff += sw->nb;
+            koff += (sw->nb + 3) / 4 * 4;  // four-value alignment of the stage workspace
             blk += (sw->nb + 255) / 256;
             if (blk >= (1LL << 31)) throw std::runtime_error("sweep batch too large");
             wgjobs.resize(size_t(blk), jobs.back());
+            BoxJob j4 = jobs.back();
+            j4.blk0 = int(blk4);
+            blk4 += (sw->nb + 1023) / 1024;
+            wgjobs4.resize(size_t(blk4), j4);
         }
         sb->njobs = int(jobs.size());
         sb->box_wgs = int(blk);
+        sb->box_wgs4 = int(blk4);
         sb->kdof = koff;
         auto up = [](const auto* h, size_t n, auto*& d) {
             if (hipMalloc(&d, std::max<size_t>(1, n) * sizeof(*h)) != hipSuccess) throw std::bad_alloc();
@@ -2333,6 +2456,7 @@ kmx_sweep_batch* kmx_sweep_batch_build(const std::vector<kmx_plan*>& members, in
         up(wgop.data(), wgop.size(), sb->d_wgop);
         up(flat.data(), flat.size(), sb->d_recs);
         up(wgjobs.data(), wgjobs.size(), sb->d_wgjobs);
+        up(wgjobs4.data(), wgjobs4.size(), sb->d_wgjobs4);
     } catch (...) {
         kmx_sweep_batch_free(sb);
         throw;
@@ -2459,20 +2583,26 @@ int kmx_sweep_batch_rk3(kmx_sweep_batch* sb, int slot, double* const* fs, const 
             launch_sep_phase(sb, P, slot, s);
         }
     };
-    auto stage = [&](auto kern, int next) { hipLaunchKernelGGL(kern, bg, bb, 0, s, sb->d_wgjobs, a, next); };
-    if (slot == 0) stage(k_box_rk3<double, 0>, 0);
-    else stage(k_box_rk3<float, 0>, 0);
+    auto stage0 = [&](auto kern) { hipLaunchKernelGGL(kern, bg, bb, 0, s, sb->d_wgjobs, a, 0); };
+    const bool vec = rk3_vec_on();
+    const dim3 bg4(unsigned(std::max(1, sb->box_wgs4)));
+    auto stage = [&](auto kern, auto kernv, int next) {
+        if (vec) hipLaunchKernelGGL(kernv, bg4, bb, 0, s, sb->d_wgjobs4, a, next);
+        else hipLaunchKernelGGL(kern, bg, bb, 0, s, sb->d_wgjobs, a, next);
+    };
+    if (slot == 0) stage0(k_box_rk3<double, 0>);
+    else stage0(k_box_rk3<float, 0>);
     for (int st = 0; st < nsteps; ++st) {
         const int next = st + 1 < nsteps ? 1 : 0;
         ops();
-        if (slot == 0) stage(k_box_rk3<double, 1>, 0);
-        else stage(k_box_rk3<float, 1>, 0);
+        if (slot == 0) stage(k_box_rk3<double, 1>, k_box_rk3v<double, 1>, 0);
+        else stage(k_box_rk3<float, 1>, k_box_rk3v<float, 1>, 0);
         ops();
-        if (slot == 0) stage(k_box_rk3<double, 2>, 0);
-        else stage(k_box_rk3<float, 2>, 0);
+        if (slot == 0) stage(k_box_rk3<double, 2>, k_box_rk3v<double, 2>, 0);
+        else stage(k_box_rk3<float, 2>, k_box_rk3v<float, 2>, 0);
         ops();
-        if (slot == 0) stage(k_box_rk3<double, 3>, next);
-        else stage(k_box_rk3<float, 3>, next);
+        if (slot == 0) stage(k_box_rk3<double, 3>, k_box_rk3v<double, 3>, next);
+        else stage(k_box_rk3<float, 3>, k_box_rk3v<float, 3>, next);
     }
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
@@ -2489,6 +2619,7 @@ void kmx_sweep_batch_free(kmx_sweep_batch* sb) {
     if (sb->d_wgop) hipFree(sb->d_wgop);
     if (sb->d_recs) hipFree(sb->d_recs);
     if (sb->d_wgjobs) hipFree(sb->d_wgjobs);
+    if (sb->d_wgjobs4) hipFree(sb->d_wgjobs4);
     delete sb;
 }
 
