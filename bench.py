@@ -57,10 +57,10 @@ MEASURED_TRAFFIC = {
 # HBM bytes of one FP32 merged-sweep batched apply of the 64-instance workload
 # (its three launches together: box copies of x, every dimension's op, slot
 # sums into y): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE of the third batched apply
-# of `bench.py --pmc-pass`, taken from a profile of commit 2e17f65 -- a recorded
+# of `bench.py --pmc-pass`, taken from a profile of commit c2bfc7a -- a recorded
 # figure, not measured by this run (it goes stale when the kernels change).
-MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = 454053632
-TRAFFIC_SOURCE = "profile-derived: profiles/batch_sweep_hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, commit 2e17f65)"
+MEASURED_TRAFFIC_BATCH_SWEEP_FP32 = 428517248
+TRAFFIC_SOURCE = "profile-derived: profiles/batch_sweep_hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, commit c2bfc7a)"
 
 
 def host_threads():
