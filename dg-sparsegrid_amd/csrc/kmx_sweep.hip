@@ -1277,7 +1277,7 @@ bool sep_quad_on() {
 
 // Workgroups of a merged separable launch dealt to one XCD in runs of this many
 // consecutive entries of the work list (KMX_SEP_XCD_BLOCK; 1: the list order as is).
-// 64-instance batch, FP32 (same box, results identical): fused RK3 step 0.228 ms in
+// 64-instance batch, FP32 (alternating runs on one GPU, results identical): fused RK3 step 0.228 ms in
 // list order, 0.220 at runs of 8, 0.214-0.217 at 16, 0.216-0.218 at 32, 0.215 at 64,
 // 0.219 at 128; batched apply 0.095-0.098 -> 0.094.
 int sep_xcd_block() {
