@@ -320,7 +320,7 @@ class DeviceGMRES:
             rel = beta / bnorm
             if rel <= tol:
                 break
-            self.V[0].copy_(r / beta)
+            torch.div(r, beta, out=self.V[0])
             ls = HessenbergLSQ(beta, m)
             nch = m // self.chunk
             done_ev = {}
@@ -396,9 +396,8 @@ def backward_euler(plan, f0, dt, rhs=None, tol=1e-10, inner_prec="fp32", inner_t
     torch = _torch()
     b = f0.clone() if rhs is None else f0 + dt * rhs
     bnorm = float(torch.linalg.vector_norm(b))
-
-    def op64(v):
-        return v - dt * plan.apply(v, prec="fp64")
+    af = torch.empty_like(b)  # A f, FP64
+    r = torch.empty_like(b)
 
     key = (float(dt), inner_prec, int(restart), f0.numel())
     cache = _SOLVERS.setdefault(plan, {})
@@ -411,7 +410,8 @@ def backward_euler(plan, f0, dt, rhs=None, tol=1e-10, inner_prec="fp32", inner_t
     f = f0.clone()  # f0 is a good first guess for small dt
     hist, inner = [], 0
     for _ in range(max_refine):
-        r = b - op64(f)
+        plan.apply(f, af, prec="fp64")
+        torch.sub(b, f, out=r).add_(af, alpha=dt)  # r = b - (f - dt A f)
         rnorm = float(torch.linalg.vector_norm(r))
         rel = rnorm / bnorm if bnorm else 0.0
         hist.append(rel)
@@ -419,5 +419,5 @@ def backward_euler(plan, f0, dt, rhs=None, tol=1e-10, inner_prec="fp32", inner_t
             break
         d, _, its = solver.solve(r, tol=inner_tol, maxiter=max_inner, bnorm=rnorm)
         inner += its
-        f = f + d
+        f.add_(d)
     return f, hist, inner
