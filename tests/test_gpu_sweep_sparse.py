@@ -254,3 +254,49 @@ def test_fused_rk3_matches_applies(kmx_gpu, prec):
     g2.step()
     torch.cuda.synchronize()
     assert torch.equal(fused.f, one)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_launch_layout_switches_keep_results(kmx_gpu, prec):
+    """The merged sweep's work list dealt to the XCDs in list order
+    (KMX_SEP_XCD_BLOCK=1) or in runs (the default), and the fused RK3 stage
+    launches one box value per thread (KMX_RK3_VEC=0) or four (the default), give
+    bit-identical applies and RK3 states: they change where and how values move,
+    not what is summed or in which order."""
+    import os
+
+    import torch
+
+    from kmx import batch as BT
+
+    inst = BT.make_instances(count=8, seed=21, lev2d=(4, 6), lev3d=(4, 5))
+
+    def run():
+        sysm = BT.BatchedSystem(inst, sweep=(prec,))
+        x = torch.from_numpy(np.random.default_rng(4).standard_normal(sysm.dof)).cuda()
+        y = torch.empty_like(x)
+        sysm.apply(x, y, prec)
+        f = sysm.f.clone()
+        sysm.rk3(3, prec, f=f)
+        torch.cuda.synchronize()
+        return y, f
+
+    saved = {k: os.environ.get(k) for k in ("KMX_SEP_XCD_BLOCK", "KMX_RK3_VEC")}
+    try:
+        for k in saved:
+            os.environ.pop(k, None)
+        y0, f0 = run()
+        os.environ["KMX_SEP_XCD_BLOCK"] = "1"
+        y1, f1 = run()
+        os.environ.pop("KMX_SEP_XCD_BLOCK")
+        os.environ["KMX_RK3_VEC"] = "0"
+        y2, f2 = run()
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    assert torch.equal(y0, y1) and torch.equal(f0, f1)
+    assert torch.equal(y0, y2) and torch.equal(f0, f2)
+    assert float(torch.linalg.vector_norm(f0)) > 0.0
