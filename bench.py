@@ -14,10 +14,12 @@ Also reported: FP32 against FP64 on the same merged sweeps (the precision gain
 alone), the single 3D operator (SG
 lev 7, deg 3, 3 terms, connect_nD, P = 1,903,000) in FP64 / FP32 pair by pair
 and through the dimension sweep (--no-single skips), the 6D lev 4 deg 2 precision study with FP32 / BF16
-errors and a refined backward Euler step (--no-precision skips), and the CPU
-oracle on the host cores for both workloads (--no-cpu skips).
+errors and a refined backward Euler step (--no-precision skips), the batched
+GEMM entry (gemm_vbatched.m's kmx_gemm_vbatched, FP64) on batches of small,
+mid-size and one large GEMM (--no-gemm skips), and the CPU oracle on the host
+cores for both workloads (--no-cpu skips).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-single] [--no-precision]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-single] [--no-precision] [--no-gemm]
 
 Prints ONE JSON line.
 """
@@ -466,6 +468,41 @@ def single_instance_leg(plan, tuned, op, xd, yd, steps, warmup):
     return out, y64
 
 
+def gemm_leg():
+    """kmx_gemm_vbatched (gemm_vbatched.m, FP64, 'N','N', beta = 0) on three batches:
+    2,000 GEMMs of 27 x 81 x 27 (a kronmult3 stage at deg 3), 200 of 256^3 and one
+    of 2048^3; host time of the call (it waits for its launch), best of three, and
+    the largest error against torch's FP64 batched product."""
+    import torch
+
+    from kmx.plan import gemm_vbatched
+
+    out = {}
+    for name, cnt, m, n, k in (("2000 x 27x81x27", 2000, 27, 81, 27), ("200 x 256x256x256", 200, 256, 256, 256),
+                               ("1 x 2048x2048x2048", 1, 2048, 2048, 2048)):
+        g = torch.Generator(device="cuda").manual_seed(5)
+        A = torch.randn(cnt, k, m, dtype=torch.float64, device="cuda", generator=g)  # column-major m x k
+        B = torch.randn(cnt, n, k, dtype=torch.float64, device="cuda", generator=g)
+        Cm = torch.zeros(cnt, n, m, dtype=torch.float64, device="cuda")
+        args = ("N", "N", [m] * cnt, [n] * cnt, [k] * cnt, 1.0, [A[i].data_ptr() for i in range(cnt)], [m] * cnt,
+                [B[i].data_ptr() for i in range(cnt)], [k] * cnt, 0.0, [Cm[i].data_ptr() for i in range(cnt)],
+                [m] * cnt)
+        gemm_vbatched(*args)
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            gemm_vbatched(*args)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        ref = torch.bmm(A.transpose(1, 2).contiguous(), B.transpose(1, 2).contiguous())
+        err = float((Cm.transpose(1, 2) - ref).abs().max() / ref.abs().max())
+        out[name] = {"ms": round(best * 1e3, 3), "TFLOPs": round(2.0 * cnt * m * n * k / best / 1e12, 2),
+                     "rel_err_vs_torch_fp64": err}
+        del A, B, Cm, ref
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -477,6 +514,7 @@ def main():
                     help="skip the 64-instance legs (profiling runs of the single-operator kernels: "
                          "the line then has no value)")
     ap.add_argument("--no-precision", action="store_true", help="skip the 6D precision leg")
+    ap.add_argument("--no-gemm", action="store_true", help="skip the batched GEMM leg")
     ap.add_argument("--sweep-study", action="store_true",
                     help="only time the pair kernels against the opt-in dimension-sweep path (one JSON line)")
     ap.add_argument("--pmc-pass", action="store_true",
@@ -611,6 +649,8 @@ def main():
         out["single_instance"] = single
     if not a.no_precision:
         out["precision_6d"] = precision_leg()
+    if not a.no_gemm:
+        out["gemm_vbatched"] = gemm_leg()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline_batch(inst)
         if not a.no_single:

@@ -44,23 +44,68 @@ struct GemmDescDev {
     int m, n, k, lda, ldb, ldc;
 };
 
-// One workgroup per GEMM of the batch; threads stride over C entries.
+// The batch's GEMMs in 64 x 64 tiles of C: workgroup (g, t) takes tiles t, t +
+// gridDim.y, ... of GEMM g (row-tile fastest).  k runs in chunks of 16: the chunk's
+// 64 x 16 values of op(A) and 16 x 64 of op(B) are staged in LDS, each loaded
+// along the matrix's contiguous index, and each thread accumulates a 4 x 4 set of
+// C entries (rows tx + 16 r, columns ty + 16 c) over the chunk.  Every entry sums
+// its k products in order with FMAs, then C = alpha sum + beta C (C not read when
+// beta = 0), as the reference's gemm does entry by entry.
+constexpr int GV_T = 64, GV_K = 16;
 __global__ void __launch_bounds__(256) k_gemm_vbatched(const GemmDescDev* __restrict__ d, int count, int ta, int tb,
                                                        double alpha, double beta) {
+    __shared__ double As[GV_K][GV_T + 1];  // As[l][i] = op(A)(i0 + i, l0 + l)
+    __shared__ double Bs[GV_K][GV_T + 1];  // Bs[l][j] = op(B)(l0 + l, j0 + j)
     const int g = blockIdx.x;
     if (g >= count) return;
     const GemmDescDev q = d[g];
-    const long long mn = (long long)q.m * q.n;
-    for (long long e = threadIdx.x; e < mn; e += 256) {
-        const int i = int(e % q.m), j = int(e / q.m);
-        double s = 0.0;
-        for (int l = 0; l < q.k; ++l) {
-            const double a = ta ? q.A[(long long)i * q.lda + l] : q.A[(long long)l * q.lda + i];
-            const double b = tb ? q.B[(long long)l * q.ldb + j] : q.B[(long long)j * q.ldb + l];
-            s = __builtin_fma(a, b, s);
+    const int mt = (q.m + GV_T - 1) / GV_T, nt = (q.n + GV_T - 1) / GV_T;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    for (int tile = blockIdx.y; tile < mt * nt; tile += gridDim.y) {
+        const int i0 = (tile % mt) * GV_T, j0 = (tile / mt) * GV_T;
+        double acc[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+        for (int l0 = 0; l0 < q.k; l0 += GV_K) {
+            for (int t = threadIdx.x; t < GV_T * GV_K; t += 256) {
+                const int ii = ta ? t / GV_K : t % GV_T, ll = ta ? t % GV_K : t / GV_T;
+                const int i = i0 + ii, l = l0 + ll;
+                As[ll][ii] = (i < q.m && l < q.k) ? (ta ? q.A[(long long)i * q.lda + l] : q.A[(long long)l * q.lda + i])
+                                                  : 0.0;
+            }
+            for (int t = threadIdx.x; t < GV_T * GV_K; t += 256) {
+                const int jj = tb ? t % GV_T : t / GV_K, ll = tb ? t / GV_T : t % GV_K;
+                const int j = j0 + jj, l = l0 + ll;
+                Bs[ll][jj] = (j < q.n && l < q.k) ? (tb ? q.B[(long long)l * q.ldb + j] : q.B[(long long)j * q.ldb + l])
+                                                  : 0.0;
+            }
+            __syncthreads();
+            const int lk = min(GV_K, q.k - l0);
+            for (int l = 0; l < lk; ++l) {
+                double a[4], b[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) a[r] = As[l][tx + 16 * r];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) b[c] = Bs[l][ty + 16 * c];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) acc[r][c] = __builtin_fma(a[r], b[c], acc[r][c]);
+            }
+            __syncthreads();  // the chunk's reads before the next chunk's stores
         }
-        double& c = q.C[(long long)j * q.ldc + i];
-        c = (beta == 0.0 ? 0.0 : beta * c) + alpha * s;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + tx + 16 * r, j = j0 + ty + 16 * c;
+                if (i < q.m && j < q.n) {
+                    double& cv = q.C[(long long)j * q.ldc + i];
+                    cv = (beta == 0.0 ? 0.0 : beta * cv) + alpha * acc[r][c];
+                }
+            }
     }
 }
 
@@ -376,8 +421,11 @@ extern "C" int kmx_gemm_vbatched(char transA, char transB, const int* m, const i
         return KMX_E_NOMEM;
     }
     hipMemcpyAsync(d, h.data(), sizeof(GemmDescDev) * count, hipMemcpyHostToDevice, s);
-    hipLaunchKernelGGL(k_gemm_vbatched, dim3(count), dim3(256), 0, s, d, count, int(tr(transA)), int(tr(transB)),
-                       alpha, beta);
+    long long tiles = 1;  // C tiles of the batch's largest GEMM (grid y; more loop)
+    for (int i = 0; i < count; ++i)
+        tiles = std::max(tiles, (long long)((m[i] + 63) / 64) * ((n[i] + 63) / 64));
+    hipLaunchKernelGGL(k_gemm_vbatched, dim3(unsigned(count), unsigned(std::min<long long>(tiles, 65535))), dim3(256),
+                       0, s, d, count, int(tr(transA)), int(tr(transB)), alpha, beta);
     hipError_t e = hipGetLastError();
     hipStreamSynchronize(s);
     hipFree(d);

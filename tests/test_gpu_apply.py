@@ -312,6 +312,54 @@ def test_gemm_vbatched_and_batch_lists(kmx_gpu):
     assert rel(Y, np.kron(np.kron(mats[0], mats[1]), mats[2]) @ X) <= 1e-14
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_gemm_vbatched_tiled_shapes(kmx_gpu, ta, tb):
+    """gemm_vbatched.m's C = alpha op(A) op(B) + beta C on GEMMs that span several
+    64 x 64 tiles of C and 16-wide chunks of k, sizes multiples of neither, leading
+    dimensions above the minimum, k = 0 and beta = 0 (C not read: NaN there stays
+    out), every transpose combination: within 1e-13 of numpy."""
+    import torch
+
+    rng = np.random.default_rng(77)
+    shapes = [(1, 1, 1), (3, 5, 7), (64, 64, 16), (65, 130, 17), (200, 33, 150), (16, 80, 0), (129, 2, 40)]
+    alpha, beta = 1.25, -0.75
+    A, B, C, dA, dB, dC, lda, ldb, ldc, want = [], [], [], [], [], [], [], [], [], []
+    for q, (m, n, k) in enumerate(shapes):
+        ra, ca = (k, m) if ta == "T" else (m, k)
+        rb, cb = (n, k) if tb == "T" else (k, n)
+        la, lb, lc = max(1, ra) + 3, max(1, rb) + 1, m + 2
+        a = rng.standard_normal((la, max(1, ca)))
+        b = rng.standard_normal((lb, max(1, cb)))
+        c = rng.standard_normal((lc, n))
+        opa = (a[:ra, :ca].T if ta == "T" else a[:ra, :ca])
+        opb = (b[:rb, :cb].T if tb == "T" else b[:rb, :cb])
+        bq = 0.0 if q == 2 else beta
+        w = c.copy()
+        w[:m, :] = alpha * (opa @ opb) + (bq * c[:m, :] if bq else 0.0)
+        if q == 2:
+            c[:m, :] = np.nan  # beta = 0: C must not be read
+        want.append((w, bq))
+        for lst, arr in ((dA, a), (dB, b), (dC, c)):
+            lst.append(torch.from_numpy(arr.ravel(order="F").copy()).cuda())
+        lda.append(la)
+        ldb.append(lb)
+        ldc.append(lc)
+    ms, ns, ks = [s_[0] for s_ in shapes], [s_[1] for s_ in shapes], [s_[2] for s_ in shapes]
+    # beta differs for GEMM 2: run it on its own, the rest as one batch
+    rest = [q for q in range(len(shapes)) if q != 2]
+    for idx, bq in ((rest, beta), ([2], 0.0)):
+        kmx_gpu.gemm_vbatched(ta, tb, [ms[q] for q in idx], [ns[q] for q in idx], [ks[q] for q in idx], alpha,
+                              [dA[q] for q in idx], [lda[q] for q in idx], [dB[q] for q in idx],
+                              [ldb[q] for q in idx], bq, [dC[q] for q in idx], [ldc[q] for q in idx])
+    for q, (m, n, k) in enumerate(shapes):
+        got = dC[q].cpu().numpy().reshape((ldc[q], n), order="F")
+        w = want[q][0]
+        assert np.all(np.isfinite(got[:m, :])), q
+        assert np.max(np.abs(got[:m, :] - w[:m, :])) <= 1e-13 * max(1.0, np.max(np.abs(w[:m, :]))), q
+        if q != 2:
+            assert np.array_equal(got[m:, :], w[m:, :]), q  # rows past m untouched
+
+
 def test_batch_matches_single_plans(kmx_gpu):
     """One persistent launch over instances of mixed (d, lev, deg) == their own applies."""
     import torch
