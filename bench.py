@@ -500,6 +500,29 @@ def gemm_leg():
         out[name] = {"ms": round(best * 1e3, 3), "TFLOPs": round(2.0 * cnt * m * n * k / best / 1e12, 2),
                      "rel_err_vs_torch_fp64": err}
         del A, B, Cm, ref
+    # kron_multd.m's chain (kmx_kronmult_noalloc: the library's scratch array, no
+    # allocation in the call): four 8 x 8 factors on 128 vectors, 524,288 values per
+    # intermediate; bytes = each mode product reading and writing its values once
+    from kmx.plan import kronmult
+
+    g = torch.Generator(device="cuda").manual_seed(6)
+    mats = [torch.randn(8, 8, dtype=torch.float64, device="cuda", generator=g) for _ in range(4)]
+    X = torch.randn(4096, 128, dtype=torch.float64, device="cuda", generator=g)
+    Y = torch.empty(4096 * 128, dtype=torch.float64, device="cuda")
+    kronmult(mats, X, noalloc=True, Y=Y)
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(10):
+        t0 = time.perf_counter()
+        kronmult(mats, X, noalloc=True, Y=Y)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    got = Y.view(128, -1).t()
+    ref = torch.einsum("ai,bj,ck,dl,ijklv->abcdv", *mats, X.reshape(8, 8, 8, 8, 128)).reshape(4096, 128)
+    out["kronmult 4 x 8x8, 128 vectors"] = {
+        "us": round(best * 1e6, 1), "GBs": round(4 * 2 * 8 * 4096 * 128 / best / 1e9, 1),
+        "rel_err_vs_torch_fp64": float((got - ref).abs().max() / ref.abs().max()),
+        "note": "host time of the call (four mode-product launches plus the wrapper's column-major copies)"}
     return out
 
 
