@@ -289,7 +289,7 @@ class DeviceGMRES:
         back until a later chunk shows it is needed.  Timing only: a wrong guess
         costs one wait for the device, never a different result."""
         span = min(self.chunk, len(rs) - 1)
-        if span <= 0 or rs[-1] <= 0.0 or rs[-1 - span] <= 0.0:
+        if span <= 0 or target <= 0.0 or rs[-1] <= 0.0 or rs[-1 - span] <= 0.0:
             return False
         rate = (rs[-1] / rs[-1 - span]) ** (1.0 / span)
         if rate >= 1.0:

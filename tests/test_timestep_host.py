@@ -40,3 +40,4 @@ def test_gmres_holds_back_chunks_only_when_the_queued_steps_converge():
     assert not held(g, [1.0, 2.0, 3.0], 1e-3)            # growing
     assert not held(g, [1.0], 1e-3)                      # no step yet
     assert not held(SimpleNamespace(chunk=4, depth=1), rs, 1e-11)  # nothing queued
+    assert not held(g, rs, 0.0)  # tol = 0: never converges by estimate
