@@ -20,6 +20,7 @@ mid-size and one large GEMM (--no-gemm skips), and the CPU oracle on the host
 cores for both workloads (--no-cpu skips).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-single] [--no-precision] [--no-gemm]
+    python bench.py --gemm-study   (only the batched GEMM / Kronecker-chain leg)
 
 Prints ONE JSON line.
 """
@@ -538,6 +539,8 @@ def main():
                          "the line then has no value)")
     ap.add_argument("--no-precision", action="store_true", help="skip the 6D precision leg")
     ap.add_argument("--no-gemm", action="store_true", help="skip the batched GEMM leg")
+    ap.add_argument("--gemm-study", action="store_true",
+                    help="only the batched GEMM and Kronecker-chain leg (one JSON line)")
     ap.add_argument("--sweep-study", action="store_true",
                     help="only time the pair kernels against the opt-in dimension-sweep path (one JSON line)")
     ap.add_argument("--pmc-pass", action="store_true",
@@ -558,6 +561,9 @@ def main():
     torch.cuda.set_device(0)
     if a.sweep_study:
         print(json.dumps({"sweep_study": sweep_study()}), flush=True)
+        return
+    if a.gemm_study:
+        print(json.dumps({"gemm_vbatched": gemm_leg()}), flush=True)
         return
     op = build_operator()
     xd = torch.from_numpy(op["x"]).cuda()
